@@ -1,0 +1,122 @@
+/*
+ * sepgpu.h -- MI355X extension of the seplib C99 API.
+ *
+ * Nothing here has a counterpart in the reference; the reference API in
+ * sepinit.h ... sepcoulomb.h is served unchanged by libsep.so and runs its
+ * hot path on the GPU.  This header adds what a device library needs on
+ * top of that API:
+ *
+ *   - error reporting (the reference's functions are void and its
+ *     SEP_SUCCESS == SEP_FAILURE, so failures are reported here);
+ *   - the precision mode of the pair kernels;
+ *   - device residency: with sep_gpu_resident(atoms, sys, 1) the reference
+ *     API calls on those atoms work on device state only and return without
+ *     synchronising; sep_gpu_sync() brings atoms, sys and every sepret /
+ *     Nose-Hoover alpha used since then back to the host;
+ *   - a fused multi-step run of a recorded call sequence, replayed from a
+ *     HIP graph;
+ *   - diagnostics used by tests and benchmarks (neighbour pairs, timings).
+ *
+ * Every function returns SEP_GPU_OK (0) or an error code and never exits.
+ */
+#ifndef SEP_MI355X_SEPGPU_H
+#define SEP_MI355X_SEPGPU_H
+
+#include <stddef.h>
+#include "sepstrct.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- errors ---------------------------------------------------------- */
+#define SEP_GPU_OK 0
+#define SEP_GPU_EDEVICE 1      /* HIP runtime failure (message has details)   */
+#define SEP_GPU_EUNSUPPORTED 2 /* potential / option not on the device path   */
+#define SEP_GPU_ENEIGHB 3      /* per-atom neighbour capacity exceeded        */
+#define SEP_GPU_ECELL 4        /* atom maps outside the cell grid (x>=L, NaN) */
+#define SEP_GPU_EARG 5         /* invalid argument                            */
+#define SEP_GPU_EHOST 6        /* sep_error() was called                      */
+#define SEP_GPU_ESTATE 7       /* call not valid in the current mode          */
+
+/* Code of the most recent failure since the last clear (0 if none). */
+int sep_gpu_last_error(void);
+/* Text of the most recent failure ("" if none). */
+const char *sep_gpu_last_error_message(void);
+void sep_gpu_clear_error(void);
+
+/* ---- precision ------------------------------------------------------- */
+#define SEP_GPU_FP64 0  /* all pair arithmetic in FP64 (parity mode)          */
+#define SEP_GPU_MIXED 1 /* FP32 pair arithmetic on FP64-formed dx, FP64 sums,
+                           FP64 positions and integrator                      */
+int sep_gpu_set_precision(seppart *atoms, sepsys *sys, int mode);
+
+/* ---- residency ------------------------------------------------------- */
+/* on = 1: upload atoms/sys and keep the device copy authoritative; on = 0:
+ * synchronise back to the host and return to per-call synchronisation. */
+int sep_gpu_resident(seppart *atoms, sepsys *sys, int on);
+/* Copy device state of a resident system back: x, v, f, a, xn, crossings,
+ * cross_neighb, sys->{neighb_flag, nupdate_neighb, max_dist2}, and every
+ * sepret and alpha that device calls have used. */
+int sep_gpu_sync(seppart *atoms, sepsys *sys);
+/* Push host edits of a resident system (positions, velocities, masses,
+ * charges, types, box) to the device. */
+int sep_gpu_upload(seppart *atoms, sepsys *sys);
+
+/* ---- fused run ------------------------------------------------------- */
+/* One entry of a per-step call sequence; the fields used depend on op and
+ * mirror the arguments of the reference function named in the comment. */
+enum {
+	SEP_GPU_OP_PAIRS = 1,     /* sep_force_pairs(types, cf=p[0], fun, opt)          */
+	SEP_GPU_OP_LJ = 2,        /* sep_force_lj(types, p[0..3], opt)                  */
+	SEP_GPU_OP_COULOMB = 3,   /* sep_coulomb_sf(cf=p[0], opt)                        */
+	SEP_GPU_OP_BOND = 4,      /* sep_stretch_harmonic(type, lbond=p[0], ks=p[1])    */
+	SEP_GPU_OP_ANGLE = 5,     /* sep_angle_harmonic(type, angle0=p[0], k=p[1])      */
+	SEP_GPU_OP_COSSQ = 6,     /* sep_angle_cossq(type, angle0=p[0], k=p[1])         */
+	SEP_GPU_OP_TORSION = 7,   /* sep_torsion_Ryckaert(type, g=p[0..5])              */
+	SEP_GPU_OP_NOSEHOOVER = 8,/* sep_nosehoover(Td=p[0], alpha, Q=p[1])             */
+	SEP_GPU_OP_LEAPFROG = 9   /* sep_leapfrog()                                      */
+};
+typedef struct {
+	int op;
+	char types[2];
+	int type;
+	unsigned opt;
+	double p[6];
+	double (*fun)(double, char);
+	double *alpha;
+} sep_gpu_op;
+
+#define SEP_GPU_RUN_GRAPH 1 /* replay the step from a captured HIP graph */
+
+/* Runs nsteps of { sep_reset_retval(ret); sep_reset_force(); ops[0..nops) }
+ * on a resident system without host synchronisation; ret (and each alpha)
+ * hold the last step's values after sep_gpu_sync. */
+int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret,
+		const sep_gpu_op *ops, int nops, long nsteps, int flags);
+
+/* ---- diagnostics ----------------------------------------------------- */
+/* Unordered neighbour pairs (i < j, original atom indices) of the current
+ * device list, written as pairs[2k], pairs[2k+1]; *npairs receives the
+ * count (also when cap is too small, then nothing is written). */
+int sep_gpu_neighbour_pairs(seppart *atoms, sepsys *sys, int *pairs,
+			    long cap, long *npairs);
+
+/* Kernel timing with HIP events on the library's stream.  kernel names:
+ * "pair" (the dominant pair-force kernel), "step" (all device work of one
+ * step).  sep_gpu_timing_reset starts a new window. */
+int sep_gpu_timing_enable(seppart *atoms, int on);
+int sep_gpu_timing_reset(seppart *atoms);
+int sep_gpu_timing_get(seppart *atoms, const char *kernel, double *total_ms,
+		       long *count);
+
+/* Mean full-list length of the current device list (entries per atom). */
+int sep_gpu_list_stats(seppart *atoms, double *mean_full, int *max_full);
+
+/* Device synchronisation barrier for timing. */
+int sep_gpu_synchronize(seppart *atoms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,59 @@
+/*
+ * sepmisc.h -- potentials, wrap macro and small setters (seplib C99 API).
+ */
+#ifndef SEP_MI355X_SEPMISC_H
+#define SEP_MI355X_SEPMISC_H
+
+#include "sepstrct.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Square and single-image minimum-image wrap, as the reference's macros
+ * (include/sepmisc.h:69, 81-85): at most one box length is added or
+ * subtracted, with strict comparisons against +-L/2. */
+#define sep_Sq(x) ((x) * (x))
+#define sep_Wrap(x, y)                          \
+	{                                       \
+		if ((x) > 0.5 * (y)) (x) -= (y);        \
+		else if ((x) < -0.5 * (y)) (x) += (y);  \
+	}
+
+/* Plain LJ (eps = sigma = 1): 'f' returns 48 r^-14 - 24 r^-8 (force / r),
+ * 'u' returns 4 (r^-12 - r^-6).  Replaces include/sepmisc.h:115
+ * (source/sepmisc.c:115-130).  Its address selects the device kernel in
+ * sep_force_pairs. */
+double sep_lj(double r2, char opt);
+
+/* LJ shifted by SEP_LJCF2 (zero energy at rc = 2.5).  Replaces
+ * include/sepmisc.h:122 (source/sepmisc.c:132-147). */
+double sep_lj_shift(double r2, char opt);
+
+/* WCA: LJ shifted by +1 (use with rc = 2^(1/6)).  Replaces
+ * include/sepmisc.h:129 (source/sepmisc.c:149-164). */
+double sep_wca(double r2, char opt);
+
+/* f = 0 for every atom and sys->max_dist2 = 0.  Replaces
+ * include/sepmisc.h:245 (source/sepmisc.c:393-400). */
+void sep_reset_force(seppart *ptr, sepsys *sys);
+
+/* Sets sys->omp_flag and sys->nthreads.  The device path ignores the thread
+ * count.  Replaces include/sepmisc.h:303 (source/sepmisc.c:1116-1123). */
+void sep_set_omp(unsigned nthreads, sepsys *sys);
+
+/* Sets the list skin; like the reference it does not recompute the cell
+ * grid.  Replaces include/sepmisc.h:311 (source/sepmisc.c:1125-1129). */
+void sep_set_skin(sepsys *sys, double value);
+
+/* Diagnostics printed as "sep-error -> ..." / "sep-warning -> ..." on
+ * stdout.  sep_error does not exit (the reference's -DOCTAVE behaviour,
+ * source/sepmisc.c:59-65); it records SEP_GPU_EHOST for sep_gpu_last_error.
+ * Replaces include/sepmisc.h:101,108. */
+void sep_error(char *str, ...);
+void sep_warning(char *str, ...);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

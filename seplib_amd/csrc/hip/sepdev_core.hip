@@ -1,0 +1,679 @@
+// sepdev_core.hip -- device state lifetime, host<->device transfers in the
+// caller's atom order, return/scalar slots, timing and graph capture.
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sepdev_impl.h"
+
+// ---- errors ---------------------------------------------------------------
+
+static char g_msg[512];
+static int g_code;
+
+int sepk_fail(int code, const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_msg, sizeof g_msg, fmt, ap);
+	va_end(ap);
+	g_code = code;
+	return code;
+}
+
+extern "C" const char *sepdev_error_message(void) { return g_msg; }
+extern "C" int sepdev_last_code(void) { return g_code; }
+extern "C" void sepdev_clear_error(void)
+{
+	g_code = 0;
+	g_msg[0] = '\0';
+}
+
+// ---- allocation -------------------------------------------------------------
+
+template <class T>
+static int dalloc(T **p, size_t count)
+{
+	*p = nullptr;
+	if (count == 0)
+		count = 1;
+	SEPK_CHECK(hipMalloc((void **)p, count * sizeof(T)));
+	return 0;
+}
+
+static int alloc_state(sepk_state *s, long npad)
+{
+	memset(s, 0, sizeof *s);
+	int rc = 0;
+	rc |= dalloc(&s->pos, npad);
+	rc |= dalloc(&s->v, 3 * npad);
+	rc |= dalloc(&s->f, 3 * npad);
+	rc |= dalloc(&s->a, 3 * npad);
+	rc |= dalloc(&s->xn, 3 * npad);
+	rc |= dalloc(&s->cn, 3 * npad);
+	rc |= dalloc(&s->cr, 3 * npad);
+	rc |= dalloc(&s->m, npad);
+	rc |= dalloc(&s->type, npad);
+	rc |= dalloc(&s->mol, npad);
+	rc |= dalloc(&s->orig, npad);
+	return rc;
+}
+
+static void free_state(sepk_state *s)
+{
+	void *ptrs[] = {s->pos, s->v, s->f, s->a, s->xn, s->cn, s->cr,
+			s->m, s->type, s->mol, s->orig, s->partners};
+	for (void *p : ptrs)
+		if (p)
+			(void)hipFree(p);
+	memset(s, 0, sizeof *s);
+}
+
+__global__ void k_init_identity(sepk_state s, int *iperm, long n, long npad)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= npad)
+		return;
+	s.orig[t] = t < n ? (int)t : -1;
+	if (t < n)
+		iperm[t] = (int)t;
+	s.pos[t] = sepk_pos{0.0, 0.0, 0.0, 0.0};
+	s.m[t] = 1.0;
+	s.type[t] = 'A';
+	s.mol[t] = -1;
+	for (int c = 0; c < 3; c++) {
+		s.v[c * npad + t] = 0.0;
+		s.f[c * npad + t] = 0.0;
+		s.a[c * npad + t] = 0.0;
+		s.xn[c * npad + t] = 0.0;
+		s.cn[c * npad + t] = 0;
+		s.cr[c * npad + t] = 0;
+	}
+}
+
+extern "C" sepdev *sepdev_create(long npart, int *err)
+{
+	*err = 0;
+	if (npart <= 0) {
+		*err = sepk_fail(SEP_GPU_EARG, "sepdev_create: npart = %ld", npart);
+		return nullptr;
+	}
+	sepdev *d = (sepdev *)calloc(1, sizeof(sepdev));
+	d->n = npart;
+	d->npad = (npart + 63) / 64 * 64;
+	int rc = 0;
+	if (hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking) != hipSuccess) {
+		*err = sepk_fail(SEP_GPU_EDEVICE, "hipStreamCreate failed");
+		free(d);
+		return nullptr;
+	}
+	rc |= alloc_state(&d->s[0], d->npad);
+	rc |= alloc_state(&d->s[1], d->npad);
+	rc |= dalloc(&d->iperm, d->npad);
+	rc |= dalloc(&d->cellid_old, d->npad);
+	rc |= dalloc(&d->cellid_new, d->npad);
+	rc |= dalloc(&d->order_tmp, d->npad);
+	rc |= dalloc(&d->neworder, d->npad);
+	rc |= dalloc(&d->nbr_count, d->npad);
+	d->partials_words = (long)sepk_blocks(d->npad) * 16 + 64;
+	rc |= dalloc(&d->partials, d->partials_words);
+	rc |= dalloc(&d->counters, SEPK_NCTR);
+	rc |= dalloc(&d->sys, 1);
+	d->ret_cap = 16;
+	rc |= dalloc(&d->ret, (size_t)d->ret_cap * SEPDEV_RET_WORDS);
+	d->scal_cap = 16;
+	rc |= dalloc(&d->scal, d->scal_cap);
+	d->stage_bytes = (size_t)d->npad * 40 * sizeof(int);
+	if (d->stage_bytes < (size_t)d->npad * 3 * sizeof(double))
+		d->stage_bytes = (size_t)d->npad * 3 * sizeof(double);
+	if (hipHostMalloc(&d->hstage, d->stage_bytes, hipHostMallocDefault) != hipSuccess)
+		rc |= sepk_fail(SEP_GPU_EDEVICE, "hipHostMalloc staging failed");
+	rc |= dalloc((char **)&d->dstage, d->stage_bytes);
+	if (rc) {
+		*err = rc;
+		sepdev_destroy(d);
+		return nullptr;
+	}
+	// list lengths start at zero so that a pair kernel that runs before any
+	// build reads no list entries
+	(void)hipMemsetAsync(d->nbr_count, 0, d->npad * sizeof(int), d->st);
+	(void)hipMemsetAsync(d->counters, 0, SEPK_NCTR * sizeof(unsigned), d->st);
+	(void)hipMemsetAsync(d->sys, 0, sizeof(sepdev_sys), d->st);
+	(void)hipMemsetAsync(d->ret, 0, (size_t)d->ret_cap * SEPDEV_RET_WORDS * sizeof(double), d->st);
+	for (int b = 0; b < 2; b++) {
+		k_init_identity<<<sepk_blocks(d->npad), SEPK_BLOCK, 0, d->st>>>(d->s[b], d->iperm, d->n, d->npad);
+	}
+	if (hipStreamSynchronize(d->st) != hipSuccess) {
+		*err = sepk_fail(SEP_GPU_EDEVICE, "device init failed");
+		sepdev_destroy(d);
+		return nullptr;
+	}
+	return d;
+}
+
+extern "C" void sepdev_destroy(sepdev *d)
+{
+	if (!d)
+		return;
+	if (d->st)
+		(void)hipStreamSynchronize(d->st);
+	sepdev_graph_drop(d);
+	free_state(&d->s[0]);
+	free_state(&d->s[1]);
+	void *ptrs[] = {d->iperm, d->cell_count, d->cell_start, d->cell_fill,
+			d->cellid_old, d->cellid_new, d->order_tmp, d->neworder,
+			d->nbr, d->nbr_count, d->partials, d->counters, d->ret,
+			d->scal, d->sys, d->dstage, d->blist, d->alist, d->dlist,
+			d->bcontrib, d->acontrib, d->dcontrib, d->bcsr_off,
+			d->bcsr_idx, d->acsr_off, d->acsr_idx, d->dcsr_off,
+			d->dcsr_idx, d->blen, d->aval, d->dval};
+	for (void *p : ptrs)
+		if (p)
+			(void)hipFree(p);
+	if (d->hstage)
+		(void)hipHostFree(d->hstage);
+	if (d->timing) {
+		for (int i = 0; i < 64; i++) {
+			(void)hipEventDestroy(d->t_pair.beg[i]);
+			(void)hipEventDestroy(d->t_pair.end[i]);
+		}
+	}
+	if (d->st)
+		(void)hipStreamDestroy(d->st);
+	free(d);
+}
+
+// ---- transfers ----------------------------------------------------------------
+
+// dst block (3 x npad, device order) <- src (3n interleaved, original order)
+__global__ void k_scatter_vec(double *dst, const double *src, const int *iperm,
+			      long n, long npad)
+{
+	long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= n)
+		return;
+	const long t = iperm[o];
+	dst[t] = src[3 * o];
+	dst[npad + t] = src[3 * o + 1];
+	dst[2 * npad + t] = src[3 * o + 2];
+}
+
+__global__ void k_scatter_pos(sepk_pos *pos, const double *src, const int *iperm,
+			      long n)
+{
+	long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= n)
+		return;
+	const long t = iperm[o];
+	sepk_pos p = pos[t];
+	p.x = src[3 * o];
+	p.y = src[3 * o + 1];
+	p.z = src[3 * o + 2];
+	pos[t] = p;
+}
+
+__global__ void k_gather_vec(double *dst, const double *src, const int *iperm,
+			     long n, long npad)
+{
+	long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= n)
+		return;
+	const long t = iperm[o];
+	dst[3 * o] = src[t];
+	dst[3 * o + 1] = src[npad + t];
+	dst[3 * o + 2] = src[2 * npad + t];
+}
+
+__global__ void k_gather_pos(double *dst, const sepk_pos *pos, const int *iperm,
+			     long n)
+{
+	long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= n)
+		return;
+	const sepk_pos p = pos[iperm[o]];
+	dst[3 * o] = p.x;
+	dst[3 * o + 1] = p.y;
+	dst[3 * o + 2] = p.z;
+}
+
+__global__ void k_scatter_ivec(int *dst, const int *src, const int *iperm,
+			       long n, long npad)
+{
+	long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= n)
+		return;
+	const long t = iperm[o];
+	dst[t] = src[3 * o];
+	dst[npad + t] = src[3 * o + 1];
+	dst[2 * npad + t] = src[3 * o + 2];
+}
+
+__global__ void k_gather_ivec(int *dst, const int *src, const int *iperm,
+			      long n, long npad)
+{
+	long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= n)
+		return;
+	const long t = iperm[o];
+	dst[3 * o] = src[t];
+	dst[3 * o + 1] = src[npad + t];
+	dst[3 * o + 2] = src[2 * npad + t];
+}
+
+__global__ void k_zero_blocks(double *p, long count)
+{
+	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < count)
+		p[t] = 0.0;
+}
+
+static int stage_in(sepdev *d, const void *src, size_t bytes)
+{
+	if (bytes > d->stage_bytes)
+		return sepk_fail(SEP_GPU_EARG, "staging overflow (%zu bytes)", bytes);
+	// the pinned buffer may still feed an earlier copy
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	memcpy(d->hstage, src, bytes);
+	SEPK_CHECK(hipMemcpyAsync(d->dstage, d->hstage, bytes, hipMemcpyHostToDevice, d->st));
+	return 0;
+}
+
+static int stage_out(sepdev *d, void *dst, size_t bytes)
+{
+	SEPK_CHECK(hipMemcpyAsync(d->hstage, d->dstage, bytes, hipMemcpyDeviceToHost, d->st));
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	memcpy(dst, d->hstage, bytes);
+	return 0;
+}
+
+static double *vec_field(sepdev *d, int which)
+{
+	sepk_state &s = d->s[0];
+	switch (which) {
+	case SEPDEV_V: return s.v;
+	case SEPDEV_F: return s.f;
+	case SEPDEV_A: return s.a;
+	case SEPDEV_XN: return s.xn;
+	}
+	return nullptr;
+}
+
+extern "C" int sepdev_put_vec(sepdev *d, int which, const double *src3n)
+{
+	if (d->capturing)
+		return sepk_fail(SEP_GPU_ESTATE, "transfer during graph capture");
+	int rc = stage_in(d, src3n, (size_t)d->n * 3 * sizeof(double));
+	if (rc)
+		return rc;
+	const unsigned nb = sepk_blocks(d->n);
+	if (which == SEPDEV_X) {
+		k_scatter_pos<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0].pos, (const double *)d->dstage, d->iperm, d->n);
+	} else {
+		double *dst = vec_field(d, which);
+		if (!dst)
+			return sepk_fail(SEP_GPU_EARG, "put_vec: field %d", which);
+		k_scatter_vec<<<nb, SEPK_BLOCK, 0, d->st>>>(dst, (const double *)d->dstage, d->iperm, d->n, d->npad);
+		if (which == SEPDEV_F)
+			d->f_zero_pending = 0;
+	}
+	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+extern "C" int sepdev_get_vec(sepdev *d, int which, double *dst3n)
+{
+	if (d->capturing)
+		return sepk_fail(SEP_GPU_ESTATE, "transfer during graph capture");
+	const unsigned nb = sepk_blocks(d->n);
+	if (which == SEPDEV_F) {
+		int rc = sepk_materialize_forces(d);
+		if (rc)
+			return rc;
+	}
+	if (which == SEPDEV_X) {
+		k_gather_pos<<<nb, SEPK_BLOCK, 0, d->st>>>((double *)d->dstage, d->s[0].pos, d->iperm, d->n);
+	} else {
+		double *src = vec_field(d, which);
+		if (!src)
+			return sepk_fail(SEP_GPU_EARG, "get_vec: field %d", which);
+		k_gather_vec<<<nb, SEPK_BLOCK, 0, d->st>>>((double *)d->dstage, src, d->iperm, d->n, d->npad);
+	}
+	SEPK_LAUNCH_CHECK();
+	return stage_out(d, dst3n, (size_t)d->n * 3 * sizeof(double));
+}
+
+extern "C" int sepdev_put_ivec(sepdev *d, int which, const int *src3n)
+{
+	int rc = stage_in(d, src3n, (size_t)d->n * 3 * sizeof(int));
+	if (rc)
+		return rc;
+	int *dst = which == SEPDEV_CROSS_NB ? d->s[0].cn : d->s[0].cr;
+	k_scatter_ivec<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(dst, (const int *)d->dstage, d->iperm, d->n, d->npad);
+	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+extern "C" int sepdev_get_ivec(sepdev *d, int which, int *dst3n)
+{
+	const int *src = which == SEPDEV_CROSS_NB ? d->s[0].cn : d->s[0].cr;
+	k_gather_ivec<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>((int *)d->dstage, src, d->iperm, d->n, d->npad);
+	SEPK_LAUNCH_CHECK();
+	return stage_out(d, dst3n, (size_t)d->n * 3 * sizeof(int));
+}
+
+__global__ void k_scatter_scalars(sepk_state s, const double *mass, const double *q,
+				  const signed char *type, const int *mol,
+				  const int *iperm, long n)
+{
+	long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= n)
+		return;
+	const long t = iperm[o];
+	if (mass)
+		s.m[t] = mass[o];
+	if (q)
+		s.pos[t].q = q[o];
+	if (type)
+		s.type[t] = type[o];
+	if (mol)
+		s.mol[t] = mol[o];
+}
+
+extern "C" int sepdev_put_scalars(sepdev *d, const double *mass, const double *charge,
+				  const signed char *type, const int *molindex)
+{
+	// pack into one staging upload: mass | charge | mol | type
+	const long n = d->n;
+	const size_t bytes = (size_t)n * (2 * sizeof(double) + sizeof(int) + 1);
+	if (bytes > d->stage_bytes)
+		return sepk_fail(SEP_GPU_EARG, "staging overflow");
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	char *h = (char *)d->hstage;
+	double *hm = (double *)h;
+	double *hq = hm + n;
+	int *hmol = (int *)(hq + n);
+	signed char *ht = (signed char *)(hmol + n);
+	if (mass) memcpy(hm, mass, n * sizeof(double));
+	if (charge) memcpy(hq, charge, n * sizeof(double));
+	if (molindex) memcpy(hmol, molindex, n * sizeof(int));
+	if (type) memcpy(ht, type, n);
+	SEPK_CHECK(hipMemcpyAsync(d->dstage, d->hstage, bytes, hipMemcpyHostToDevice, d->st));
+	char *g = (char *)d->dstage;
+	double *gm = (double *)g;
+	double *gq = gm + n;
+	int *gmol = (int *)(gq + n);
+	signed char *gt = (signed char *)(gmol + n);
+	k_scatter_scalars<<<sepk_blocks(n), SEPK_BLOCK, 0, d->st>>>(
+		d->s[0], mass ? gm : nullptr, charge ? gq : nullptr,
+		type ? gt : nullptr, molindex ? gmol : nullptr, d->iperm, n);
+	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+extern "C" int sepdev_put_charges(sepdev *d, const double *charge)
+{
+	return sepdev_put_scalars(d, nullptr, charge, nullptr, nullptr);
+}
+
+extern "C" int sepdev_put_types(sepdev *d, const signed char *type)
+{
+	return sepdev_put_scalars(d, nullptr, nullptr, type, nullptr);
+}
+
+__global__ void k_scatter_partners(int *dst, const int *src, const int *iperm,
+				   long n, long npad)
+{
+	long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= n)
+		return;
+	const long t = iperm[o];
+	for (int k = 0; k < 40; k++)
+		dst[k * npad + t] = src[40 * o + k];
+}
+
+extern "C" int sepdev_put_partners(sepdev *d, const int *partners40n)
+{
+	for (int b = 0; b < 2; b++)
+		if (!d->s[b].partners) {
+			int rc = dalloc(&d->s[b].partners, (size_t)40 * d->npad);
+			if (rc)
+				return rc;
+		}
+	int rc = stage_in(d, partners40n, (size_t)d->n * 40 * sizeof(int));
+	if (rc)
+		return rc;
+	k_scatter_partners<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(
+		d->s[0].partners, (const int *)d->dstage, d->iperm, d->n, d->npad);
+	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+extern "C" int sepdev_put_sys(sepdev *d, const sepdev_sys *s)
+{
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	memcpy(d->hstage, s, sizeof *s);
+	SEPK_CHECK(hipMemcpyAsync(d->sys, d->hstage, sizeof *s, hipMemcpyHostToDevice, d->st));
+	return 0;
+}
+
+extern "C" int sepdev_get_sys(sepdev *d, sepdev_sys *s)
+{
+	SEPK_CHECK(hipMemcpyAsync(d->hstage, d->sys, sizeof *s, hipMemcpyDeviceToHost, d->st));
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	memcpy(s, d->hstage, sizeof *s);
+	return 0;
+}
+
+// ---- slots --------------------------------------------------------------------
+
+extern "C" int sepdev_ret_alloc(sepdev *d)
+{
+	if (d->nret == d->ret_cap) {
+		double *nr;
+		int rc = dalloc(&nr, (size_t)2 * d->ret_cap * SEPDEV_RET_WORDS);
+		if (rc)
+			return -1;
+		SEPK_CHECK(hipMemsetAsync(nr, 0, (size_t)2 * d->ret_cap * SEPDEV_RET_WORDS * sizeof(double), d->st));
+		SEPK_CHECK(hipMemcpyAsync(nr, d->ret, (size_t)d->ret_cap * SEPDEV_RET_WORDS * sizeof(double), hipMemcpyDeviceToDevice, d->st));
+		SEPK_CHECK(hipStreamSynchronize(d->st));
+		(void)hipFree(d->ret);
+		d->ret = nr;
+		d->ret_cap *= 2;
+	}
+	return d->nret++;
+}
+
+extern "C" int sepdev_ret_zero(sepdev *d, int slot)
+{
+	if (slot < 0 || slot >= d->nret)
+		return sepk_fail(SEP_GPU_EARG, "ret slot %d", slot);
+	// a fill kernel rather than a memset node, so that it can sit in a graph
+	k_zero_blocks<<<1, 64, 0, d->st>>>(d->ret + (size_t)slot * SEPDEV_RET_WORDS, SEPDEV_RET_WORDS);
+	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+extern "C" int sepdev_ret_get(sepdev *d, int slot, double *words)
+{
+	if (slot < 0 || slot >= d->nret)
+		return sepk_fail(SEP_GPU_EARG, "ret slot %d", slot);
+	SEPK_CHECK(hipMemcpyAsync(d->hstage, d->ret + (size_t)slot * SEPDEV_RET_WORDS,
+				  SEPDEV_RET_WORDS * sizeof(double), hipMemcpyDeviceToHost, d->st));
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	memcpy(words, d->hstage, SEPDEV_RET_WORDS * sizeof(double));
+	return 0;
+}
+
+extern "C" int sepdev_ret_put(sepdev *d, int slot, const double *words)
+{
+	if (slot < 0 || slot >= d->nret)
+		return sepk_fail(SEP_GPU_EARG, "ret slot %d", slot);
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	memcpy(d->hstage, words, SEPDEV_RET_WORDS * sizeof(double));
+	SEPK_CHECK(hipMemcpyAsync(d->ret + (size_t)slot * SEPDEV_RET_WORDS, d->hstage,
+				  SEPDEV_RET_WORDS * sizeof(double), hipMemcpyHostToDevice, d->st));
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	return 0;
+}
+
+extern "C" int sepdev_scalar_alloc(sepdev *d, double init)
+{
+	if (d->nscal == d->scal_cap) {
+		double *ns;
+		if (dalloc(&ns, (size_t)2 * d->scal_cap))
+			return -1;
+		SEPK_CHECK(hipMemcpyAsync(ns, d->scal, (size_t)d->scal_cap * sizeof(double), hipMemcpyDeviceToDevice, d->st));
+		SEPK_CHECK(hipStreamSynchronize(d->st));
+		(void)hipFree(d->scal);
+		d->scal = ns;
+		d->scal_cap *= 2;
+	}
+	int slot = d->nscal++;
+	if (sepdev_scalar_put(d, slot, init))
+		return -1;
+	return slot;
+}
+
+extern "C" int sepdev_scalar_get(sepdev *d, int slot, double *v)
+{
+	SEPK_CHECK(hipMemcpyAsync(d->hstage, d->scal + slot, sizeof(double), hipMemcpyDeviceToHost, d->st));
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	memcpy(v, d->hstage, sizeof(double));
+	return 0;
+}
+
+extern "C" int sepdev_scalar_put(sepdev *d, int slot, double v)
+{
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	memcpy(d->hstage, &v, sizeof(double));
+	SEPK_CHECK(hipMemcpyAsync(d->scal + slot, d->hstage, sizeof(double), hipMemcpyHostToDevice, d->st));
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	return 0;
+}
+
+// ---- misc -----------------------------------------------------------------------
+
+extern "C" int sepdev_synchronize(sepdev *d)
+{
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	return 0;
+}
+
+extern "C" int sepdev_timing(sepdev *d, int on)
+{
+	if (on && !d->timing) {
+		for (int i = 0; i < 64; i++) {
+			SEPK_CHECK(hipEventCreate(&d->t_pair.beg[i]));
+			SEPK_CHECK(hipEventCreate(&d->t_pair.end[i]));
+		}
+		d->t_pair.used = 0;
+		d->t_pair.ms = 0.0;
+		d->t_pair.count = 0;
+		d->timing = 1;
+	} else if (!on && d->timing) {
+		SEPK_CHECK(hipStreamSynchronize(d->st));
+		for (int i = 0; i < 64; i++) {
+			(void)hipEventDestroy(d->t_pair.beg[i]);
+			(void)hipEventDestroy(d->t_pair.end[i]);
+		}
+		d->timing = 0;
+	}
+	return 0;
+}
+
+// Folds recorded event pairs into the running total.
+static int timer_flush(sepk_timer *t)
+{
+	for (int i = 0; i < t->used; i++) {
+		SEPK_CHECK(hipEventSynchronize(t->end[i]));
+		float ms = 0.0f;
+		SEPK_CHECK(hipEventElapsedTime(&ms, t->beg[i], t->end[i]));
+		t->ms += ms;
+		t->count++;
+	}
+	t->used = 0;
+	return 0;
+}
+
+int sepk_timer_begin(sepdev *d, sepk_timer *t)
+{
+	if (!d->timing || d->capturing)
+		return 0;
+	if (t->used == 64) {
+		int rc = timer_flush(t);
+		if (rc)
+			return rc;
+	}
+	SEPK_CHECK(hipEventRecord(t->beg[t->used], d->st));
+	return 0;
+}
+
+int sepk_timer_end(sepdev *d, sepk_timer *t)
+{
+	if (!d->timing || d->capturing)
+		return 0;
+	SEPK_CHECK(hipEventRecord(t->end[t->used], d->st));
+	t->used++;
+	return 0;
+}
+
+extern "C" int sepdev_timing_reset(sepdev *d)
+{
+	if (!d->timing)
+		return 0;
+	int rc = timer_flush(&d->t_pair);
+	d->t_pair.ms = 0.0;
+	d->t_pair.count = 0;
+	return rc;
+}
+
+extern "C" int sepdev_timing_get(sepdev *d, const char *name, double *ms, long *count)
+{
+	if (!d->timing)
+		return sepk_fail(SEP_GPU_ESTATE, "timing not enabled");
+	if (strcmp(name, "pair") != 0)
+		return sepk_fail(SEP_GPU_EARG, "unknown timer '%s'", name);
+	int rc = timer_flush(&d->t_pair);
+	*ms = d->t_pair.ms;
+	*count = d->t_pair.count;
+	return rc;
+}
+
+extern "C" int sepdev_graph_begin(sepdev *d)
+{
+	sepdev_graph_drop(d);
+	SEPK_CHECK(hipStreamBeginCapture(d->st, hipStreamCaptureModeRelaxed));
+	d->capturing = 1;
+	return 0;
+}
+
+extern "C" int sepdev_graph_end(sepdev *d)
+{
+	d->capturing = 0;
+	SEPK_CHECK(hipStreamEndCapture(d->st, &d->graph));
+	SEPK_CHECK(hipGraphInstantiate(&d->exec, d->graph, nullptr, nullptr, 0));
+	return 0;
+}
+
+extern "C" int sepdev_graph_launch(sepdev *d, long times)
+{
+	if (!d->exec)
+		return sepk_fail(SEP_GPU_ESTATE, "no graph");
+	for (long i = 0; i < times; i++)
+		SEPK_CHECK(hipGraphLaunch(d->exec, d->st));
+	return 0;
+}
+
+extern "C" int sepdev_graph_drop(sepdev *d)
+{
+	if (d->exec) {
+		(void)hipGraphExecDestroy(d->exec);
+		d->exec = nullptr;
+	}
+	if (d->graph) {
+		(void)hipGraphDestroy(d->graph);
+		d->graph = nullptr;
+	}
+	return 0;
+}

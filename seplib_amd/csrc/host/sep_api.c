@@ -1,0 +1,295 @@
+/*
+ * sep_api.c -- allocation, system setup, return values, potentials and
+ * diagnostics of the seplib C99 API (host side; no device work).
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sephost.h"
+
+int seph_each_ctx(int (*fn)(seph_ctx *, void *), void *arg);
+
+/* ---- diagnostics: include/sepmisc.h:101,108 --------------------------- */
+
+/* Expands the %d %i %f %c %s subset the reference's printer handles. */
+static void print_tagged(const char *tag, const char *str, va_list ap)
+{
+	printf("%s", tag);
+	for (const char *p = str; *p; p++) {
+		if (*p != '%') {
+			putchar(*p);
+			continue;
+		}
+		switch (*++p) {
+		case 'd':
+		case 'i': printf("%d", va_arg(ap, int)); break;
+		case 'f': printf("%f", va_arg(ap, double)); break;
+		case 'c': putchar(va_arg(ap, int)); break;
+		case 's': fputs(va_arg(ap, char *), stdout); break;
+		case '\0': p--; break;
+		default: putchar(*p); break;
+		}
+	}
+	printf("\n");
+	fflush(stdout);
+}
+
+void sep_error(char *str, ...)
+{
+	va_list ap;
+	va_start(ap, str);
+	print_tagged("sep-error -> ", str, ap);
+	va_end(ap);
+	seph_fail(SEP_GPU_EHOST, "sep_error: %s", str);
+}
+
+void sep_warning(char *str, ...)
+{
+	va_list ap;
+	va_start(ap, str);
+	print_tagged("sep-warning -> ", str, ap);
+	va_end(ap);
+}
+
+/* ---- allocation: source/sepinit.c:16-65 ------------------------------- */
+
+seppart *sep_init(size_t npart, size_t nneighb)
+{
+	seppart *p = calloc(npart ? npart : 1, sizeof(seppart));
+	if (!p) {
+		sep_error("%s: Couldn't allocate memory", __func__);
+		return NULL;
+	}
+	for (size_t i = 0; i < npart; i++) {
+		p[i].neighb = malloc((nneighb ? nneighb : 1) * sizeof(int));
+		if (!p[i].neighb) {
+			sep_error("%s: Couldn't allocate memory", __func__);
+			return NULL;
+		}
+		if (nneighb)
+			p[i].neighb[0] = -1;
+		p[i].type = 'A';
+		p[i].m = 1.0;
+		p[i].molindex = -1;
+		p[i].ldiff = 1.0;
+		for (int k = 0; k < SEP_BOND; k++) p[i].bond[k] = -1;
+		for (int k = 0; k < SEP_ANGLE; k++) p[i].angle[k] = -1;
+		for (int k = 0; k < SEP_DIHED; k++) p[i].dihed[k] = -1;
+	}
+	seph_register(p, (long)npart, (int)nneighb);
+	return p;
+}
+
+void sep_close(seppart *ptr, size_t npart)
+{
+	if (!ptr)
+		return;
+	seph_unregister(ptr);
+	for (size_t i = 0; i < npart; i++)
+		free(ptr[i].neighb);
+	free(ptr);
+}
+
+/* Text format "N\nLx Ly Lz\n" then "type x y z vx vy vz m q" per atom
+ * (source/sepinit.c:69-109). */
+seppart *sep_init_xyz(double *lbox, int *npart, const char *file, char verbose)
+{
+	if (verbose == 'v')
+		fprintf(stdout, "Opening %s\n", file);
+	FILE *fin = fopen(file, "r");
+	if (!fin) {
+		sep_error("%s: Couldn't open file", __func__);
+		return NULL;
+	}
+	if (fscanf(fin, "%d", npart) != 1 || *npart <= 0) {
+		fclose(fin);
+		sep_error("%s: Error reading xyz file", __func__);
+		return NULL;
+	}
+	seppart *p = sep_init((size_t)*npart, SEP_NUM_NEIGHB);
+	if (!p) {
+		fclose(fin);
+		return NULL;
+	}
+	if (fscanf(fin, "%lf%lf%lf\n", &lbox[0], &lbox[1], &lbox[2]) != 3) {
+		fclose(fin);
+		sep_error("%s: Error reading xyz file", __func__);
+		return p;
+	}
+	for (int i = 0; i < *npart; i++) {
+		if (fscanf(fin, "%c%lf%lf%lf%lf%lf%lf%lf%lf\n", &p[i].type, &p[i].x[0],
+			   &p[i].x[1], &p[i].x[2], &p[i].v[0], &p[i].v[1], &p[i].v[2],
+			   &p[i].m, &p[i].z) != 9) {
+			sep_error("%s: Error reading xyz file", __func__);
+			break;
+		}
+	}
+	if (verbose == 'v') {
+		fprintf(stdout, "Closing file\n");
+		fprintf(stdout, "Number of particles: %d\n", *npart);
+		fflush(stdout);
+	}
+	fclose(fin);
+	return p;
+}
+
+/* ---- system: source/sepinit.c:246-315 --------------------------------- */
+
+sepsys sep_sys_setup(double lengthx, double lengthy, double lengthz,
+		     double maxsyscf, double dt, size_t npart, size_t update)
+{
+	sepsys s;
+	memset(&s, 0, sizeof s);
+	const double skin0 = 0.25;
+	const double L[3] = {lengthx, lengthy, lengthz};
+	const char axis[3] = {'x', 'y', 'z'};
+	for (int k = 0; k < 3; k++) {
+		s.length[k] = L[k];
+		s.nsubbox[k] = (int)(L[k] / (maxsyscf + skin0));
+		if (update > 1 && s.nsubbox[k] < 3)
+			printf("sep-warning -> %s: Number of subboxes in %c direction are less than three\n",
+			       __func__, axis[k]);
+		s.lsubbox[k] = L[k] / s.nsubbox[k];
+	}
+	s.volume = lengthx * lengthy * lengthz;
+	s.npart = (long)npart;
+	s.cf = maxsyscf;
+	s.dt = dt;
+	s.ndof = (unsigned)(3 * npart - 3);
+	s.tnow = 0.0;
+	s.nupdate_neighb = 0;
+	s.neighb_update = (unsigned)update;
+	s.neighb_flag = 1;
+	s.skin = skin0;
+	s.molptr = calloc(1, sizeof(sepmolinfo));
+	s.omp_flag = false;
+	s.fun_cstate = 0;
+	return s;
+}
+
+void sep_free_sys(sepsys *ptr)
+{
+	sepmolinfo *m = ptr->molptr;
+	if (!m)
+		return;
+	if (m->flag_bonds == 1) {
+		free(m->blist);
+		free(m->blengths);
+		m->blist = NULL;
+		m->blengths = NULL;
+	}
+	if (m->flag_angles == 1) {
+		free(m->alist);
+		free(m->angles);
+		m->alist = NULL;
+		m->angles = NULL;
+	}
+	if (m->flag_dihedrals == 1) {
+		free(m->dlist);
+		free(m->dihedrals);
+		m->dlist = NULL;
+		m->dihedrals = NULL;
+	}
+}
+
+void sep_set_omp(unsigned nthreads, sepsys *sys)
+{
+	sys->omp_flag = true;
+	sys->nthreads = nthreads;
+}
+
+void sep_set_skin(sepsys *sys, double value) { sys->skin = value; }
+
+/* ---- return values: source/sepret.c:19-82 ----------------------------- */
+
+static int zero_bound_ret(seph_ctx *c, void *arg)
+{
+	if (!c->resident)
+		return 0;
+	for (int b = 0; b < c->nrets; b++)
+		if (c->rets[b].host == arg)
+			return seph_dev_fail(sepdev_ret_zero(c->dev, c->rets[b].slot));
+	return 0;
+}
+
+void sep_reset_retval(sepret *r)
+{
+	r->epot = 0.0;
+	r->ecoul = 0.0;
+	r->ekin = 0.0;
+	r->sumv2 = 0.0;
+	memset(r->P, 0, sizeof r->P);
+	memset(r->kin_P, 0, sizeof r->kin_P);
+	memset(r->pot_P, 0, sizeof r->pot_P);
+	memset(r->P_mol, 0, sizeof r->P_mol);
+	memset(r->kin_P_mol, 0, sizeof r->kin_P_mol);
+	memset(r->pot_P_mol, 0, sizeof r->pot_P_mol);
+	memset(r->pot_P_conservative, 0, sizeof r->pot_P_conservative);
+	memset(r->pot_P_random, 0, sizeof r->pot_P_random);
+	memset(r->pot_P_dissipative, 0, sizeof r->pot_P_dissipative);
+	memset(r->pot_P_bond, 0, sizeof r->pot_P_bond);
+	memset(r->pot_T_mol, 0, sizeof r->pot_T_mol);
+	memset(r->kin_T_mol, 0, sizeof r->kin_T_mol);
+	memset(r->T_mol, 0, sizeof r->T_mol);
+	seph_each_ctx(zero_bound_ret, r);
+}
+
+void sep_pressure_tensor(sepret *r, sepsys *sys)
+{
+	const double ivol = 1.0 / sys->volume;
+	for (int a = 0; a < 3; a++)
+		for (int b = 0; b < 3; b++)
+			r->P[a][b] = (r->kin_P[a][b] + r->pot_P[a][b]) * ivol;
+	double tr = 0.0;
+	for (int a = 0; a < 3; a++)
+		tr += r->P[a][a];
+	r->p = tr / 3.0;
+}
+
+double sep_get_pressure(sepret *r, sepsys *sys)
+{
+	sep_pressure_tensor(r, sys);
+	return r->p;
+}
+
+double sep_get_temperature(sepret *r, sepsys *sys)
+{
+	return 2.0 / (3.0 * sys->ndof) * r->ekin;
+}
+
+/* ---- potentials: source/sepmisc.c:115-164 ----------------------------- */
+/* Host evaluations for callers; sep_force_pairs recognises these three by
+ * address and evaluates the same expressions on the device. */
+
+double sep_lj(double r2, char opt)
+{
+	const double inv = 1.0 / r2, inv3 = inv * inv * inv;
+	if (opt == 'f')
+		return 48. * inv3 * (inv3 - 0.5) * inv;
+	if (opt == 'u')
+		return 4.0 * inv3 * (inv3 - 1.0);
+	return 0.0;
+}
+
+double sep_lj_shift(double r2, char opt)
+{
+	const double inv = 1.0 / r2, inv3 = inv * inv * inv;
+	if (opt == 'f')
+		return 48.0 * inv3 * (inv3 - 0.5) * inv;
+	if (opt == 'u')
+		return 4.0 * inv3 * (inv3 - 1.0) + SEP_LJCF2;
+	return 0.0;
+}
+
+double sep_wca(double r2, char opt)
+{
+	const double inv = 1.0 / r2, inv3 = inv * inv * inv;
+	if (opt == 'f')
+		return 48.0 * inv3 * (inv3 - 0.5) * inv;
+	if (opt == 'u')
+		return 4.0 * inv3 * (inv3 - 1.0) + 1.0;
+	return 0.0;
+}

@@ -1,0 +1,218 @@
+/*
+ * sep_gpu.c -- the MI355X extension of the API (include/sepgpu.h):
+ * precision, residency, fused multi-step runs, diagnostics.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sephost.h"
+
+int sep_gpu_set_precision(seppart *atoms, sepsys *sys, int mode)
+{
+	if (mode != SEP_GPU_FP64 && mode != SEP_GPU_MIXED)
+		return seph_fail(SEP_GPU_EARG, "precision mode %d", mode);
+	seph_ctx *c = seph_ctx_for(atoms, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	c->precision = mode;
+	return 0;
+}
+
+int sep_gpu_resident(seppart *atoms, sepsys *sys, int on)
+{
+	seph_ctx *c = seph_ctx_for(atoms, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	if (on && !c->resident) {
+		int rc = seph_put_all(c, sys);
+		if (rc)
+			return rc;
+		c->resident = 1;
+		return 0;
+	}
+	if (!on && c->resident) {
+		int rc = seph_get_all(c, sys);
+		c->resident = 0;
+		c->nrets = 0;
+		c->nalphas = 0;
+		return rc;
+	}
+	return 0;
+}
+
+int sep_gpu_sync(seppart *atoms, sepsys *sys)
+{
+	seph_ctx *c = seph_ctx_for(atoms, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	if (!c->resident)
+		return 0;
+	return seph_get_all(c, sys);
+}
+
+int sep_gpu_upload(seppart *atoms, sepsys *sys)
+{
+	seph_ctx *c = seph_ctx_for(atoms, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	if (!c->resident)
+		return 0;
+	/* bound sepret / alpha values are re-read from the host as well */
+	int rc = seph_put_all(c, sys);
+	if (rc)
+		return rc;
+	c->nrets = 0;
+	c->nalphas = 0;
+	return 0;
+}
+
+/* ---- fused run -------------------------------------------------------- */
+
+static int run_one_step(seppart *atoms, sepsys *sys, sepret *ret,
+			const sep_gpu_op *ops, int nops)
+{
+	sep_reset_retval(ret);
+	sep_reset_force(atoms, sys);
+	for (int i = 0; i < nops; i++) {
+		const sep_gpu_op *o = &ops[i];
+		switch (o->op) {
+		case SEP_GPU_OP_PAIRS:
+			sep_force_pairs(atoms, o->types, o->p[0], o->fun, sys, ret, o->opt);
+			break;
+		case SEP_GPU_OP_LJ:
+			sep_force_lj(atoms, o->types, o->p, sys, ret, o->opt);
+			break;
+		case SEP_GPU_OP_COULOMB:
+			sep_coulomb_sf(atoms, o->p[0], sys, ret, o->opt);
+			break;
+		case SEP_GPU_OP_BOND:
+			sep_stretch_harmonic(atoms, o->type, o->p[0], o->p[1], sys, ret);
+			break;
+		case SEP_GPU_OP_ANGLE:
+			sep_angle_harmonic(atoms, o->type, o->p[0], o->p[1], sys, ret);
+			break;
+		case SEP_GPU_OP_COSSQ:
+			sep_angle_cossq(atoms, o->type, o->p[0], o->p[1], sys, ret);
+			break;
+		case SEP_GPU_OP_TORSION:
+			sep_torsion_Ryckaert(atoms, o->type, o->p, sys, ret);
+			break;
+		case SEP_GPU_OP_NOSEHOOVER:
+			sep_nosehoover(atoms, o->p[0], o->alpha, o->p[1], sys);
+			break;
+		case SEP_GPU_OP_LEAPFROG:
+			sep_leapfrog(atoms, sys, ret);
+			break;
+		default:
+			return seph_fail(SEP_GPU_EARG, "unknown op %d", o->op);
+		}
+	}
+	return sep_gpu_last_error();
+}
+
+int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
+		int nops, long nsteps, int flags)
+{
+	seph_ctx *c = seph_ctx_for(atoms, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	if (!c->resident)
+		return seph_fail(SEP_GPU_ESTATE, "sep_gpu_run needs a resident system");
+	if (nsteps <= 0)
+		return 0;
+	sep_gpu_clear_error();
+	/* the first step runs eagerly: it binds slots and sizes every buffer */
+	int rc = run_one_step(atoms, sys, ret, ops, nops);
+	if (rc || nsteps == 1)
+		return rc;
+	if (!(flags & SEP_GPU_RUN_GRAPH)) {
+		for (long s = 1; s < nsteps; s++)
+			if ((rc = run_one_step(atoms, sys, ret, ops, nops)))
+				return rc;
+		return 0;
+	}
+	if ((rc = seph_dev_fail(sepdev_graph_begin(c->dev))))
+		return rc;
+	rc = run_one_step(atoms, sys, ret, ops, nops);
+	int rc2 = seph_dev_fail(sepdev_graph_end(c->dev));
+	if (rc || rc2)
+		return rc ? rc : rc2;
+	/* the captured step was not executed; replay it nsteps - 1 times */
+	return seph_dev_fail(sepdev_graph_launch(c->dev, nsteps - 1));
+}
+
+/* ---- diagnostics ------------------------------------------------------ */
+
+int sep_gpu_neighbour_pairs(seppart *atoms, sepsys *sys, int *pairs, long cap,
+			    long *npairs)
+{
+	seph_ctx *c = seph_ctx_for(atoms, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	const int stride = sepdev_list_capacity(c->dev);
+	if (stride <= 0)
+		return seph_fail(SEP_GPU_ESTATE, "no neighbour list on the device");
+	int *counts = malloc(sizeof(int) * (size_t)c->n);
+	int *half = malloc(sizeof(int) * (size_t)c->n * (size_t)stride);
+	int maxc = 0;
+	int rc = seph_dev_fail(sepdev_get_half_list(c->dev, counts, half, stride, &maxc));
+	if (rc == 0) {
+		long tot = 0;
+		for (long i = 0; i < c->n; i++)
+			tot += counts[i];
+		*npairs = tot;
+		if (pairs && tot <= cap) {
+			long w = 0;
+			for (long i = 0; i < c->n; i++)
+				for (int e = 0; e < counts[i]; e++) {
+					pairs[2 * w] = (int)i;
+					pairs[2 * w + 1] = half[(size_t)i * stride + e];
+					w++;
+				}
+		}
+	}
+	free(counts);
+	free(half);
+	return rc;
+}
+
+int sep_gpu_timing_enable(seppart *atoms, int on)
+{
+	seph_ctx *c = seph_find(atoms);
+	if (!c)
+		return seph_fail(SEP_GPU_ESTATE, "no device context for these atoms");
+	return seph_dev_fail(sepdev_timing(c->dev, on));
+}
+
+int sep_gpu_timing_reset(seppart *atoms)
+{
+	seph_ctx *c = seph_find(atoms);
+	if (!c)
+		return seph_fail(SEP_GPU_ESTATE, "no device context for these atoms");
+	return seph_dev_fail(sepdev_timing_reset(c->dev));
+}
+
+int sep_gpu_timing_get(seppart *atoms, const char *kernel, double *total_ms, long *count)
+{
+	seph_ctx *c = seph_find(atoms);
+	if (!c)
+		return seph_fail(SEP_GPU_ESTATE, "no device context for these atoms");
+	return seph_dev_fail(sepdev_timing_get(c->dev, kernel, total_ms, count));
+}
+
+int sep_gpu_list_stats(seppart *atoms, double *mean_full, int *max_full)
+{
+	seph_ctx *c = seph_find(atoms);
+	if (!c)
+		return seph_fail(SEP_GPU_ESTATE, "no device context for these atoms");
+	return seph_dev_fail(sepdev_list_stats(c->dev, mean_full, max_full));
+}
+
+int sep_gpu_synchronize(seppart *atoms)
+{
+	seph_ctx *c = seph_find(atoms);
+	if (!c)
+		return seph_fail(SEP_GPU_ESTATE, "no device context for these atoms");
+	return seph_dev_fail(sepdev_synchronize(c->dev));
+}

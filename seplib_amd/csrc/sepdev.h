@@ -1,0 +1,166 @@
+/*
+ * sepdev.h -- internal C ABI between the C99 host layer (csrc/host) and
+ * the HIP device module (csrc/hip) of libsep.so.
+ *
+ * Plain pointers and sizes only.  Host arrays passed in or out are in the
+ * caller's ORIGINAL atom order; the device keeps its own cell-sorted order
+ * and converts.  Every function returns SEPDEV_OK or a SEP_GPU_* code from
+ * sepgpu.h and records a message retrievable with sepdev_error_message().
+ */
+#ifndef SEPDEV_H
+#define SEPDEV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SEPDEV_OK 0
+
+typedef struct sepdev sepdev;
+
+/* ---- return-value slots: a device mirror of the sepret fields the hot
+ * path writes.  Layout of one slot (doubles): */
+enum {
+	SEPDEV_RET_EPOT = 0,
+	SEPDEV_RET_ECOUL = 1,
+	SEPDEV_RET_EKIN = 2,
+	SEPDEV_RET_POTP = 3,   /* 9 doubles, row-major */
+	SEPDEV_RET_KINP = 12,  /* 9 */
+	SEPDEV_RET_BONDP = 21, /* 9 */
+	SEPDEV_RET_WORDS = 32
+};
+
+/* ---- system scalars mirrored on the device */
+typedef struct {
+	double max_dist2;
+	double tnow;
+	unsigned neighb_flag;
+	unsigned nupdate_neighb;
+	unsigned xn_pending; /* xn = x and cross_neighb = 0 still owed */
+	int error;           /* sticky device error code (SEP_GPU_*)   */
+	int max_full;        /* largest full-list length of the last build */
+	int pad;
+	double sumv2;        /* last sum m v^2 (Nose-Hoover)           */
+} sepdev_sys;
+
+/* ---- cell grid + stencil of one neighbour-list build */
+#define SEPDEV_MAX_OFF 28
+enum { SEPDEV_RULE_NE = 0, SEPDEV_RULE_GT = 1, SEPDEV_RULE_LT = 2 };
+typedef struct {
+	int n[3];        /* cells per axis                                  */
+	int ref_index;   /* 1: reference cell formula (truncation, linear
+			    index aliasing); 0: floor + periodic wrap        */
+	double inv[3];   /* unused by ref_index (which divides by l)        */
+	double l[3];     /* cell edge per axis                              */
+	double L[3];     /* box                                             */
+	double rlist2;   /* list cutoff squared                             */
+	int noff;
+	int opt;         /* SEP_ALL / SEP_EXCL_BONDED / SEP_EXCL_SAME_MOL    */
+	int bonded_all;  /* EXCL_BONDED: 1 bonds+angles+dihedrals, 0 bonds  */
+	int pad;
+	int off[SEPDEV_MAX_OFF][4]; /* dx, dy, dz, rule                     */
+} sepdev_grid;
+
+/* ---- lifetime */
+sepdev *sepdev_create(long npart, int *err);
+void sepdev_destroy(sepdev *d);
+const char *sepdev_error_message(void);
+int sepdev_last_code(void);
+void sepdev_clear_error(void);
+
+/* ---- host <-> device transfers (original order, AoS-interleaved 3-vectors
+ * x0 y0 z0 x1 y1 z1 ...) */
+enum {
+	SEPDEV_X = 0, SEPDEV_V = 1, SEPDEV_F = 2, SEPDEV_A = 3, SEPDEV_XN = 4
+};
+enum { SEPDEV_CROSS_NB = 0, SEPDEV_CROSS = 1 };
+int sepdev_put_vec(sepdev *d, int which, const double *src3n);
+int sepdev_get_vec(sepdev *d, int which, double *dst3n);
+int sepdev_put_ivec(sepdev *d, int which, const int *src3n);
+int sepdev_get_ivec(sepdev *d, int which, int *dst3n);
+int sepdev_put_scalars(sepdev *d, const double *mass, const double *charge,
+		       const signed char *type, const int *molindex);
+int sepdev_put_charges(sepdev *d, const double *charge);
+int sepdev_put_types(sepdev *d, const signed char *type);
+/* bonded partners, 40 ints per atom (bond[10], angle[10], dihed[20]) */
+int sepdev_put_partners(sepdev *d, const int *partners40n);
+int sepdev_put_sys(sepdev *d, const sepdev_sys *s);
+int sepdev_get_sys(sepdev *d, sepdev_sys *s);
+
+/* ---- return slots */
+int sepdev_ret_alloc(sepdev *d);                 /* returns slot id >= 0 */
+int sepdev_ret_zero(sepdev *d, int slot);
+int sepdev_ret_get(sepdev *d, int slot, double *words);
+int sepdev_ret_put(sepdev *d, int slot, const double *words);
+/* scalar slots (Nose-Hoover alpha) */
+int sepdev_scalar_alloc(sepdev *d, double init);
+int sepdev_scalar_get(sepdev *d, int slot, double *v);
+int sepdev_scalar_put(sepdev *d, int slot, double v);
+
+/* ---- neighbour list.  conditional = 1: do nothing unless the device
+ * neighb_flag is set (resident mode); the flag is cleared afterwards. */
+int sepdev_build_list(sepdev *d, const sepdev_grid *g, int conditional);
+/* grow the per-atom full-list capacity (drops the current list) */
+int sepdev_reserve_list(sepdev *d, int cap);
+int sepdev_list_capacity(sepdev *d);
+/* sys->max_dist2 = 0 on the device (resident sep_reset_force) */
+int sepdev_reset_max_dist(sepdev *d);
+/* half list in original numbering: counts[o] entries j > o for atom o,
+ * written to half[o*stride + e]; returns the largest count in *maxc */
+int sepdev_get_half_list(sepdev *d, int *counts, int *half, int stride,
+			 int *maxc);
+int sepdev_list_stats(sepdev *d, double *mean_full, int *max_full);
+
+/* ---- forces */
+typedef struct {
+	double cf2;             /* interaction cutoff squared               */
+	double eps48, eps4, awh, aw, sig2, ushift;  /* LJ constants         */
+	double L[3];
+	char t0, t1;            /* type filter                              */
+	int typed;              /* 0: every atom has a matching type        */
+	int epot_assign;        /* 1: ret.epot = e (sep_force_pairs list)   */
+	int mixed;              /* FP32 pair arithmetic                      */
+} sepdev_lj_args;
+int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot);
+
+typedef struct {
+	double cf, cf2, icf, icf2;
+	double L[3];
+	int mixed;
+} sepdev_coul_args;
+int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot);
+
+/* bonded terms; lists hold ORIGINAL atom indices as in sepmolinfo */
+int sepdev_put_topology(sepdev *d, const unsigned *blist, int nb,
+			const unsigned *alist, int na, const unsigned *dlist,
+			int nd);
+int sepdev_bond(sepdev *d, int type, double lbond, double ks,
+		const double *L, int ret_slot);
+int sepdev_angle(sepdev *d, int type, double angle0, double k, int cossq,
+		 const double *L, int ret_slot);
+int sepdev_torsion(sepdev *d, int type, const double *g, const double *L,
+		   int ret_slot);
+int sepdev_get_bonded_values(sepdev *d, double *blengths, double *angles,
+			     double *dihedrals);
+
+/* ---- integrators */
+int sepdev_zero_forces(sepdev *d);
+int sepdev_nosehoover(sepdev *d, double Td, double tau, double dt,
+		      int alpha_slot);
+int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *L,
+		    int ret_slot);
+
+/* ---- misc */
+int sepdev_synchronize(sepdev *d);
+int sepdev_timing(sepdev *d, int on);
+int sepdev_timing_reset(sepdev *d);
+int sepdev_timing_get(sepdev *d, const char *name, double *ms, long *count);
+int sepdev_graph_begin(sepdev *d);
+int sepdev_graph_end(sepdev *d);
+int sepdev_graph_launch(sepdev *d, long times);
+int sepdev_graph_drop(sepdev *d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

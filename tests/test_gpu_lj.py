@@ -1,0 +1,219 @@
+"""GPU parity of the LJ hot path against the C oracle (the oracle itself is
+pinned to the compiled reference by tests/test_oracle_*.py).
+
+The runs are state-synchronised: every ``block`` steps the oracle's state
+is copied into the GPU system and both rebuild their lists from identical
+positions, so chaotic divergence never enters the comparison.  Bars (the
+plain FP64 path):
+  * neighbour pair set: bit-exact (sorted (i, j), i < j);
+  * forces: max|dF_i| / RMS|F| <= 1e-12;
+  * potential energy, kinetic energy, virial and kinetic stress: <= 1e-12
+    relative (tensors relative to their largest component);
+  * positions, velocities, crossings, rebuild flags after each leapfrog.
+"""
+import numpy as np
+import pytest
+
+from seplib_amd import sep as S
+from seplib_amd import workloads as W
+from seplib_amd.sim import SepSystem
+from tests.parity import FP64_TOL, force_error, rel, tensor_error
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(ga, orc, *, types="AA", cf=2.5, fun="sep_lj_shift", opt=S.SEP_ALL, lj=None):
+    for s in (ga, orc):
+        s.reset_retval()
+        s.reset_force()
+        if lj is None:
+            s.force_pairs(types, cf, fun, opt)
+        else:
+            s.force_lj(types, lj, opt)
+
+
+def _check_forces(ga, orc, tol=FP64_TOL):
+    a, b = ga.ret_dict(), orc.ret_dict()
+    ef = force_error(np.array(ga.view["f"]), np.array(orc.view["f"]))
+    assert ef <= tol, "force error %.3e" % ef
+    scale = np.abs(b["pot_P"]).max()
+    assert rel(a["epot"], b["epot"]) <= tol, (a["epot"], b["epot"])
+    assert tensor_error(a["pot_P"], b["pot_P"]) <= tol
+    return ef
+
+
+def _check_after_leapfrog(ga, orc, tol=FP64_TOL):
+    a, b = ga.state(), orc.state()
+    for k in ("x", "v", "a", "xn"):
+        d = np.abs(a[k] - b[k]).max()
+        assert d <= tol * max(1.0, np.abs(b[k]).max()), (k, d)
+    for k in ("cross_neighb", "crossings"):
+        assert (a[k] == b[k]).all(), k
+    assert a["neighb_flag"] == b["neighb_flag"]
+    assert a["nupdate_neighb"] == b["nupdate_neighb"]
+    assert rel(a["max_dist2"], b["max_dist2"]) <= 1e-9
+    ra, rb = ga.ret_dict(), orc.ret_dict()
+    assert rel(ra["ekin"], rb["ekin"]) <= tol
+    assert tensor_error(ra["kin_P"], rb["kin_P"]) <= tol
+
+
+def _warm(orc, steps=20, **kw):
+    """Moves the oracle off its lattice start, where all forces vanish up to
+    rounding and a relative force error is meaningless."""
+    for _ in range(steps):
+        orc.reset_retval()
+        orc.reset_force()
+        if "lj" in kw:
+            for t, p in kw["lj"].items():
+                orc.force_lj(t, p)
+        else:
+            orc.force_pairs(kw.get("types", "AA"), kw.get("cf", 2.5),
+                            kw.get("fun", "sep_lj_shift"))
+        orc.leapfrog()
+
+
+def _sync_from(ga, orc):
+    st = orc.state()
+    st["neighb_flag"] = 1
+    orc.sys.neighb_flag = 1
+    ga.set_state(st)
+
+
+@pytest.mark.parametrize("nh", [False, True], ids=["nve", "nvt"])
+def test_lj_cfg1_state_synchronised(product, oracle, nh):
+    """Config 1 (SC 10^3, rho 0.8442, T 0.728, rc 2.5, skin 0.5, dt 0.005)."""
+    w = W.lj_sc(10, 0.8442, 0.728, seed=1)
+    ga = SepSystem.from_workload(product, w, cf=2.5, dt=0.005, skin=0.5)
+    orc = SepSystem.from_workload(oracle, w, cf=2.5, dt=0.005, skin=0.5)
+    _warm(orc)
+    block, nstep = 10, 120
+    worst = 0.0
+    for step in range(nstep):
+        if step % block == 0:
+            _sync_from(ga, orc)
+        rebuild = orc.sys.neighb_flag == 1
+        _pair(ga, orc)
+        if rebuild and step % block == 0:
+            hp = orc.half_pairs()
+            assert np.array_equal(ga.half_pairs(), hp)
+            assert np.array_equal(ga.gpu_pairs(), hp)
+        worst = max(worst, _check_forces(ga, orc))
+        if nh:
+            for s in (ga, orc):
+                s.nosehoover(1.0, 10.0)
+            assert rel(ga.alpha.value, orc.alpha.value, 1e-3) <= 1e-12
+            ef = force_error(np.array(ga.view["f"]), np.array(orc.view["f"]))
+            assert ef <= FP64_TOL
+        for s in (ga, orc):
+            s.leapfrog()
+        _check_after_leapfrog(ga, orc)
+    assert orc.sys.nupdate_neighb > 0
+    ga.close()
+    orc.close()
+
+
+def test_lj_resident_matches_synchronised(product):
+    """Resident mode (device-authoritative, no per-call sync), its graph
+    replay and the synchronised API give bit-identical trajectories."""
+    w = W.lj_sc(10, 0.8442, 0.728, seed=1)
+    runs = []
+    for mode in ("sync", "resident", "graph"):
+        s = SepSystem.from_workload(product, w, cf=2.5, dt=0.005, skin=0.5)
+        if mode == "sync":
+            for _ in range(60):
+                s.reset_retval()
+                s.reset_force()
+                s.force_pairs("AA", 2.5)
+                s.nosehoover(1.0, 10.0)
+                s.leapfrog()
+        else:
+            s.gpu_resident(True)
+            ops = [s.op_pairs("AA", 2.5), s.op_nosehoover(1.0, 10.0), s.op_leapfrog()]
+            s.gpu_run(ops, 60, graph=(mode == "graph"))
+            s.gpu_sync()
+        st = s.state()
+        st.update(epot=s.ret.epot, ekin=s.ret.ekin, potP=S.mat(s.ret.pot_P))
+        runs.append(st)
+        s.close()
+    for other in runs[1:]:
+        for k in ("x", "v", "f", "a", "xn", "cross_neighb", "crossings"):
+            assert np.array_equal(runs[0][k], other[k]), k
+        for k in ("epot", "ekin", "alpha", "max_dist2", "nupdate_neighb", "tnow"):
+            assert runs[0][k] == other[k], k
+        assert np.array_equal(runs[0]["potP"], other["potP"])
+
+
+@pytest.mark.parametrize("update", [S.SEP_NEIGHBLIST, S.SEP_BRUTE], ids=["n2list", "brute"])
+def test_lj_other_force_paths(product, oracle, update):
+    """The O(N^2) list mode and the brute-force mode (each rebuilt on the
+    device with its own grid) against the oracle's direct loops."""
+    w = W.lj_sc(8, 0.8442, 1.0, seed=4)
+    ga = SepSystem.from_workload(product, w, cf=2.5, dt=0.005, update=update, skin=0.4)
+    orc = SepSystem.from_workload(oracle, w, cf=2.5, dt=0.005, update=update, skin=0.4)
+    _warm(orc)
+    for step in range(20):
+        if step % 5 == 0:
+            _sync_from(ga, orc)
+        _pair(ga, orc)
+        if update == S.SEP_NEIGHBLIST and step % 5 == 0:
+            assert np.array_equal(ga.half_pairs(), orc.half_pairs())
+        _check_forces(ga, orc)
+        for s in (ga, orc):
+            s.leapfrog()
+        _check_after_leapfrog(ga, orc)
+    ga.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("fun", ["sep_lj", "sep_wca"])
+def test_lj_potential_dispatch(product, oracle, fun):
+    """sep_force_pairs recognises sep_lj and sep_wca by address."""
+    w = W.lj_sc(8, 0.7, 1.0, seed=6)
+    cf = 2.5 if fun == "sep_lj" else 2 ** (1.0 / 6.0)
+    ga = SepSystem.from_workload(product, w, cf=cf, dt=0.005)
+    orc = SepSystem.from_workload(oracle, w, cf=cf, dt=0.005)
+    _warm(orc, cf=cf, fun=fun)
+    _sync_from(ga, orc)
+    _pair(ga, orc, cf=cf, fun=fun)
+    _check_forces(ga, orc)
+    ga.close()
+    orc.close()
+
+
+def test_force_lj_mixture(product, oracle):
+    """sep_force_lj on a Kob-Andersen mixture: three typed calls sharing one
+    list (param = {rc, eps, sigma, aw}, source/sepprfrc.c:785)."""
+    w = W.ka_mixture(1500, 1.2, 0.8, seed=8)
+    prm = {"AA": (2.5, 1.0, 1.0, 1.0), "AB": (2.0, 1.5, 0.8, 1.0), "BB": (2.2, 0.5, 0.88, 1.0)}
+    ga = SepSystem.from_workload(product, w, cf=2.5, dt=0.004, skin=0.3)
+    orc = SepSystem.from_workload(oracle, w, cf=2.5, dt=0.004, skin=0.3)
+    _warm(orc, lj=prm)
+    for step in range(6):
+        if step % 3 == 0:
+            _sync_from(ga, orc)
+        for s in (ga, orc):
+            s.reset_retval()
+            s.reset_force()
+            for t, p in prm.items():
+                s.force_lj(t, p)
+        _check_forces(ga, orc)
+        for s in (ga, orc):
+            s.leapfrog()
+        _check_after_leapfrog(ga, orc)
+    ga.close()
+    orc.close()
+
+
+def test_unknown_potential_is_rejected(product):
+    """A potential the device cannot evaluate is an error, never a silent
+    CPU fallback."""
+    import ctypes as C
+    w = W.lj_sc(5, 0.8, 1.0, seed=9)
+    ga = SepSystem.from_workload(product, w, cf=2.5, dt=0.005)
+    cb = S.POTFN(lambda r2, o: 0.0)
+    product.lib.sep_gpu_clear_error()
+    ga.api.lib.sep_force_pairs(ga.atoms, b"AA", 2.5, C.cast(cb, C.c_void_p).value,
+                               ga.sysp, ga.retp, S.SEP_ALL)
+    assert product.lib.sep_gpu_last_error() == S.SEP_GPU_EUNSUPPORTED
+    product.lib.sep_gpu_clear_error()
+    ga.close()
