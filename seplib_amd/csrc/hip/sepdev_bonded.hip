@@ -85,6 +85,10 @@ extern "C" int sepdev_put_topology(sepdev *d, const unsigned *blist, int nb,
 {
 	if (d->capturing)
 		return sepk_fail(SEP_GPU_ESTATE, "topology upload during capture");
+	// one partial row per 256 terms must fit a reduction region
+	const long maxterms = (d->region_rows - 8) * (long)SEPK_BLOCK;
+	if (nb > maxterms || na > maxterms || nd > maxterms)
+		return sepk_fail(SEP_GPU_EARG, "too many bonded terms for %ld atoms", d->n);
 	int rc = 0;
 	rc |= upload_family(d, blist, nb, 3, 2, &d->blist, &d->bcontrib, &d->bcsr_off,
 			    &d->bcsr_idx, &d->blen);
@@ -130,26 +134,17 @@ static int gather(sepdev *d, const double *contrib, const int *off, const int *i
 	return 0;
 }
 
-// Shared last-block finish for the bonded kernels: acc = {e, w6}.
-__device__ __forceinline__ void sepk_bonded_finish(double (&acc)[7], double *partial,
-						   unsigned *counter, double *ret,
-						   int with_stress)
+static int bonded_finish(sepdev *d, unsigned nb, int ret_slot, int with_stress)
 {
-	sepk_block_sum<7>(acc, partial);
-	if (sepk_last_block(counter)) {
-		__shared__ double tot[7];
-		if (threadIdx.x < 7)
-			tot[threadIdx.x] = sepk_sum_partials(partial, gridDim.x, 7, threadIdx.x);
-		__syncthreads();
-		if (threadIdx.x == 0) {
-			ret[SEPDEV_RET_EPOT] += tot[0];
-			if (with_stress) {
-				sepk_add_sym9(ret + SEPDEV_RET_POTP, tot + 1, 1.0);
-				sepk_add_sym9(ret + SEPDEV_RET_BONDP, tot + 1, 1.0);
-			}
-			*counter = 0;
-		}
-	}
+	sepk_task f;
+	memset(&f, 0, sizeof f);
+	f.mode = SEPK_FIN_BONDED;
+	f.region = SEPK_R_PAIR;
+	f.nblk = (int)nb;
+	f.nv = 7;
+	f.with_stress = with_stress;
+	sepk_task_ret(d, f, ret_slot);
+	return sepk_queue_task(d, f);
 }
 
 __device__ __forceinline__ sepk_pos sepk_at(const sepk_state &s, const int *iperm, unsigned o)
@@ -162,7 +157,7 @@ __device__ __forceinline__ sepk_pos sepk_at(const sepk_state &s, const int *iper
 __global__ __launch_bounds__(SEPK_BLOCK) void k_bond(
 	sepk_state s, const int *iperm, const unsigned *bl, int nb, int type,
 	double lbond, double ks, double L0, double L1, double L2, double *contrib,
-	double *blen, double *partial, unsigned *counter, double *ret)
+	double *blen, double *partial, long rows)
 {
 	const int n = blockIdx.x * blockDim.x + threadIdx.x;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -192,7 +187,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_bond(
 			contrib[3 * (2 * n + 1) + k] = -f[k];
 		}
 	}
-	sepk_bonded_finish(acc, partial, counter, ret, 1);
+	sepk_block_partial<7>(acc, 0u, partial, blockIdx.x, rows);
 }
 
 extern "C" int sepdev_bond(sepdev *d, int type, double lbond, double ks,
@@ -200,11 +195,16 @@ extern "C" int sepdev_bond(sepdev *d, int type, double lbond, double ks,
 {
 	if (d->nb == 0)
 		return 0;
-	k_bond<<<sepk_blocks(d->nb), SEPK_BLOCK, 0, d->st>>>(
-		d->s[0], d->iperm, d->blist, d->nb, type, lbond, ks, L[0], L[1], L[2],
-		d->bcontrib, d->blen, d->partials, d->counters + SEPK_CTR_BOND,
-		d->ret + (size_t)ret_slot * SEPDEV_RET_WORDS);
+	int rc = sepk_flush_nh(d);
+	if (rc || (rc = sepk_claim_region(d, SEPK_R_PAIR)))
+		return rc;
+	const unsigned nb = sepk_blocks(d->nb);
+	k_bond<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->iperm, d->blist, d->nb, type, lbond,
+					      ks, L[0], L[1], L[2], d->bcontrib, d->blen,
+					      sepk_region(d, SEPK_R_PAIR), d->region_rows);
 	SEPK_LAUNCH_CHECK();
+	if ((rc = bonded_finish(d, nb, ret_slot, 1)))
+		return rc;
 	return gather(d, d->bcontrib, d->bcsr_off, d->bcsr_idx);
 }
 
@@ -218,8 +218,7 @@ __device__ __forceinline__ double sepk_dot3(const double *a, const double *b)
 __global__ __launch_bounds__(SEPK_BLOCK) void k_angle(
 	sepk_state s, const int *iperm, const unsigned *al, int na, int type,
 	double angle0, double kth, double ccon, int cossq, double L0, double L1,
-	double L2, double *contrib, double *aval, double *partial, unsigned *counter,
-	double *ret)
+	double L2, double *contrib, double *aval, double *partial, long rows)
 {
 	const int n = blockIdx.x * blockDim.x + threadIdx.x;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -262,7 +261,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_angle(
 			contrib[3 * (3 * n + 2) + k] = fc[k];
 		}
 	}
-	sepk_bonded_finish(acc, partial, counter, ret, 0);
+	sepk_block_partial<7>(acc, 0u, partial, blockIdx.x, rows);
 }
 
 extern "C" int sepdev_angle(sepdev *d, int type, double angle0, double k, int cossq,
@@ -272,11 +271,16 @@ extern "C" int sepdev_angle(sepdev *d, int type, double angle0, double k, int co
 		return 0;
 	// cos(pi - angle0) on the host with the C library, as the reference
 	const double ccon = cos(SEPK_PI - angle0);
-	k_angle<<<sepk_blocks(d->na), SEPK_BLOCK, 0, d->st>>>(
-		d->s[0], d->iperm, d->alist, d->na, type, angle0, k, ccon, cossq, L[0], L[1],
-		L[2], d->acontrib, d->aval, d->partials, d->counters + SEPK_CTR_BOND,
-		d->ret + (size_t)ret_slot * SEPDEV_RET_WORDS);
+	int rc = sepk_flush_nh(d);
+	if (rc || (rc = sepk_claim_region(d, SEPK_R_PAIR)))
+		return rc;
+	const unsigned nb = sepk_blocks(d->na);
+	k_angle<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->iperm, d->alist, d->na, type, angle0,
+					       k, ccon, cossq, L[0], L[1], L[2], d->acontrib,
+					       d->aval, sepk_region(d, SEPK_R_PAIR), d->region_rows);
 	SEPK_LAUNCH_CHECK();
+	if ((rc = bonded_finish(d, nb, ret_slot, 0)))
+		return rc;
 	return gather(d, d->acontrib, d->acsr_off, d->acsr_idx);
 }
 
@@ -289,7 +293,7 @@ struct sepk_rb {
 __global__ __launch_bounds__(SEPK_BLOCK) void k_torsion(
 	sepk_state s, const int *iperm, const unsigned *dl, int nd, int type,
 	sepk_rb g, double L0, double L1, double L2, double *contrib, double *dval,
-	double *partial, unsigned *counter, double *ret)
+	double *partial, long rows)
 {
 	const int n = blockIdx.x * blockDim.x + threadIdx.x;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_torsion(
 			for (int k = 0; k < 3; k++)
 				contrib[3 * (4 * n + r) + k] = F[r][k];
 	}
-	sepk_bonded_finish(acc, partial, counter, ret, 0);
+	sepk_block_partial<7>(acc, 0u, partial, blockIdx.x, rows);
 }
 
 extern "C" int sepdev_torsion(sepdev *d, int type, const double *g, const double *L,
@@ -345,17 +349,25 @@ extern "C" int sepdev_torsion(sepdev *d, int type, const double *g, const double
 	sepk_rb gg;
 	for (int i = 0; i < 6; i++)
 		gg.g[i] = g[i];
-	k_torsion<<<sepk_blocks(d->nd), SEPK_BLOCK, 0, d->st>>>(
-		d->s[0], d->iperm, d->dlist, d->nd, type, gg, L[0], L[1], L[2], d->dcontrib,
-		d->dval, d->partials, d->counters + SEPK_CTR_BOND,
-		d->ret + (size_t)ret_slot * SEPDEV_RET_WORDS);
+	int rc = sepk_flush_nh(d);
+	if (rc || (rc = sepk_claim_region(d, SEPK_R_PAIR)))
+		return rc;
+	const unsigned nb = sepk_blocks(d->nd);
+	k_torsion<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->iperm, d->dlist, d->nd, type, gg,
+						 L[0], L[1], L[2], d->dcontrib, d->dval,
+						 sepk_region(d, SEPK_R_PAIR), d->region_rows);
 	SEPK_LAUNCH_CHECK();
+	if ((rc = bonded_finish(d, nb, ret_slot, 0)))
+		return rc;
 	return gather(d, d->dcontrib, d->dcsr_off, d->dcsr_idx);
 }
 
 extern "C" int sepdev_get_bonded_values(sepdev *d, double *blengths, double *angles,
 					double *dihedrals)
 {
+	int rc = sepk_flush_tasks(d);
+	if (rc)
+		return rc;
 	if (blengths && d->nb)
 		SEPK_CHECK(hipMemcpyAsync(blengths, d->blen, sizeof(double) * d->nb, hipMemcpyDeviceToHost, d->st));
 	if (angles && d->na)

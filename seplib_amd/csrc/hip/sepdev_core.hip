@@ -116,12 +116,23 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 	rc |= dalloc(&d->order_tmp, d->npad);
 	rc |= dalloc(&d->neworder, d->npad);
 	rc |= dalloc(&d->nbr_count, d->npad);
-	d->partials_words = (long)sepk_blocks(d->npad) * 16 + 64;
+	{
+		// rows per region: the widest launch is a list kernel (SEPK_SUB
+		// lanes per atom, XCD-padded) or a bonded kernel over up to 20
+		// terms per atom
+		long rows = d->npad * SEPK_SUB / SEPK_BLOCK + 2 * SEPK_NXCD;
+		const long brows = d->npad * 20 / SEPK_BLOCK + 8;
+		if (brows > rows)
+			rows = brows;
+		d->region_rows = rows;
+		d->region_words = rows * SEPK_MAXNV;
+		d->partials_words = d->region_words * SEPK_NREGION;
+	}
 	rc |= dalloc(&d->partials, d->partials_words);
-	rc |= dalloc(&d->counters, SEPK_NCTR);
 	rc |= dalloc(&d->sys, 1);
 	d->ret_cap = 16;
 	rc |= dalloc(&d->ret, (size_t)d->ret_cap * SEPDEV_RET_WORDS);
+	d->ret_zero = (unsigned char *)calloc(d->ret_cap, 1);
 	d->scal_cap = 16;
 	rc |= dalloc(&d->scal, d->scal_cap);
 	d->stage_bytes = (size_t)d->npad * 40 * sizeof(int);
@@ -138,7 +149,6 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 	// list lengths start at zero so that a pair kernel that runs before any
 	// build reads no list entries
 	(void)hipMemsetAsync(d->nbr_count, 0, d->npad * sizeof(int), d->st);
-	(void)hipMemsetAsync(d->counters, 0, SEPK_NCTR * sizeof(unsigned), d->st);
 	(void)hipMemsetAsync(d->sys, 0, sizeof(sepdev_sys), d->st);
 	(void)hipMemsetAsync(d->ret, 0, (size_t)d->ret_cap * SEPDEV_RET_WORDS * sizeof(double), d->st);
 	for (int b = 0; b < 2; b++) {
@@ -163,7 +173,7 @@ extern "C" void sepdev_destroy(sepdev *d)
 	free_state(&d->s[1]);
 	void *ptrs[] = {d->iperm, d->cell_count, d->cell_start, d->cell_fill,
 			d->cellid_old, d->cellid_new, d->order_tmp, d->neworder,
-			d->nbr, d->nbr_count, d->partials, d->counters, d->ret,
+			d->nbr, d->nbr_count, d->partials, d->ret,
 			d->scal, d->sys, d->dstage, d->blist, d->alist, d->dlist,
 			d->bcontrib, d->acontrib, d->dcontrib, d->bcsr_off,
 			d->bcsr_idx, d->acsr_off, d->acsr_idx, d->dcsr_off,
@@ -173,6 +183,7 @@ extern "C" void sepdev_destroy(sepdev *d)
 			(void)hipFree(p);
 	if (d->hstage)
 		(void)hipHostFree(d->hstage);
+	free(d->ret_zero);
 	if (d->timing) {
 		for (int i = 0; i < 64; i++) {
 			(void)hipEventDestroy(d->t_pair.beg[i]);
@@ -303,8 +314,12 @@ extern "C" int sepdev_put_vec(sepdev *d, int which, const double *src3n)
 {
 	if (d->capturing)
 		return sepk_fail(SEP_GPU_ESTATE, "transfer during graph capture");
-	int rc = stage_in(d, src3n, (size_t)d->n * 3 * sizeof(double));
+	int rc = sepk_flush_nh(d);
 	if (rc)
+		return rc;
+	if (which == SEPDEV_V)
+		d->sumv2_valid = 0;
+	if ((rc = stage_in(d, src3n, (size_t)d->n * 3 * sizeof(double))))
 		return rc;
 	const unsigned nb = sepk_blocks(d->n);
 	if (which == SEPDEV_X) {
@@ -326,11 +341,11 @@ extern "C" int sepdev_get_vec(sepdev *d, int which, double *dst3n)
 	if (d->capturing)
 		return sepk_fail(SEP_GPU_ESTATE, "transfer during graph capture");
 	const unsigned nb = sepk_blocks(d->n);
-	if (which == SEPDEV_F) {
-		int rc = sepk_materialize_forces(d);
-		if (rc)
-			return rc;
-	}
+	int rc = sepk_flush_nh(d);
+	if (rc)
+		return rc;
+	if (which == SEPDEV_F && (rc = sepk_materialize_forces(d)))
+		return rc;
 	if (which == SEPDEV_X) {
 		k_gather_pos<<<nb, SEPK_BLOCK, 0, d->st>>>((double *)d->dstage, d->s[0].pos, d->iperm, d->n);
 	} else {
@@ -385,6 +400,8 @@ extern "C" int sepdev_put_scalars(sepdev *d, const double *mass, const double *c
 {
 	// pack into one staging upload: mass | charge | mol | type
 	const long n = d->n;
+	if (mass)
+		d->sumv2_valid = 0;
 	const size_t bytes = (size_t)n * (2 * sizeof(double) + sizeof(int) + 1);
 	if (bytes > d->stage_bytes)
 		return sepk_fail(SEP_GPU_EARG, "staging overflow");
@@ -451,6 +468,9 @@ extern "C" int sepdev_put_partners(sepdev *d, const int *partners40n)
 
 extern "C" int sepdev_put_sys(sepdev *d, const sepdev_sys *s)
 {
+	// the host record replaces everything, including the cached sum m v^2
+	d->max_reset_pending = 0;
+	d->sumv2_valid = 0;
 	SEPK_CHECK(hipStreamSynchronize(d->st));
 	memcpy(d->hstage, s, sizeof *s);
 	SEPK_CHECK(hipMemcpyAsync(d->sys, d->hstage, sizeof *s, hipMemcpyHostToDevice, d->st));
@@ -459,9 +479,14 @@ extern "C" int sepdev_put_sys(sepdev *d, const sepdev_sys *s)
 
 extern "C" int sepdev_get_sys(sepdev *d, sepdev_sys *s)
 {
+	int rc = sepk_flush_tasks(d);
+	if (rc)
+		return rc;
 	SEPK_CHECK(hipMemcpyAsync(d->hstage, d->sys, sizeof *s, hipMemcpyDeviceToHost, d->st));
 	SEPK_CHECK(hipStreamSynchronize(d->st));
 	memcpy(s, d->hstage, sizeof *s);
+	if (d->max_reset_pending)
+		s->max_dist2 = 0.0;
 	return 0;
 }
 
@@ -479,6 +504,8 @@ extern "C" int sepdev_ret_alloc(sepdev *d)
 		SEPK_CHECK(hipStreamSynchronize(d->st));
 		(void)hipFree(d->ret);
 		d->ret = nr;
+		d->ret_zero = (unsigned char *)realloc(d->ret_zero, 2 * d->ret_cap);
+		memset(d->ret_zero + d->ret_cap, 0, d->ret_cap);
 		d->ret_cap *= 2;
 	}
 	return d->nret++;
@@ -488,9 +515,8 @@ extern "C" int sepdev_ret_zero(sepdev *d, int slot)
 {
 	if (slot < 0 || slot >= d->nret)
 		return sepk_fail(SEP_GPU_EARG, "ret slot %d", slot);
-	// a fill kernel rather than a memset node, so that it can sit in a graph
-	k_zero_blocks<<<1, 64, 0, d->st>>>(d->ret + (size_t)slot * SEPDEV_RET_WORDS, SEPDEV_RET_WORDS);
-	SEPK_LAUNCH_CHECK();
+	// owed to the next kernel that writes the slot (it zeroes it first)
+	d->ret_zero[slot] = 1;
 	return 0;
 }
 
@@ -498,6 +524,16 @@ extern "C" int sepdev_ret_get(sepdev *d, int slot, double *words)
 {
 	if (slot < 0 || slot >= d->nret)
 		return sepk_fail(SEP_GPU_EARG, "ret slot %d", slot);
+	int rc = sepk_flush_tasks(d);
+	if (rc)
+		return rc;
+	if (d->ret_zero[slot]) {
+		// a fill kernel rather than a memset node, so it may sit in a graph
+		k_zero_blocks<<<1, 64, 0, d->st>>>(d->ret + (size_t)slot * SEPDEV_RET_WORDS,
+						   SEPDEV_RET_WORDS);
+		SEPK_LAUNCH_CHECK();
+		d->ret_zero[slot] = 0;
+	}
 	SEPK_CHECK(hipMemcpyAsync(d->hstage, d->ret + (size_t)slot * SEPDEV_RET_WORDS,
 				  SEPDEV_RET_WORDS * sizeof(double), hipMemcpyDeviceToHost, d->st));
 	SEPK_CHECK(hipStreamSynchronize(d->st));
@@ -510,6 +546,7 @@ extern "C" int sepdev_ret_put(sepdev *d, int slot, const double *words)
 	if (slot < 0 || slot >= d->nret)
 		return sepk_fail(SEP_GPU_EARG, "ret slot %d", slot);
 	SEPK_CHECK(hipStreamSynchronize(d->st));
+	d->ret_zero[slot] = 0;
 	memcpy(d->hstage, words, SEPDEV_RET_WORDS * sizeof(double));
 	SEPK_CHECK(hipMemcpyAsync(d->ret + (size_t)slot * SEPDEV_RET_WORDS, d->hstage,
 				  SEPDEV_RET_WORDS * sizeof(double), hipMemcpyHostToDevice, d->st));
@@ -537,6 +574,9 @@ extern "C" int sepdev_scalar_alloc(sepdev *d, double init)
 
 extern "C" int sepdev_scalar_get(sepdev *d, int slot, double *v)
 {
+	int rc = sepk_flush_nh(d);
+	if (rc || (rc = sepk_flush_tasks(d)))
+		return rc;
 	SEPK_CHECK(hipMemcpyAsync(d->hstage, d->scal + slot, sizeof(double), hipMemcpyDeviceToHost, d->st));
 	SEPK_CHECK(hipStreamSynchronize(d->st));
 	memcpy(v, d->hstage, sizeof(double));
@@ -556,6 +596,9 @@ extern "C" int sepdev_scalar_put(sepdev *d, int slot, double v)
 
 extern "C" int sepdev_synchronize(sepdev *d)
 {
+	int rc = sepk_flush_tasks(d);
+	if (rc)
+		return rc;
 	SEPK_CHECK(hipStreamSynchronize(d->st));
 	return 0;
 }
@@ -650,7 +693,14 @@ extern "C" int sepdev_graph_begin(sepdev *d)
 
 extern "C" int sepdev_graph_end(sepdev *d)
 {
+	// the captured step must leave nothing queued for the host side
+	int rc = sepk_flush_tasks(d);
 	d->capturing = 0;
+	if (rc) {
+		hipGraph_t g;
+		(void)hipStreamEndCapture(d->st, &g);
+		return rc;
+	}
 	SEPK_CHECK(hipStreamEndCapture(d->st, &d->graph));
 	SEPK_CHECK(hipGraphInstantiate(&d->exec, d->graph, nullptr, nullptr, 0));
 	return 0;

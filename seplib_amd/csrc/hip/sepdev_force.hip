@@ -48,20 +48,27 @@ extern "C" int sepdev_zero_forces(sepdev *d)
 
 struct sepk_lj {
 	double cf2, eps48, eps4, awh, aw, sig2, ushift;
-	double L0, L1, L2;
+	double L0, L1, L2, iL0, iL1, iL2;
 	float eps48f, eps4f, awhf, awf, sig2f, ushiftf;
 	signed char t0, t1;
 };
 
-// Shared epilogue: write or add the per-atom force, reduce energy + stress.
-// acc = {e, wxx, wxy, wxz, wyy, wyz, wzz}, each summed over the full list.
+// Shared epilogue of the list kernels: combine the SEPK_SUB lanes of each
+// atom (fixed xor order), write or add its force from lane 0, and leave the
+// block's energy + stress row in the pair region.  acc = {e, wxx, wxy, wxz,
+// wyy, wyz, wzz}, each summed over the lane's share of the full list.
 template <bool OVERWRITE>
 __device__ __forceinline__ void sepk_pair_epilogue(
-	long t, long n, long npad, bool act, double fx, double fy, double fz,
-	double (&acc)[7], double *f, double *partial, unsigned *counter,
-	double *ret, int epot_assign, int also_ecoul)
+	long t, int sub, long n, long npad, bool act, double fx, double fy, double fz,
+	double (&acc)[7], double *f, double *partial, long row, long rows)
 {
-	if (t < n) {
+#pragma unroll
+	for (int o = 1; o < SEPK_SUB; o <<= 1) {
+		fx += __shfl_xor(fx, o, SEPK_WAVE);
+		fy += __shfl_xor(fy, o, SEPK_WAVE);
+		fz += __shfl_xor(fz, o, SEPK_WAVE);
+	}
+	if (t < n && sub == 0) {
 		if (OVERWRITE) {
 			f[t] = fx;
 			f[npad + t] = fy;
@@ -72,33 +79,23 @@ __device__ __forceinline__ void sepk_pair_epilogue(
 			f[2 * npad + t] += fz;
 		}
 	}
-	sepk_block_sum<7>(acc, partial);
-	if (sepk_last_block(counter)) {
-		__shared__ double tot[7];
-		if (threadIdx.x < 7)
-			tot[threadIdx.x] = sepk_sum_partials(partial, gridDim.x, 7, threadIdx.x);
-		__syncthreads();
-		if (threadIdx.x == 0) {
-			const double e = 0.5 * tot[0];
-			if (epot_assign)
-				ret[SEPDEV_RET_EPOT] = e;
-			else
-				ret[SEPDEV_RET_EPOT] += e;
-			if (also_ecoul)
-				ret[SEPDEV_RET_ECOUL] += e;
-			sepk_add_sym9(ret + SEPDEV_RET_POTP, tot + 1, 0.5);
-			*counter = 0;
-		}
-	}
+	sepk_block_partial<7>(acc, 0u, partial, row, rows);
 }
 
-template <bool TYPED, bool MIXED, bool OVERWRITE>
+template <bool FAST>
+__device__ __forceinline__ double sepk_image(double d, double L, double iL)
+{
+	return FAST ? sepk_wrap_fast(d, L, iL) : sepk_wrap(d, L);
+}
+
+template <bool TYPED, bool MIXED, bool OVERWRITE, bool FAST>
 __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt,
-	long n, long npad, sepk_lj c, double *partial, unsigned *counter,
-	double *ret, int epot_assign)
+	long n, long npad, sepk_lj c, double *partial, long rows)
 {
-	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long lb = sepk_xcd_block();
+	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
+	const int sub = threadIdx.x % SEPK_SUB;
 	double fx = 0.0, fy = 0.0, fz = 0.0;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	bool act = t < n;
@@ -110,17 +107,17 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 	if (act) {
 		const sepk_pos pi = s.pos[t];
 		const int nc = cnt[t];
-		for (int k = 0; k < nc; k++) {
-			const int j = nbr[(long)k * npad + t];
+		for (int e = 0; e * SEPK_SUB + sub < nc; e++) {
+			const int j = nbr[sepk_slot(e, t, npad, sub)];
 			if (TYPED) {
 				const signed char tj = s.type[j];
 				if (!((ti == c.t0 && tj == c.t1) || (ti == c.t1 && tj == c.t0)))
 					continue;
 			}
 			const sepk_pos pj = s.pos[j];
-			const double dx = sepk_wrap(pi.x - pj.x, c.L0);
-			const double dy = sepk_wrap(pi.y - pj.y, c.L1);
-			const double dz = sepk_wrap(pi.z - pj.z, c.L2);
+			const double dx = sepk_image<FAST>(pi.x - pj.x, c.L0, c.iL0);
+			const double dy = sepk_image<FAST>(pi.y - pj.y, c.L1, c.iL1);
+			const double dz = sepk_image<FAST>(pi.z - pj.z, c.L2, c.iL2);
 			const double r2 = dx * dx + dy * dy + dz * dz;
 			if (!(r2 < c.cf2))
 				continue;
@@ -157,17 +154,38 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 			}
 		}
 	}
-	sepk_pair_epilogue<OVERWRITE>(t, n, npad, act, fx, fy, fz, acc, s.f, partial,
-				      counter, ret, epot_assign, 0);
+	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, acc, s.f, partial, lb,
+				      rows);
 }
 
 typedef void (*sepk_lj_kernel)(sepk_state, const int *, const int *, long, long,
-			       sepk_lj, double *, unsigned *, double *, int);
+			       sepk_lj, double *, long);
+
+template <bool T, bool M, bool O>
+static sepk_lj_kernel pick_lj3(bool fast)
+{
+	return fast ? k_lj<T, M, O, true> : k_lj<T, M, O, false>;
+}
 
 template <bool T, bool M>
-static sepk_lj_kernel pick_lj(bool over)
+static sepk_lj_kernel pick_lj(bool over, bool fast)
 {
-	return over ? k_lj<T, M, true> : k_lj<T, M, false>;
+	return over ? pick_lj3<T, M, true>(fast) : pick_lj3<T, M, false>(fast);
+}
+
+// The fast image is exact for listed pairs when half the box exceeds the
+// list cutoff plus the skin with room to spare (see sepk_wrap_fast).
+static bool fast_image_ok(const sepdev *d, const double *L)
+{
+	if (!d->list_built)
+		return false;
+	// listed pairs start within the list cutoff and each atom moves at
+	// most half the skin before a rebuild; twice the cutoff bounds both
+	const double reach = 2.0 * sqrt(d->grid.rlist2);
+	for (int k = 0; k < 3; k++)
+		if (!(0.5 * L[k] > 1.05 * reach))
+			return false;
+	return true;
 }
 
 extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
@@ -187,6 +205,9 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 	c.L0 = a->L[0];
 	c.L1 = a->L[1];
 	c.L2 = a->L[2];
+	c.iL0 = 1.0 / a->L[0];
+	c.iL1 = 1.0 / a->L[1];
+	c.iL2 = 1.0 / a->L[2];
 	c.eps48f = (float)a->eps48;
 	c.eps4f = (float)a->eps4;
 	c.awhf = (float)a->awh;
@@ -195,38 +216,54 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 	c.ushiftf = (float)a->ushift;
 	c.t0 = a->t0;
 	c.t1 = a->t1;
-	const bool over = d->f_zero_pending != 0;
-	sepk_lj_kernel k;
-	if (a->typed)
-		k = a->mixed ? pick_lj<true, true>(over) : pick_lj<true, false>(over);
-	else
-		k = a->mixed ? pick_lj<false, true>(over) : pick_lj<false, false>(over);
-	int rc = sepk_timer_begin(d, &d->t_pair);
+	int rc = sepk_flush_nh(d);
 	if (rc)
 		return rc;
-	k<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(
-		d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c, d->partials,
-		d->counters + SEPK_CTR_PAIR, d->ret + (size_t)ret_slot * SEPDEV_RET_WORDS,
-		a->epot_assign);
+	const bool over = d->f_zero_pending != 0;
+	const bool fast = fast_image_ok(d, a->L);
+	sepk_lj_kernel k;
+	if (a->typed)
+		k = a->mixed ? pick_lj<true, true>(over, fast) : pick_lj<true, false>(over, fast);
+	else
+		k = a->mixed ? pick_lj<false, true>(over, fast) : pick_lj<false, false>(over, fast);
+	if ((rc = sepk_claim_region(d, SEPK_R_PAIR)))
+		return rc;
+	if ((rc = sepk_timer_begin(d, &d->t_pair)))
+		return rc;
+	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
+	double *part = sepk_region(d, SEPK_R_PAIR);
+	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c, part,
+					  d->region_rows);
 	SEPK_LAUNCH_CHECK();
 	d->f_zero_pending = 0;
-	return sepk_timer_end(d, &d->t_pair);
+	if ((rc = sepk_timer_end(d, &d->t_pair)))
+		return rc;
+	sepk_task f;
+	memset(&f, 0, sizeof f);
+	f.mode = SEPK_FIN_PAIR;
+	f.region = SEPK_R_PAIR;
+	f.nblk = (int)nb;
+	f.nv = 7;
+	f.epot_assign = a->epot_assign;
+	sepk_task_ret(d, f, ret_slot);
+	return sepk_queue_task(d, f);
 }
 
 // ---- shifted-force Coulomb (source/sepcoulomb.c:96-160) ---------------------
 
 struct sepk_coul {
 	double cf, cf2, icf, icf2;
-	double L0, L1, L2;
+	double L0, L1, L2, iL0, iL1, iL2;
 };
 
-template <bool OVERWRITE>
+template <bool OVERWRITE, bool FAST>
 __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt,
-	long n, long npad, sepk_coul c, double *partial, unsigned *counter,
-	double *ret)
+	long n, long npad, sepk_coul c, double *partial, long rows)
 {
-	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long lb = sepk_xcd_block();
+	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
+	const int sub = threadIdx.x % SEPK_SUB;
 	double fx = 0.0, fy = 0.0, fz = 0.0;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	bool act = t < n;
@@ -237,14 +274,14 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 	}
 	if (act) {
 		const int nc = cnt[t];
-		for (int k = 0; k < nc; k++) {
-			const int j = nbr[(long)k * npad + t];
+		for (int e = 0; e * SEPK_SUB + sub < nc; e++) {
+			const int j = nbr[sepk_slot(e, t, npad, sub)];
 			const sepk_pos pj = s.pos[j];
 			if (fabs(pj.q) < DBL_EPSILON)
 				continue;
-			const double dx = sepk_wrap(pi.x - pj.x, c.L0);
-			const double dy = sepk_wrap(pi.y - pj.y, c.L1);
-			const double dz = sepk_wrap(pi.z - pj.z, c.L2);
+			const double dx = sepk_image<FAST>(pi.x - pj.x, c.L0, c.iL0);
+			const double dy = sepk_image<FAST>(pi.y - pj.y, c.L1, c.iL1);
+			const double dz = sepk_image<FAST>(pi.z - pj.z, c.L2, c.iL2);
 			const double r2 = dx * dx + dy * dy + dz * dz;
 			if (!(r2 < c.cf2))
 				continue;
@@ -264,8 +301,8 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 			acc[6] += gz * dz;
 		}
 	}
-	sepk_pair_epilogue<OVERWRITE>(t, n, npad, act, fx, fy, fz, acc, s.f, partial,
-				      counter, ret, 0, 1);
+	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, acc, s.f, partial, lb,
+				      rows);
 }
 
 extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot)
@@ -274,23 +311,41 @@ extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot
 		return sepk_fail(SEP_GPU_ESTATE, "Coulomb forces before a neighbour list was built");
 	if (ret_slot < 0 || ret_slot >= d->nret)
 		return sepk_fail(SEP_GPU_EARG, "ret slot %d", ret_slot);
-	sepk_coul c = {a->cf, a->cf2, a->icf, a->icf2, a->L[0], a->L[1], a->L[2]};
+	sepk_coul c = {a->cf, a->cf2, a->icf, a->icf2, a->L[0], a->L[1], a->L[2],
+		       1.0 / a->L[0], 1.0 / a->L[1], 1.0 / a->L[2]};
+	int rc = sepk_flush_nh(d);
+	if (rc)
+		return rc;
+	if ((rc = sepk_claim_region(d, SEPK_R_PAIR)))
+		return rc;
 	const bool over = d->f_zero_pending != 0;
-	auto k = over ? k_coulomb<true> : k_coulomb<false>;
-	k<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(
-		d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c, d->partials,
-		d->counters + SEPK_CTR_COUL, d->ret + (size_t)ret_slot * SEPDEV_RET_WORDS);
+	const bool fast = fast_image_ok(d, a->L);
+	auto k = over ? (fast ? k_coulomb<true, true> : k_coulomb<true, false>)
+		      : (fast ? k_coulomb<false, true> : k_coulomb<false, false>);
+	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
+	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c,
+					  sepk_region(d, SEPK_R_PAIR), d->region_rows);
 	SEPK_LAUNCH_CHECK();
 	d->f_zero_pending = 0;
-	return 0;
+	sepk_task f;
+	memset(&f, 0, sizeof f);
+	f.mode = SEPK_FIN_PAIR;
+	f.region = SEPK_R_PAIR;
+	f.nblk = (int)nb;
+	f.nv = 7;
+	f.also_ecoul = 1;
+	sepk_task_ret(d, f, ret_slot);
+	return sepk_queue_task(d, f);
 }
 
 // ---- Nose-Hoover (source/sepintgr.c:149-168) ---------------------------------
+//
+// When the sum of m v^2 of the current velocities is known (the previous
+// leapfrog computed it) the call is only recorded and the next leapfrog
+// applies f -= alpha m v itself; otherwise: sum kernel, finisher, apply.
 
-__global__ __launch_bounds__(SEPK_BLOCK) void k_nh_sum(
-	sepk_state s, long n, long npad, double npart, double Td,
-	double dt_tau2, double *alpha, double *partial, unsigned *counter,
-	sepdev_sys *sys)
+__global__ __launch_bounds__(SEPK_BLOCK) void k_nh_sum(sepk_state s, long n, long npad,
+							double *partial, long rows)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	double acc[1] = {0.0};
@@ -301,18 +356,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_nh_sum(
 			acc[0] += v * v * m;
 		}
 	}
-	sepk_block_sum<1>(acc, partial);
-	if (sepk_last_block(counter)) {
-		if (threadIdx.x == 0) {
-			const double sum = sepk_sum_partials(partial, gridDim.x, 1, 0);
-			sys->sumv2 = sum;
-			// ekin = 0.5*sum/npart; T = 0.666667*ekin
-			const double ekin = 0.5 * sum / npart;
-			const double temp = 0.666667 * ekin;
-			*alpha = *alpha + dt_tau2 * (temp / Td - 1.0);
-			*counter = 0;
-		}
-	}
+	sepk_block_partial<1>(acc, 0u, partial, blockIdx.x, rows);
 }
 
 __global__ void k_nh_apply(sepk_state s, long n, long npad, const double *alpha)
@@ -331,36 +375,66 @@ extern "C" int sepdev_nosehoover(sepdev *d, double Td, double tau, double dt,
 {
 	if (alpha_slot < 0 || alpha_slot >= d->nscal)
 		return sepk_fail(SEP_GPU_EARG, "alpha slot %d", alpha_slot);
-	int rc = sepk_materialize_forces(d);
+	int rc = sepk_flush_nh(d);
 	if (rc)
 		return rc;
-	double *alpha = d->scal + alpha_slot;
-	k_nh_sum<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(
-		d->s[0], d->n, d->npad, (double)d->n, Td, dt / (tau * tau), alpha,
-		d->partials, d->counters + SEPK_CTR_NH, d->sys);
+	if (d->sumv2_valid) {
+		d->nh_pending = 1;
+		d->nh_slot = alpha_slot;
+		d->nh_Td = Td;
+		d->nh_dt_tau2 = dt / (tau * tau);
+		return 0;
+	}
+	if ((rc = sepk_materialize_forces(d)))
+		return rc;
+	const unsigned nb = sepk_blocks(d->n);
+	if ((rc = sepk_flush_tasks(d)))
+		return rc;
+	k_nh_sum<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad,
+						sepk_region(d, SEPK_R_MISC), d->region_rows);
 	SEPK_LAUNCH_CHECK();
-	k_nh_apply<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad, alpha);
+	sepk_task f;
+	memset(&f, 0, sizeof f);
+	f.mode = SEPK_FIN_NH;
+	f.region = SEPK_R_MISC;
+	f.nblk = (int)nb;
+	f.nv = 1;
+	f.alpha = d->scal + alpha_slot;
+	f.Td = Td;
+	f.dt_tau2 = dt / (tau * tau);
+	f.npart = (double)d->n;
+	if ((rc = sepk_queue_task(d, f)) || (rc = sepk_flush_tasks(d)))
+		return rc;
+	d->sumv2_valid = 1;
+	k_nh_apply<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad, d->scal + alpha_slot);
 	SEPK_LAUNCH_CHECK();
 	return 0;
 }
 
 // ---- leapfrog (source/sepintgr.c:18-88) ---------------------------------------
+//
+// Per atom: optional fused Nose-Hoover force term, a = f/m, v += a dt,
+// x += v dt, wrap once per axis counting crossings, squared displacement
+// from xn.  Row: {ekin sum, kin_P (6), max d2, sum m v_new^2}.
 
 __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 	sepk_state s, long n, long npad, double dt, double L0, double L1, double L2,
-	double skin, double *partial, unsigned *counter, sepdev_sys *sys,
-	double *ret)
+	const sepdev_sys *sys, int nh, const double *alpha, double Td, double dt_tau2,
+	double *partial, long rows)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-	double d2 = 0.0;
+	double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	const int pending = sys->xn_pending;
+	double al = 0.0;
+	if (nh)
+		al = sepk_nh_alpha(*alpha, sys->sumv2, (double)n, Td, dt_tau2);
 	if (t < n) {
 		const double L[3] = {L0, L1, L2};
 		sepk_pos p = s.pos[t];
 		double x[3] = {p.x, p.y, p.z};
 		const double m = s.m[t];
 		double vh[3];
+		double d2 = 0.0;
 		for (int k = 0; k < 3; k++) {
 			const long i = k * npad + t;
 			double xn = s.xn[i];
@@ -370,11 +444,18 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 				cn = 0;
 				s.xn[i] = xn;
 			}
-			const double a = s.f[i] / m;
-			double v = s.v[i] + a * dt;
+			double v = s.v[i];
+			double f = s.f[i];
+			if (nh) {
+				f -= al * m * v;
+				s.f[i] = f;
+			}
+			const double a = f / m;
+			v += a * dt;
 			x[k] += v * dt;
 			vh[k] = v - 0.5 * a * dt;
 			acc[0] += vh[k] * vh[k] * m;
+			acc[8] += v * v * m;
 			int cr = s.cr[i];
 			if (x[k] > L[k]) {
 				x[k] -= L[k];
@@ -402,36 +483,9 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 		acc[4] = vh[1] * vh[1] * m;
 		acc[5] = vh[1] * vh[2] * m;
 		acc[6] = vh[2] * vh[2] * m;
+		acc[7] = d2;
 	}
-	const long nblk = gridDim.x;
-	sepk_block_sum<7>(acc, partial);
-	sepk_block_max(d2, partial + nblk * 7 + blockIdx.x);
-	if (sepk_last_block(counter)) {
-		__shared__ double tot[7];
-		if (threadIdx.x < 7)
-			tot[threadIdx.x] = sepk_sum_partials(partial, nblk, 7, threadIdx.x);
-		__syncthreads();
-		if (threadIdx.x == 0) {
-			double mx = sys->max_dist2;
-			for (long b = 0; b < nblk; b++) {
-				const double v = sepk_ld(partial + nblk * 7 + b);
-				if (v > mx)
-					mx = v;
-			}
-			sys->max_dist2 = mx;
-			ret[SEPDEV_RET_EKIN] += 0.5 * tot[0];
-			sepk_add_sym9(ret + SEPDEV_RET_KINP, tot + 1, 1.0);
-			if (pending)
-				sys->xn_pending = 0;
-			if (sqrt(mx) > skin * 0.5) {
-				sys->neighb_flag = 1;
-				sys->nupdate_neighb++;
-				sys->xn_pending = 1;
-			}
-			sys->tnow += dt;
-			*counter = 0;
-		}
-	}
+	sepk_block_partial<9>(acc, 1u << 7, partial, blockIdx.x, rows);
 }
 
 extern "C" int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *L,
@@ -442,10 +496,38 @@ extern "C" int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *
 	int rc = sepk_materialize_forces(d);
 	if (rc)
 		return rc;
-	k_leapfrog<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(
-		d->s[0], d->n, d->npad, dt, L[0], L[1], L[2], skin, d->partials,
-		d->counters + SEPK_CTR_LF, d->sys, d->ret + (size_t)ret_slot * SEPDEV_RET_WORDS);
+	if ((rc = sepk_claim_region(d, SEPK_R_LF)))
+		return rc;
+	const int nh = d->nh_pending;
+	d->nh_pending = 0;
+	const unsigned nb = sepk_blocks(d->n);
+	double *alpha = nh ? d->scal + d->nh_slot : d->scal;
+	k_leapfrog<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad, dt, L[0], L[1], L[2],
+						  d->sys, nh, alpha, d->nh_Td, d->nh_dt_tau2,
+						  sepk_region(d, SEPK_R_LF), d->region_rows);
 	SEPK_LAUNCH_CHECK();
+	sepk_task f;
+	memset(&f, 0, sizeof f);
+	f.mode = SEPK_FIN_LF;
+	f.region = SEPK_R_LF;
+	f.nblk = (int)nb;
+	f.nv = 9;
+	f.maxmask = 1u << 7;
+	f.nh_fused = nh;
+	f.alpha = alpha;
+	f.Td = d->nh_Td;
+	f.dt_tau2 = d->nh_dt_tau2;
+	f.npart = (double)d->n;
+	f.reset_max = d->max_reset_pending;
+	d->max_reset_pending = 0;
+	f.skin = skin;
+	f.dt = dt;
+	sepk_task_ret(d, f, ret_slot);
+	// one finisher launch for the whole step: any queued build and pair
+	// results, then the leapfrog's
+	if ((rc = sepk_queue_task(d, f)) || (rc = sepk_flush_tasks(d)))
+		return rc;
+	d->sumv2_valid = 1;
 	return 0;
 }
 
@@ -460,7 +542,7 @@ __global__ void k_half_list(const int *orig, const int *nbr, const int *cnt, lon
 	const int o = orig[t];
 	int c = 0;
 	for (int k = 0; k < cnt[t]; k++) {
-		const int oj = orig[nbr[(long)k * npad + t]];
+		const int oj = orig[nbr[sepk_slot(k / SEPK_SUB, t, npad, k % SEPK_SUB)]];
 		if (oj > o) {
 			if (c < stride)
 				half[(long)o * stride + c] = oj;
@@ -475,6 +557,9 @@ extern "C" int sepdev_get_half_list(sepdev *d, int *counts, int *half, int strid
 {
 	if (!d->list_built)
 		return sepk_fail(SEP_GPU_ESTATE, "no neighbour list");
+	int rc0 = sepk_flush_tasks(d);
+	if (rc0)
+		return rc0;
 	int *dc = nullptr, *dh = nullptr;
 	SEPK_CHECK(hipMalloc(&dc, d->n * sizeof(int)));
 	SEPK_CHECK(hipMalloc(&dh, (size_t)d->n * stride * sizeof(int)));

@@ -10,6 +10,16 @@
 
 #define SEPK_BLOCK 256
 #define SEPK_WAVE 64
+// Lanes per atom in the list kernels.  Entry k of atom t's list is stored
+// at slot ((k/SEPK_SUB)*npad + t)*SEPK_SUB + k%SEPK_SUB, and lane k%SEPK_SUB
+// of the atom reads it: a wave covers 64/SEPK_SUB consecutive atoms, each of
+// its list reads is 256 contiguous bytes, and since every list is in cell
+// order the lanes of one atom gather neighbours from the same cache lines.
+#define SEPK_SUB SEPDEV_LANES
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2); list kernels remap so that each XCD gets one contiguous range of
+// cell-sorted atoms, whose neighbours are mostly in the same range.
+#define SEPK_NXCD 8
 
 // 32-byte aligned position record: x, y, z and the point charge.  One
 // record per atom so that a neighbour gather is a single 32-byte access.
@@ -31,6 +41,38 @@ struct sepk_state {
 	int *partners; // 40 blocks of npad, allocated with the topology
 };
 
+// What the finisher does with the combined rows.  Partial rows live in
+// separate regions so that a build, a pair call and a leapfrog of one step
+// can all be finished by a single launch after the leapfrog.
+enum { SEPK_FIN_PAIR = 0, SEPK_FIN_BONDED = 1, SEPK_FIN_NH = 2, SEPK_FIN_LF = 3,
+       SEPK_FIN_BUILD = 4 };
+enum { SEPK_R_BUILD = 0, SEPK_R_PAIR = 1, SEPK_R_LF = 2, SEPK_R_MISC = 3, SEPK_NREGION = 4 };
+#define SEPK_MAXNV 10
+#define SEPK_MAXTASK 4
+
+struct sepk_task {
+	int mode, nblk, nv, region;
+	unsigned maxmask;
+	double *ret;      // return slot (32 words) or null
+	int zero_first;   // the slot was reset since its last write
+	int epot_assign;  // PAIR: epot = e instead of +=
+	int also_ecoul;   // PAIR: Coulomb energy also goes to ecoul
+	int with_stress;  // BONDED: add the stress to pot_P and pot_P_bond
+	double *alpha;    // NH / LF with a fused thermostat
+	double Td, dt_tau2, npart;
+	int nh_fused;     // LF: the leapfrog applied a pending Nose-Hoover
+	int reset_max;    // LF: max_dist2 was reset since the last leapfrog
+	double skin, dt;
+	int conditional;  // BUILD: only when the rebuild flag is set
+};
+
+struct sepk_fin {
+	sepk_task t[SEPK_MAXTASK];
+	int ntask;
+	long region_words, region_rows;
+	sepdev_sys *sys;
+};
+
 struct sepk_timer {
 	hipEvent_t beg[64], end[64];
 	int used;
@@ -50,16 +92,21 @@ struct sepdev {
 	int *cell_count, *cell_start, *cell_fill;
 	int *cellid_old, *cellid_new, *order_tmp, *neworder;
 
-	// full neighbour list, entry k of atom t at nbr[k*npad + t]
-	int cap;
+	// full neighbour list: entry k of atom t at slot
+	// ((k/SEPK_SUB)*npad + t)*SEPK_SUB + k%SEPK_SUB, nbr_count[t] entries,
+	// at most cap = lcap*SEPK_SUB
+	int cap, lcap;
 	int *nbr, *nbr_count;
 	int list_built;
 	sepdev_grid grid; // grid of the last build
 
-	// reductions: per-block partial sums and last-block counters
+	// reductions: one row of partial results per block, read by k_finish;
+	// SEPK_NREGION regions of region_words = SEPK_MAXNV * region_rows
+	// doubles, each column-major
 	double *partials;
-	long partials_words;
-	unsigned *counters;
+	long partials_words, region_words, region_rows;
+	sepk_task pending[SEPK_MAXTASK];
+	int npending;
 
 	// return and scalar slots
 	double *ret;
@@ -72,8 +119,15 @@ struct sepdev {
 	void *hstage, *dstage;
 	size_t stage_bytes;
 
-	// forces are known to be zero on the device (lazy sep_reset_force)
-	int f_zero_pending;
+	// Deferred work, decided on the host so that a captured step replays
+	// the same kernels every time:
+	int f_zero_pending;          // f = 0 owed (the next force kernel overwrites)
+	int max_reset_pending;       // sys->max_dist2 = 0 owed (the next leapfrog starts at 0)
+	unsigned char *ret_zero;     // per return slot: zero owed before the next write
+	int nh_pending;              // a Nose-Hoover call waits to be fused into leapfrog
+	int nh_slot;
+	double nh_Td, nh_dt_tau2;
+	int sumv2_valid;             // sys->sumv2 holds sum m v^2 of the current v
 
 	// topology (original atom numbering)
 	int nb, na, nd;
@@ -114,11 +168,25 @@ static inline unsigned sepk_blocks(long n)
 	return (unsigned)((n + SEPK_BLOCK - 1) / SEPK_BLOCK);
 }
 
-// Counter indices for the last-block reductions.
-enum { SEPK_CTR_PAIR = 0, SEPK_CTR_LF = 1, SEPK_CTR_NH = 2, SEPK_CTR_BOND = 3,
-       SEPK_CTR_COUL = 4, SEPK_CTR_MISC = 5, SEPK_NCTR = 8 };
-
 // ---- device helpers ----------------------------------------------------
+
+static inline unsigned sepk_blocks_xcd(long threads)
+{
+	const unsigned nb = sepk_blocks(threads);
+	return (nb + SEPK_NXCD - 1) / SEPK_NXCD * SEPK_NXCD;
+}
+
+// Logical block of an XCD-remapped launch (gridDim.x a multiple of 8).
+__device__ __forceinline__ long sepk_xcd_block()
+{
+	const unsigned per = gridDim.x / SEPK_NXCD;
+	return (long)(blockIdx.x % SEPK_NXCD) * per + blockIdx.x / SEPK_NXCD;
+}
+
+__device__ __forceinline__ long sepk_slot(long e, long t, long npad, int sub)
+{
+	return (e * npad + t) * SEPK_SUB + sub;
+}
 
 // Single-image minimum image with strict comparisons against +-L/2, the
 // rule of the reference's sep_Wrap macro (include/sepmisc.h:81-85).
@@ -131,86 +199,70 @@ __device__ __forceinline__ double sepk_wrap(double d, double L)
 	return d;
 }
 
-// Block-wide sum of NV doubles per thread into partial[blockIdx.x*NV + i],
-// in a fixed order (wave butterfly, then waves in index order).
+// The same image with three instructions: d - L*rint(d/L).  Equal to
+// sepk_wrap bit for bit whenever |d| < 1.5 L and |d mod L| is not within an
+// ulp of L/2 -- true for every listed pair once L/2 exceeds the list
+// cutoff plus the skin, which the host checks before choosing it.
+__device__ __forceinline__ double sepk_wrap_fast(double d, double L, double invL)
+{
+	return fma(-L, rint(d * invL), d);
+}
+
+// Per-block reduction of NV values per thread into column-major partial
+// rows, partial[i*rows + row]:
+// columns whose bit is set in maxmask take the maximum, the others the sum,
+// in a fixed order (wave butterfly, then the block's waves in index order).
+// A one-block finisher kernel (k_finish) combines the rows afterwards; the
+// kernel boundary makes every row visible to it, so no fences are needed.
 template <int NV>
-__device__ __forceinline__ void sepk_block_sum(double (&v)[NV], double *partial)
+__device__ __forceinline__ void sepk_block_partial(double (&v)[NV], unsigned maxmask,
+						   double *partial, long row, long rows)
 {
 	__shared__ double sh[SEPK_BLOCK / SEPK_WAVE][NV];
 	const int lane = threadIdx.x & (SEPK_WAVE - 1), wid = threadIdx.x / SEPK_WAVE;
 #pragma unroll
 	for (int i = 0; i < NV; i++) {
 		double x = v[i];
+		const bool mx = (maxmask >> i) & 1u;
 #pragma unroll
-		for (int o = SEPK_WAVE / 2; o > 0; o >>= 1)
-			x += __shfl_xor(x, o, SEPK_WAVE);
-		v[i] = x;
+		for (int o = SEPK_WAVE / 2; o > 0; o >>= 1) {
+			const double y = __shfl_xor(x, o, SEPK_WAVE);
+			x = mx ? fmax(x, y) : x + y;
+		}
+		if (lane == 0)
+			sh[wid][i] = x;
 	}
-	if (lane == 0)
-#pragma unroll
-		for (int i = 0; i < NV; i++)
-			sh[wid][i] = v[i];
 	__syncthreads();
 	if (threadIdx.x < NV) {
-		double s = 0.0;
-		for (int w = 0; w < SEPK_BLOCK / SEPK_WAVE; w++)
-			s += sh[w][threadIdx.x];
-		partial[(size_t)blockIdx.x * NV + threadIdx.x] = s;
-	}
-}
-
-__device__ __forceinline__ double sepk_block_max(double v, double *partial_slot)
-{
-	__shared__ double shm[SEPK_BLOCK / SEPK_WAVE];
-	const int lane = threadIdx.x & (SEPK_WAVE - 1), wid = threadIdx.x / SEPK_WAVE;
-#pragma unroll
-	for (int o = SEPK_WAVE / 2; o > 0; o >>= 1)
-		v = fmax(v, __shfl_xor(v, o, SEPK_WAVE));
-	if (lane == 0)
-		shm[wid] = v;
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		double m = shm[0];
+		const int i = threadIdx.x;
+		const bool mx = (maxmask >> i) & 1u;
+		double s = sh[0][i];
 		for (int w = 1; w < SEPK_BLOCK / SEPK_WAVE; w++)
-			m = fmax(m, shm[w]);
-		*partial_slot = m;
+			s = mx ? fmax(s, sh[w][i]) : s + sh[w][i];
+		partial[(size_t)i * rows + row] = s;
 	}
-	return v;
 }
 
-// Returns true in every thread of the block that finished last.  Each block
-// has written its partials before calling; the fences make them visible to
-// the last block (device scope, across XCDs).
-__device__ __forceinline__ bool sepk_last_block(unsigned *counter)
+// Nose-Hoover friction update of source/sepintgr.c:151-162, from the sum of
+// m v^2 over all atoms: ekin = 0.5 sum / N, T = 0.666667 ekin,
+// alpha += dt/tau^2 (T/Td - 1).
+__device__ __forceinline__ double sepk_nh_alpha(double alpha, double sumv2, double npart,
+						double Td, double dt_tau2)
 {
-	__shared__ int last;
-	__threadfence();
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		unsigned t = atomicAdd(counter, 1u);
-		last = (t == gridDim.x - 1);
-	}
-	__syncthreads();
-	if (last)
-		__threadfence();
-	return last;
+	const double ekin = 0.5 * sumv2 / npart;
+	const double temp = 0.666667 * ekin;
+	return alpha + dt_tau2 * (temp / Td - 1.0);
 }
 
-__device__ __forceinline__ double sepk_ld(const double *p)
-{
-	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Sum of column i over nblk partial rows (NV words each), in block order,
-// done by the last block with one thread per column.
-__device__ __forceinline__ double sepk_sum_partials(const double *partial,
-						    int nblk, int nv, int i)
-{
-	double s = 0.0;
-	for (int b = 0; b < nblk; b++)
-		s += sepk_ld(partial + (size_t)b * nv + i);
-	return s;
-}
+// Queues a finishing task; it runs (with any queued before it) in one
+// launch at the next sepk_flush_tasks, which every leapfrog, every host
+// read of device results and the end of a graph capture perform.
+int sepk_queue_task(sepdev *d, const sepk_task &t);
+int sepk_flush_tasks(sepdev *d);
+double *sepk_region(sepdev *d, int region);
+// Finishes queued work first if a queued task still reads `region`, which
+// the caller's next kernel is about to overwrite.
+int sepk_claim_region(sepdev *d, int region);
 
 // Writes the 6 unique stress components (xx xy xz yy yz zz) into a
 // row-major 3x3 block.
@@ -228,5 +280,7 @@ __device__ __forceinline__ void sepk_add_sym9(double *dst9, const double *w6,
 	dst9[8] += scale * w6[5];
 }
 
-// internal launch helpers implemented in sepdev.hip
+// deferred-work helpers (sepdev_force.hip, sepdev_reduce.hip)
 int sepk_materialize_forces(sepdev *d);
+int sepk_flush_nh(sepdev *d);
+void sepk_task_ret(sepdev *d, sepk_task &t, int slot);

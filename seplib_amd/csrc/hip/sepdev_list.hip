@@ -1,26 +1,29 @@
 // sepdev_list.hip -- cell binning and Verlet neighbour-list build.
 //
-// A rebuild is five kernels on the library stream:
-//   k_bin        cell index per atom (reference formula or floor+wrap),
-//                per-cell counts by atomics
-//   k_scan       one workgroup: exclusive scan of the counts -> cell_start
-//   k_scatter    atom -> slot inside its cell segment (atomic fill)
-//   k_cell_sort  one wave per cell: order the segment by original atom
-//                index, so the device order and every list are independent
-//                of atomic timing
-//   k_permute    move all per-atom state into the new (cell-sorted) order
-//                in a spare copy; k_copy_back copies it home
-//   k_build      one thread per atom: scan the stencil cells' contiguous
-//                segments and append neighbours within the list cutoff
-// With conditional = 1 each kernel returns at once unless the device
-// rebuild flag is set, so the whole sequence can sit in a captured graph.
+// A rebuild is a short kernel sequence on the library stream:
+//   k_bin          cell index per atom (reference formula or floor+wrap),
+//                  per-cell counts by atomics
+//   k_scan         one workgroup: exclusive scan of the counts -> cell_start
+//   k_scatter      atom -> slot inside its cell segment (atomic fill)
+//   k_sort_permute one wave per cell: order the segment by original atom
+//                  index (so the device order and every list are independent
+//                  of atomic timing) and move each member's state to its new
+//                  cell-sorted index in the spare copy
+//   k_build        SEPK_SUB lanes per atom: copy the atom's state home and
+//                  scan the stencil cells' contiguous segments, appending
+//                  neighbours within the list cutoff to the lane's own slots
+// and a BUILD task for the step's finisher.  With conditional = 1 every
+// kernel returns at once unless the device rebuild flag is set, so the
+// sequence can sit in a captured graph.
 //
 // The pair multiset equals the reference's half list of
 // source/sepprfrc.c:419-513 (and its exclusion variants :517-700) taken in
 // both directions: every half-shell offset s contributes j in cell(i)+s
 // ("forward") and cell(i)-s ("backward"), with the reference's
 // "m1 != m2 || j2 > j1" rule applied on original indices when the offset
-// wraps back onto the home cell.
+// wraps back onto the home cell.  r2 is formed as the reference does
+// (difference, single-image wrap, x2 + y2 + z2 in that order, each
+// operation rounded on its own), so the set is identical bit for bit.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -47,7 +50,7 @@ __device__ __forceinline__ long sepk_cell_of(const sepk_pos &p, const sepdev_gri
 		if (!(fabs(q[k]) < 1.0e9))
 			return -1;
 		if (g.ref_index)
-			c[k] = (int)q[k]; // truncation, as the reference
+			c[k] = (int)q[k]; // truncation, as source/sepprfrc.c:403-405
 		else
 			c[k] = sepk_mod((int)floor(q[k]), g.n[k]);
 	}
@@ -58,21 +61,30 @@ __device__ __forceinline__ long sepk_cell_of(const sepk_pos &p, const sepdev_gri
 	return lin;
 }
 
+#define SEPK_GRID_CAP 2048 // grid-stride launches: at most this many blocks
+
 __global__ void k_bin(const sepk_pos *pos, long n, sepdev_grid g, int *cellid,
 		      int *count, sepdev_sys *sys, int conditional)
 {
 	if (conditional && sys->neighb_flag == 0)
 		return;
-	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n)
-		return;
-	long c = sepk_cell_of(pos[t], g);
-	if (c < 0) {
-		sepk_set_error(sys, SEP_GPU_ECELL);
-		c = 0;
+	for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+	     t += (long)gridDim.x * blockDim.x) {
+		const sepk_pos p = pos[t];
+		long c = sepk_cell_of(p, g);
+		if (c < 0) {
+			sepk_set_error(sys, SEP_GPU_ECELL);
+			c = 0;
+		}
+		// an atom outside [0, n*l) on some axis (allowed: it is binned by
+		// truncation or wrapping) rules out the per-cell image shifts
+		const double q[3] = {p.x / g.l[0], p.y / g.l[1], p.z / g.l[2]};
+		if (!(q[0] >= 0.0 && q[0] < g.n[0] && q[1] >= 0.0 && q[1] < g.n[1] &&
+		      q[2] >= 0.0 && q[2] < g.n[2]))
+			sys->unclean = 1;
+		cellid[t] = (int)c;
+		atomicAdd(&count[c], 1);
 	}
-	cellid[t] = (int)c;
-	atomicAdd(&count[c], 1);
 }
 
 // Exclusive scan of count[0..ncell) into start[0..ncell]; also zeroes count
@@ -127,80 +139,79 @@ __global__ void k_scatter(const int *cellid, const int *start, int *fill,
 {
 	if (conditional && sys->neighb_flag == 0)
 		return;
-	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n)
-		return;
-	const int c = cellid[t];
-	const int slot = start[c] + atomicAdd(&fill[c], 1);
-	order[slot] = (int)t;
-}
-
-// One wave per cell: rank each member by its original index.
-__global__ void k_cell_sort(const int *order, const int *start, const int *orig,
-			    int *neworder, int *cellid_new, long ncell,
-			    sepdev_sys *sys, int conditional)
-{
-	if (conditional && sys->neighb_flag == 0)
-		return;
-	const long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64;
-	const int lane = threadIdx.x & 63;
-	if (c >= ncell)
-		return;
-	const int s = start[c], e = start[c + 1];
-	for (int i = s + lane; i < e; i += 64) {
-		const int mine = order[i];
-		const int key = orig[mine];
-		int rank = 0;
-		for (int j = s; j < e; j++)
-			rank += orig[order[j]] < key;
-		neworder[s + rank] = mine;
-		cellid_new[s + rank] = (int)c;
+	for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+	     t += (long)gridDim.x * blockDim.x) {
+		const int c = cellid[t];
+		const int slot = start[c] + atomicAdd(&fill[c], 1);
+		order[slot] = (int)t;
 	}
 }
 
-__global__ void k_permute(sepk_state src, sepk_state dst, const int *neworder,
-			  int *iperm, long n, long npad, sepdev_sys *sys,
-			  int conditional, int has_partners)
+// One wave per cell (grid-stride over cells): rank each member by its
+// original index and move its state to start + rank in the spare copy.
+__global__ void k_sort_permute(const int *order, const int *start, sepk_state src,
+			       sepk_state dst, int *iperm, int *cellid_new, long ncell,
+			       long npad, sepdev_sys *sys, int conditional,
+			       int has_partners)
 {
 	if (conditional && sys->neighb_flag == 0)
 		return;
-	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n)
-		return;
-	const long o = neworder[t];
 	const int pending = sys->xn_pending;
-	const sepk_pos p = src.pos[o];
-	dst.pos[t] = p;
-	const double xyz[3] = {p.x, p.y, p.z};
-	for (int k = 0; k < 3; k++) {
-		dst.v[k * npad + t] = src.v[k * npad + o];
-		dst.f[k * npad + t] = src.f[k * npad + o];
-		dst.a[k * npad + t] = src.a[k * npad + o];
-		dst.xn[k * npad + t] = pending ? xyz[k] : src.xn[k * npad + o];
-		dst.cn[k * npad + t] = pending ? 0 : src.cn[k * npad + o];
-		dst.cr[k * npad + t] = src.cr[k * npad + o];
+	const int lane = threadIdx.x & 63;
+	const long waves = (long)gridDim.x * (blockDim.x / 64);
+	for (long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64; c < ncell; c += waves) {
+		const int s = start[c], e = start[c + 1];
+		for (int i = s + lane; i < e; i += 64) {
+			const long o = order[i];
+			const int key = src.orig[o];
+			int rank = 0;
+			for (int j = s; j < e; j++)
+				rank += src.orig[order[j]] < key;
+			const long t = s + rank;
+			cellid_new[t] = (int)c;
+			const sepk_pos p = src.pos[o];
+			dst.pos[t] = p;
+			const double xyz[3] = {p.x, p.y, p.z};
+			for (int k = 0; k < 3; k++) {
+				dst.v[k * npad + t] = src.v[k * npad + o];
+				dst.f[k * npad + t] = src.f[k * npad + o];
+				dst.a[k * npad + t] = src.a[k * npad + o];
+				dst.xn[k * npad + t] = pending ? xyz[k] : src.xn[k * npad + o];
+				dst.cn[k * npad + t] = pending ? 0 : src.cn[k * npad + o];
+				dst.cr[k * npad + t] = src.cr[k * npad + o];
+			}
+			dst.m[t] = src.m[o];
+			dst.type[t] = src.type[o];
+			dst.mol[t] = src.mol[o];
+			dst.orig[t] = key;
+			iperm[key] = (int)t;
+			if (has_partners)
+				for (int k = 0; k < 40; k++)
+					dst.partners[k * npad + t] = src.partners[k * npad + o];
+		}
 	}
-	dst.m[t] = src.m[o];
-	dst.type[t] = src.type[o];
-	dst.mol[t] = src.mol[o];
-	const int og = src.orig[o];
-	dst.orig[t] = og;
-	iperm[og] = (int)t;
-	if (has_partners)
-		for (int k = 0; k < 40; k++)
-			dst.partners[k * npad + t] = src.partners[k * npad + o];
 }
 
-// Copies the permuted spare state back into the current buffers, so that
-// the buffers every other kernel uses never change address.
-__global__ void k_copy_back(sepk_state src, sepk_state dst, long n, long npad,
-			    sepdev_sys *sys, int conditional, int has_partners)
+// Partner scan of source/sepprfrc.c:703-731: stops at the first -1 in
+// either atom's array.  Partner values are original indices.
+__device__ __forceinline__ int sepk_share(const int *P, long npad, long ti, long tj,
+					  int base, int cap, int oi, int oj)
 {
-	if (conditional && sys->neighb_flag == 0)
-		return;
-	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n)
-		return;
+	for (int k = 0; k < cap; k++) {
+		const int a = P[(base + k) * npad + ti], b = P[(base + k) * npad + tj];
+		if (a == -1 || b == -1)
+			return 0;
+		if (a == oj || b == oi)
+			return 1;
+	}
+	return 0;
+}
+
+// Copies atom t of the permuted spare state back to the current buffers,
+// whose addresses every other kernel (and a captured graph) holds.
+__device__ __forceinline__ void sepk_copy_home(const sepk_state &src, const sepk_state &dst,
+					       long t, long npad, int has_partners)
+{
 	dst.pos[t] = src.pos[t];
 	for (int k = 0; k < 3; k++) {
 		dst.v[k * npad + t] = src.v[k * npad + t];
@@ -219,100 +230,125 @@ __global__ void k_copy_back(sepk_state src, sepk_state dst, long n, long npad,
 			dst.partners[k * npad + t] = src.partners[k * npad + t];
 }
 
-// Partner scan of source/sepprfrc.c:703-731: stops at the first -1 in
-// either atom's array.  Partner values are original indices.
-__device__ __forceinline__ int sepk_share(const int *P, long npad, long ti, long tj,
-					  int base, int cap, int oi, int oj)
+// SEPK_SUB lanes per atom scan each stencil cell's contiguous segment
+// together, SEPK_SUB consecutive candidates per round; a wave ballot gives
+// every accepting lane its position, so each atom's list comes out in
+// stencil-cell order and, within a cell, in device order -- the same order
+// whatever the lane count.  Entry k goes to lane k % SEPK_SUB's slots.
+//
+// SHIFT: with at least 5 cells on every axis and every atom inside its own
+// cell, |x_i - x_j| < 2l <= 0.4 L for a target cell reached without
+// crossing the box and > L - 2l >= 0.6 L for one reached across it, so the
+// reference's single-image wrap (subtract or add L exactly when |d| > L/2)
+// is the constant shift of the target cell; d + shift rounds exactly like
+// its d - L / d + L.  Otherwise the wrap is evaluated per candidate.
+template <bool SHIFT>
+__device__ __forceinline__ void sepk_build_atom(
+	const sepk_state &s, long t, int sub, int gshift, const int *cellid,
+	const int *start, const sepdev_grid &g, long npad, int cap, int *nbr, int &m_out)
 {
-	for (int k = 0; k < cap; k++) {
-		const int a = P[(base + k) * npad + ti], b = P[(base + k) * npad + tj];
-		if (a == -1 || b == -1)
-			return 0;
-		if (a == oj || b == oi)
-			return 1;
+	const unsigned long long below = (1ull << sub) - 1ull;
+	const int c = cellid[t];
+	const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
+	const sepk_pos pi = s.pos[t];
+	const int oi = s.orig[t];
+	const int mi = s.mol[t];
+	int m = 0; // entries so far; equal in the atom's lanes
+	for (int o = 0; o < g.noff; o++) {
+		const int ux = cx + g.off[o][0], uy = cy + g.off[o][1], uz = cz + g.off[o][2];
+		const int tx = sepk_mod(ux, g.n[0]);
+		const int ty = sepk_mod(uy, g.n[1]);
+		const int tz = sepk_mod(uz, g.n[2]);
+		const double sx = ux >= g.n[0] ? -g.L[0] : ux < 0 ? g.L[0] : 0.0;
+		const double sy = uy >= g.n[1] ? -g.L[1] : uy < 0 ? g.L[1] : 0.0;
+		const double sz = uz >= g.n[2] ? -g.L[2] : uz < 0 ? g.L[2] : 0.0;
+		const int c2 = tx + ty * g.n[0] + tz * g.n[0] * g.n[1];
+		const int rule = g.off[o][3];
+		const bool same = c2 == c;
+		const int js = start[c2], je = start[c2 + 1];
+		for (int base = js; base < je; base += SEPK_SUB) {
+			const int j = base + sub;
+			bool take = j < je;
+			if (same && take) {
+				if (rule == SEPDEV_RULE_NE) {
+					take = j != t;
+				} else {
+					const int oj = s.orig[j];
+					take = rule == SEPDEV_RULE_GT ? oj > oi : oj < oi;
+				}
+			}
+			if (take && g.opt == 3 /* SEP_EXCL_SAME_MOL */)
+				take = mi == -1 || mi != s.mol[j];
+			if (take) {
+				const sepk_pos pj = s.pos[j];
+				double dx, dy, dz;
+				if (SHIFT) {
+					dx = (pi.x - pj.x) + sx;
+					dy = (pi.y - pj.y) + sy;
+					dz = (pi.z - pj.z) + sz;
+				} else {
+					dx = sepk_wrap(pi.x - pj.x, g.L[0]);
+					dy = sepk_wrap(pi.y - pj.y, g.L[1]);
+					dz = sepk_wrap(pi.z - pj.z, g.L[2]);
+				}
+				const double r2 = dx * dx + dy * dy + dz * dz;
+				take = r2 < g.rlist2;
+			}
+			if (take && g.opt == 2 /* SEP_EXCL_BONDED */) {
+				const int oj = s.orig[j];
+				int sh = sepk_share(s.partners, npad, t, j, 0, 10, oi, oj);
+				if (g.bonded_all)
+					sh += sepk_share(s.partners, npad, t, j, 10, 10, oi, oj) +
+					      sepk_share(s.partners, npad, t, j, 20, 20, oi, oj);
+				// rule 2 is the Coulomb brute-force test of
+				// source/sepcoulomb.c:44, which excludes only when
+				// exactly one kind of partnership holds
+				take = g.bonded_all == 2 ? sh != 1 : sh == 0;
+			}
+			const unsigned long long vote = __ballot(take);
+			const unsigned long long mine = (vote >> gshift) & ((1ull << SEPK_SUB) - 1ull);
+			if (take) {
+				const int k = m + __popcll(mine & below);
+				if (k < cap)
+					nbr[sepk_slot(k / SEPK_SUB, t, npad, k % SEPK_SUB)] = j;
+			}
+			m += __popcll(mine);
+		}
 	}
-	return 0;
+	m_out = m;
 }
 
+// One group of SEPK_SUB lanes per atom; lane 0 also copies the atom's
+// permuted state home.
 __global__ __launch_bounds__(SEPK_BLOCK) void k_build(
-	sepk_state s, const int *cellid, const int *start, sepdev_grid g,
-	long n, long npad, int cap, int *nbr, int *cnt, double *partial,
-	unsigned *counter, sepdev_sys *sys, int conditional)
+	sepk_state s, sepk_state home, const int *cellid, const int *start, sepdev_grid g,
+	long n, long npad, int cap, int *nbr, int *cnt, double *partial, long rows,
+	sepdev_sys *sys, int conditional, int has_partners)
 {
 	if (conditional && sys->neighb_flag == 0)
 		return;
-	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long lb = sepk_xcd_block();
+	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
+	const int sub = threadIdx.x % SEPK_SUB;
+	const int gshift = (threadIdx.x & (SEPK_WAVE - 1)) & ~(SEPK_SUB - 1);
+	const bool shift = g.shift_ok && sys->unclean == 0;
 	double mx = 0.0;
 	if (t < n) {
-		const int c = cellid[t];
-		const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
-		const sepk_pos pi = s.pos[t];
-		const int oi = s.orig[t];
-		const int mi = s.mol[t];
-		int m = 0;
-		for (int o = 0; o < g.noff; o++) {
-			const int tx = sepk_mod(cx + g.off[o][0], g.n[0]);
-			const int ty = sepk_mod(cy + g.off[o][1], g.n[1]);
-			const int tz = sepk_mod(cz + g.off[o][2], g.n[2]);
-			const int c2 = tx + ty * g.n[0] + tz * g.n[0] * g.n[1];
-			const int rule = g.off[o][3];
-			const bool same = c2 == c;
-			const int js = start[c2], je = start[c2 + 1];
-			for (int j = js; j < je; j++) {
-				if (same) {
-					if (rule == SEPDEV_RULE_NE) {
-						if (j == t)
-							continue;
-					} else {
-						const int oj = s.orig[j];
-						if (rule == SEPDEV_RULE_GT ? oj <= oi : oj >= oi)
-							continue;
-					}
-				}
-				if (g.opt == 3 /* SEP_EXCL_SAME_MOL */ && mi != -1 && mi == s.mol[j])
-					continue;
-				const sepk_pos pj = s.pos[j];
-				const double dx = sepk_wrap(pi.x - pj.x, g.L[0]);
-				const double dy = sepk_wrap(pi.y - pj.y, g.L[1]);
-				const double dz = sepk_wrap(pi.z - pj.z, g.L[2]);
-				const double r2 = dx * dx + dy * dy + dz * dz;
-				if (!(r2 < g.rlist2))
-					continue;
-				if (g.opt == 2 /* SEP_EXCL_BONDED */) {
-					const int oj = s.orig[j];
-					int sh = sepk_share(s.partners, npad, t, j, 0, 10, oi, oj);
-					if (g.bonded_all)
-						sh += sepk_share(s.partners, npad, t, j, 10, 10, oi, oj) +
-						      sepk_share(s.partners, npad, t, j, 20, 20, oi, oj);
-					// rule 2 is the Coulomb brute-force test of
-					// source/sepcoulomb.c:44, which excludes only
-					// when exactly one kind of partnership holds
-					if (g.bonded_all == 2 ? sh == 1 : sh != 0)
-						continue;
-				}
-				if (m < cap)
-					nbr[(long)m * npad + t] = j;
-				m++;
-			}
-		}
+		if (sub == 0)
+			sepk_copy_home(s, home, t, npad, has_partners);
+		int m;
+		if (shift)
+			sepk_build_atom<true>(s, t, sub, gshift, cellid, start, g, npad, cap, nbr, m);
+		else
+			sepk_build_atom<false>(s, t, sub, gshift, cellid, start, g, npad, cap, nbr, m);
 		if (m > cap)
 			sepk_set_error(sys, SEP_GPU_ENEIGHB);
-		cnt[t] = m < cap ? m : cap;
+		if (sub == 0)
+			cnt[t] = m < cap ? m : cap;
 		mx = (double)m;
 	}
-	double mxb = sepk_block_max(mx, partial + blockIdx.x);
-	(void)mxb;
-	if (sepk_last_block(counter)) {
-		if (threadIdx.x == 0) {
-			double best = 0.0;
-			for (unsigned b = 0; b < gridDim.x; b++)
-				best = fmax(best, sepk_ld(partial + b));
-			sys->max_full = (int)best;
-			sys->neighb_flag = 0;
-			sys->xn_pending = 0;
-			*counter = 0;
-		}
-	}
+	double row[1] = {mx};
+	sepk_block_partial<1>(row, 1u, partial, lb, rows);
 }
 
 // ---- host side --------------------------------------------------------------
@@ -321,26 +357,26 @@ extern "C" int sepdev_reserve_list(sepdev *d, int cap)
 {
 	if (d->capturing)
 		return sepk_fail(SEP_GPU_ESTATE, "list resize during graph capture");
-	cap = (cap + 31) / 32 * 32;
-	if (d->nbr && cap <= d->cap)
+	int lcap = (cap + SEPK_SUB - 1) / SEPK_SUB;
+	lcap = (lcap + 7) / 8 * 8;
+	if (d->nbr && lcap <= d->lcap)
 		return 0;
 	SEPK_CHECK(hipStreamSynchronize(d->st));
 	if (d->nbr)
 		(void)hipFree(d->nbr);
 	d->nbr = nullptr;
-	SEPK_CHECK(hipMalloc(&d->nbr, (size_t)cap * d->npad * sizeof(int)));
+	SEPK_CHECK(hipMalloc(&d->nbr, (size_t)lcap * SEPK_SUB * d->npad * sizeof(int)));
 	SEPK_CHECK(hipMemsetAsync(d->nbr_count, 0, d->npad * sizeof(int), d->st));
-	d->cap = cap;
+	d->lcap = lcap;
+	d->cap = lcap * SEPK_SUB;
 	d->list_built = 0;
 	return 0;
 }
 
-__global__ void k_sys_reset_max(sepdev_sys *sys) { sys->max_dist2 = 0.0; }
-
 extern "C" int sepdev_reset_max_dist(sepdev *d)
 {
-	k_sys_reset_max<<<1, 1, 0, d->st>>>(d->sys);
-	SEPK_LAUNCH_CHECK();
+	// owed to the next leapfrog, which then starts its maximum at zero
+	d->max_reset_pending = 1;
 	return 0;
 }
 
@@ -359,6 +395,9 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 		return sepk_fail(SEP_GPU_EARG, "build_list: %ld cells", ncell);
 	if (g.opt == 2 && !d->s[0].partners)
 		return sepk_fail(SEP_GPU_ESTATE, "SEP_EXCL_BONDED needs bonded partners");
+	int rc = sepk_flush_nh(d);
+	if (rc)
+		return rc;
 
 	// size buffers on the host side of any capture
 	if (ncell > d->ncell_cap) {
@@ -384,30 +423,53 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 		long cap = (long)(2.0 * expect) + 64;
 		if (cap > d->n * 4 + 64)
 			cap = d->n * 4 + 64;
-		cap = (cap + 31) / 32 * 32;
 		if (d->capturing)
 			return sepk_fail(SEP_GPU_ESTATE, "list allocated during graph capture");
-		SEPK_CHECK(hipMalloc(&d->nbr, (size_t)cap * d->npad * sizeof(int)));
-		d->cap = (int)cap;
+		if ((rc = sepdev_reserve_list(d, (int)cap)))
+			return rc;
 	}
+	if ((rc = sepk_claim_region(d, SEPK_R_BUILD)))
+		return rc;
 
-	const unsigned nb = sepk_blocks(d->n);
-	sepk_state &src = d->s[0]; // current state, fixed addresses
-	sepk_state &dst = d->s[1]; // spare: receives the permuted order
-	k_bin<<<nb, SEPK_BLOCK, 0, d->st>>>(src.pos, d->n, g, d->cellid_old, d->cell_count, d->sys, conditional);
-	k_scan<<<1, 1024, 0, d->st>>>(d->cell_count, d->cell_start, d->cell_fill, ncell, d->sys, conditional);
-	k_scatter<<<nb, SEPK_BLOCK, 0, d->st>>>(d->cellid_old, d->cell_start, d->cell_fill, d->order_tmp, d->n, d->sys, conditional);
-	k_cell_sort<<<(unsigned)((ncell * 64 + SEPK_BLOCK - 1) / SEPK_BLOCK), SEPK_BLOCK, 0, d->st>>>(
-		d->order_tmp, d->cell_start, src.orig, d->neworder, d->cellid_new, ncell, d->sys, conditional);
-	k_permute<<<nb, SEPK_BLOCK, 0, d->st>>>(src, dst, d->neworder, d->iperm, d->n, d->npad, d->sys,
-						 conditional, src.partners != nullptr);
-	k_copy_back<<<nb, SEPK_BLOCK, 0, d->st>>>(dst, src, d->n, d->npad, d->sys, conditional,
-						   src.partners != nullptr);
+	sepk_state &cur = d->s[0];   // current state, fixed addresses
+	sepk_state &spare = d->s[1]; // receives the permuted order
+	const int hp = cur.partners != nullptr;
+	unsigned nb = sepk_blocks(d->n);
+	if (nb > SEPK_GRID_CAP)
+		nb = SEPK_GRID_CAP;
+	long sb = (ncell * 64 + SEPK_BLOCK - 1) / SEPK_BLOCK;
+	if (sb > SEPK_GRID_CAP)
+		sb = SEPK_GRID_CAP;
+	k_bin<<<nb, SEPK_BLOCK, 0, d->st>>>(cur.pos, d->n, g, d->cellid_old, d->cell_count, d->sys,
+					    conditional);
+	k_scan<<<1, 1024, 0, d->st>>>(d->cell_count, d->cell_start, d->cell_fill, ncell, d->sys,
+				      conditional);
+	k_scatter<<<nb, SEPK_BLOCK, 0, d->st>>>(d->cellid_old, d->cell_start, d->cell_fill,
+						d->order_tmp, d->n, d->sys, conditional);
+	k_sort_permute<<<(unsigned)sb, SEPK_BLOCK, 0, d->st>>>(
+		d->order_tmp, d->cell_start, cur, spare, d->iperm, d->cellid_new, ncell, d->npad,
+		d->sys, conditional, hp);
 	SEPK_LAUNCH_CHECK();
-	k_build<<<nb, SEPK_BLOCK, 0, d->st>>>(src, d->cellid_new, d->cell_start, g, d->n, d->npad, d->cap,
-					      d->nbr, d->nbr_count, d->partials, d->counters + SEPK_CTR_MISC,
-					      d->sys, conditional);
+	const unsigned bb = sepk_blocks_xcd(d->n * SEPK_SUB);
+	k_build<<<bb, SEPK_BLOCK, 0, d->st>>>(spare, cur, d->cellid_new, d->cell_start, g, d->n,
+					      d->npad, d->cap, d->nbr, d->nbr_count,
+					      sepk_region(d, SEPK_R_BUILD), d->region_rows, d->sys,
+					      conditional, hp);
 	SEPK_LAUNCH_CHECK();
+	// largest list; clears the rebuild flag and the owed xn reset
+	sepk_task f;
+	memset(&f, 0, sizeof f);
+	f.mode = SEPK_FIN_BUILD;
+	f.region = SEPK_R_BUILD;
+	f.nblk = (int)bb;
+	f.nv = 1;
+	f.maxmask = 1u;
+	f.conditional = conditional;
+	if ((rc = sepk_queue_task(d, f)))
+		return rc;
+	// an unconditional build is finished at once, so the host can check it
+	if (!conditional && (rc = sepk_flush_tasks(d)))
+		return rc;
 	d->grid = g;
 	d->list_built = 1;
 	return 0;

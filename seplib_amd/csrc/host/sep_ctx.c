@@ -515,10 +515,29 @@ void seph_grid_reference(const sepsys *sys, unsigned opt, sepdev_grid *g)
 	g->rlist2 = rl * rl;
 	g->opt = (int)opt;
 	g->bonded_all = 1;
-	/* offset 0 in both directions = every other atom of the home cell;
-	 * each further half-shell offset forward (j > i when it wraps onto the
-	 * home cell) and backward (j < i) */
 	int o = 0;
+	if (g->n[0] >= 3 && g->n[1] >= 3 && g->n[2] >= 3) {
+		/* No half-shell offset wraps onto its own cell and every
+		 * (offset, -offset) pair reaches two distinct cells, so the
+		 * reference's pairs seen from both ends are exactly the 27-cell
+		 * stencil with j != i in the home cell.  Visit it in ascending
+		 * (z, y, x) order: lists then come out in cell order. */
+		for (int dz = -1; dz <= 1; dz++)
+			for (int dy = -1; dy <= 1; dy++)
+				for (int dx = -1; dx <= 1; dx++) {
+					g->off[o][0] = dx;
+					g->off[o][1] = dy;
+					g->off[o][2] = dz;
+					g->off[o++][3] = SEPDEV_RULE_NE;
+				}
+		g->noff = o;
+		g->shift_ok = g->n[0] >= 5 && g->n[1] >= 5 && g->n[2] >= 5;
+		return;
+	}
+	/* Fewer than three cells on some axis: offsets can alias.  Offset 0 in
+	 * both directions = every other atom of the home cell; each further
+	 * half-shell offset forward (j > i when it wraps onto the home cell)
+	 * and backward (j < i), reproducing the reference's duplicates. */
 	g->off[o][0] = g->off[o][1] = g->off[o][2] = 0;
 	g->off[o++][3] = SEPDEV_RULE_NE;
 	for (int s = 1; s < 14; s++) {
@@ -550,6 +569,7 @@ void seph_grid_own(const sepsys *sys, double rlist, unsigned opt, int bonded_rul
 	g->rlist2 = rlist * rlist;
 	g->opt = (int)opt;
 	g->bonded_all = bonded_rule;
+	g->shift_ok = g->n[0] >= 5 && g->n[1] >= 5 && g->n[2] >= 5;
 	int o = 0;
 	for (int dz = -span[2]; dz <= span[2]; dz++)
 		for (int dy = -span[1]; dy <= span[1]; dy++)

@@ -15,6 +15,8 @@ extern "C" {
 #endif
 
 #define SEPDEV_OK 0
+/* lanes per atom in the list kernels; list capacities are multiples */
+#define SEPDEV_LANES 4
 
 typedef struct sepdev sepdev;
 
@@ -39,7 +41,7 @@ typedef struct {
 	unsigned xn_pending; /* xn = x and cross_neighb = 0 still owed */
 	int error;           /* sticky device error code (SEP_GPU_*)   */
 	int max_full;        /* largest full-list length of the last build */
-	int pad;
+	int unclean;         /* the last binning saw an atom outside [0, n*l) */
 	double sumv2;        /* last sum m v^2 (Nose-Hoover)           */
 } sepdev_sys;
 
@@ -56,8 +58,9 @@ typedef struct {
 	double rlist2;   /* list cutoff squared                             */
 	int noff;
 	int opt;         /* SEP_ALL / SEP_EXCL_BONDED / SEP_EXCL_SAME_MOL    */
-	int bonded_all;  /* EXCL_BONDED: 1 bonds+angles+dihedrals, 0 bonds  */
-	int pad;
+	int bonded_all;  /* EXCL_BONDED: 0 bonds only, 1 bonds+angles+dihedrals,
+			    2 exclude when exactly one kind is shared          */
+	int shift_ok;    /* >= 5 cells on every axis: per-cell image shifts  */
 	int off[SEPDEV_MAX_OFF][4]; /* dx, dy, dz, rule                     */
 } sepdev_grid;
 
