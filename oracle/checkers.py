@@ -107,6 +107,7 @@ class OracleAPI:
         f("orc_torsion_ryckaert", None, S.P_ATOMS, C.c_int, S.P_D, S.P_SYS, S.P_RET)
         f("orc_read_topology", C.c_int, S.P_ATOMS, C.c_char_p, S.P_SYS)
         f("orc_list_len", C.c_int, S.P_ATOMS, C.c_long)
+        f("orc_half_pairs", C.c_long, S.P_ATOMS, C.c_long, C.c_void_p, C.c_long)
 
     def alloc(self, n, nneighb):
         return self.lib.orc_alloc(n, nneighb)

@@ -372,6 +372,25 @@ int orc_build_list(seppart *p, sepsys *s, unsigned opt, int nneighb)
 	return list_from_cells(p, s, opt, nneighb);
 }
 
+/* Unordered pairs (min, max) of the half lists p[i].neighb, written as
+ * 2 ints per pair to out (room for maxpairs); returns the pair count, or -1
+ * if out is too small.  Reads any AoS with the seppart layout. */
+long orc_half_pairs(const seppart *p, long npart, int *out, long maxpairs)
+{
+	long c = 0;
+	for (long i = 0; i < npart; i++) {
+		for (int k = 0; p[i].neighb[k] != -1; k++) {
+			const int j = p[i].neighb[k];
+			if (c >= maxpairs)
+				return -1;
+			out[2 * c] = (int)(i < j ? i : j);
+			out[2 * c + 1] = (int)(i < j ? j : i);
+			c++;
+		}
+	}
+	return c;
+}
+
 int orc_list_len(const seppart *p, long i)
 {
 	int n = 0;

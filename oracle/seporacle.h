@@ -67,5 +67,6 @@ void orc_leapfrog(seppart *p, sepsys *s, sepret *r);
 
 /* half-list helpers for tests: number of entries of atom i's list */
 int orc_list_len(const seppart *p, long i);
+long orc_half_pairs(const seppart *p, long npart, int *out, long maxpairs);
 
 #endif

@@ -88,6 +88,14 @@ __device__ __forceinline__ double sepk_image(double d, double L, double iL)
 	return FAST ? sepk_wrap_fast(d, L, iL) : sepk_wrap(d, L);
 }
 
+// Entries of one lane are taken SEPK_BATCH at a time: every index and
+// every neighbour position of the batch is loaded before any is used, so a
+// wave keeps SEPK_BATCH gathers in flight instead of one.  Slots past the
+// lane's last entry are clamped to that entry (always a valid atom) and
+// masked out, so the loads need no branches; the accumulation order is the
+// list order, as in a one-at-a-time loop.
+#define SEPK_BATCH 4
+
 template <bool TYPED, bool MIXED, bool OVERWRITE, bool FAST>
 __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt,
@@ -106,45 +114,51 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 	}
 	if (act) {
 		const sepk_pos pi = s.pos[t];
-		const int nc = cnt[t];
-		for (int e = 0; e * SEPK_SUB + sub < nc; e++) {
-			const int j = nbr[sepk_slot(e, t, npad, sub)];
-			if (TYPED) {
-				const signed char tj = s.type[j];
-				if (!((ti == c.t0 && tj == c.t1) || (ti == c.t1 && tj == c.t0)))
-					continue;
+		// entries sub, sub + SEPK_SUB, ... of the atom belong to this lane
+		const int ne = (cnt[t] - sub + SEPK_SUB - 1) / SEPK_SUB;
+		for (int e0 = 0; e0 < ne; e0 += SEPK_BATCH) {
+			int j[SEPK_BATCH];
+#pragma unroll
+			for (int u = 0; u < SEPK_BATCH; u++)
+				j[u] = nbr[sepk_slot(min(e0 + u, ne - 1), t, npad, sub)];
+			sepk_pos pj[SEPK_BATCH];
+			signed char tj[SEPK_BATCH];
+#pragma unroll
+			for (int u = 0; u < SEPK_BATCH; u++) {
+				pj[u] = s.pos[j[u]];
+				if (TYPED)
+					tj[u] = s.type[j[u]];
 			}
-			const sepk_pos pj = s.pos[j];
-			const double dx = sepk_image<FAST>(pi.x - pj.x, c.L0, c.iL0);
-			const double dy = sepk_image<FAST>(pi.y - pj.y, c.L1, c.iL1);
-			const double dz = sepk_image<FAST>(pi.z - pj.z, c.L2, c.iL2);
-			const double r2 = dx * dx + dy * dy + dz * dz;
-			if (!(r2 < c.cf2))
-				continue;
-			if (MIXED) {
-				const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
-				const float q3 = q * q * q;
-				const double ft = (double)(c.eps48f * q3 * (q3 - c.awhf) * q);
+#pragma unroll
+			for (int u = 0; u < SEPK_BATCH; u++) {
+				if (e0 + u >= ne)
+					break;
+				if (TYPED && !((ti == c.t0 && tj[u] == c.t1) ||
+					       (ti == c.t1 && tj[u] == c.t0)))
+					continue;
+				const double dx = sepk_image<FAST>(pi.x - pj[u].x, c.L0, c.iL0);
+				const double dy = sepk_image<FAST>(pi.y - pj[u].y, c.L1, c.iL1);
+				const double dz = sepk_image<FAST>(pi.z - pj[u].z, c.L2, c.iL2);
+				const double r2 = dx * dx + dy * dy + dz * dz;
+				if (!(r2 < c.cf2))
+					continue;
+				double ft, e;
+				if (MIXED) {
+					const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
+					const float q3 = q * q * q;
+					ft = (double)(c.eps48f * q3 * (q3 - c.awhf) * q);
+					e = (double)(c.eps4f * q3 * (q3 - c.awf) + c.ushiftf);
+				} else {
+					const double q = c.sig2 / r2;
+					const double q3 = q * q * q;
+					ft = c.eps48 * q3 * (q3 - c.awh) * q;
+					e = c.eps4 * q3 * (q3 - c.aw) + c.ushift;
+				}
 				const double gx = ft * dx, gy = ft * dy, gz = ft * dz;
 				fx += gx;
 				fy += gy;
 				fz += gz;
-				acc[0] += (double)(c.eps4f * q3 * (q3 - c.awf) + c.ushiftf);
-				acc[1] += gx * dx;
-				acc[2] += gx * dy;
-				acc[3] += gx * dz;
-				acc[4] += gy * dy;
-				acc[5] += gy * dz;
-				acc[6] += gz * dz;
-			} else {
-				const double q = c.sig2 / r2;
-				const double q3 = q * q * q;
-				const double ft = c.eps48 * q3 * (q3 - c.awh) * q;
-				const double gx = ft * dx, gy = ft * dy, gz = ft * dz;
-				fx += gx;
-				fy += gy;
-				fz += gz;
-				acc[0] += c.eps4 * q3 * (q3 - c.aw) + c.ushift;
+				acc[0] += e;
 				acc[1] += gx * dx;
 				acc[2] += gx * dy;
 				acc[3] += gx * dz;
@@ -273,32 +287,43 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 		act = !(fabs(pi.q) < DBL_EPSILON);
 	}
 	if (act) {
-		const int nc = cnt[t];
-		for (int e = 0; e * SEPK_SUB + sub < nc; e++) {
-			const int j = nbr[sepk_slot(e, t, npad, sub)];
-			const sepk_pos pj = s.pos[j];
-			if (fabs(pj.q) < DBL_EPSILON)
-				continue;
-			const double dx = sepk_image<FAST>(pi.x - pj.x, c.L0, c.iL0);
-			const double dy = sepk_image<FAST>(pi.y - pj.y, c.L1, c.iL1);
-			const double dz = sepk_image<FAST>(pi.z - pj.z, c.L2, c.iL2);
-			const double r2 = dx * dx + dy * dy + dz * dz;
-			if (!(r2 < c.cf2))
-				continue;
-			const double zz = pj.q * pi.q;
-			const double r = sqrt(r2);
-			const double ft = zz * (1.0 / r2 - c.icf2) / r;
-			const double gx = ft * dx, gy = ft * dy, gz = ft * dz;
-			fx += gx;
-			fy += gy;
-			fz += gz;
-			acc[0] += zz * (1.0 / r + (r - c.cf) * c.icf2 - c.icf);
-			acc[1] += gx * dx;
-			acc[2] += gx * dy;
-			acc[3] += gx * dz;
-			acc[4] += gy * dy;
-			acc[5] += gy * dz;
-			acc[6] += gz * dz;
+		const int ne = (cnt[t] - sub + SEPK_SUB - 1) / SEPK_SUB;
+		for (int e0 = 0; e0 < ne; e0 += SEPK_BATCH) {
+			int j[SEPK_BATCH];
+#pragma unroll
+			for (int u = 0; u < SEPK_BATCH; u++)
+				j[u] = nbr[sepk_slot(min(e0 + u, ne - 1), t, npad, sub)];
+			sepk_pos pj[SEPK_BATCH];
+#pragma unroll
+			for (int u = 0; u < SEPK_BATCH; u++)
+				pj[u] = s.pos[j[u]];
+#pragma unroll
+			for (int u = 0; u < SEPK_BATCH; u++) {
+				if (e0 + u >= ne)
+					break;
+				if (fabs(pj[u].q) < DBL_EPSILON)
+					continue;
+				const double dx = sepk_image<FAST>(pi.x - pj[u].x, c.L0, c.iL0);
+				const double dy = sepk_image<FAST>(pi.y - pj[u].y, c.L1, c.iL1);
+				const double dz = sepk_image<FAST>(pi.z - pj[u].z, c.L2, c.iL2);
+				const double r2 = dx * dx + dy * dy + dz * dz;
+				if (!(r2 < c.cf2))
+					continue;
+				const double zz = pj[u].q * pi.q;
+				const double r = sqrt(r2);
+				const double ft = zz * (1.0 / r2 - c.icf2) / r;
+				const double gx = ft * dx, gy = ft * dy, gz = ft * dz;
+				fx += gx;
+				fy += gy;
+				fz += gz;
+				acc[0] += zz * (1.0 / r + (r - c.cf) * c.icf2 - c.icf);
+				acc[1] += gx * dx;
+				acc[2] += gx * dy;
+				acc[3] += gx * dz;
+				acc[4] += gy * dy;
+				acc[5] += gy * dz;
+				acc[6] += gz * dz;
+			}
 		}
 	}
 	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, acc, s.f, partial, lb,

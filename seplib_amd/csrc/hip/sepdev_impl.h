@@ -49,6 +49,8 @@ enum { SEPK_FIN_PAIR = 0, SEPK_FIN_BONDED = 1, SEPK_FIN_NH = 2, SEPK_FIN_LF = 3,
 enum { SEPK_R_BUILD = 0, SEPK_R_PAIR = 1, SEPK_R_LF = 2, SEPK_R_MISC = 3, SEPK_NREGION = 4 };
 #define SEPK_MAXNV 10
 #define SEPK_MAXTASK 4
+// workgroups of the finisher; the last one to finish combines their rows
+#define SEPK_FIN_BLOCKS 64
 
 struct sepk_task {
 	int mode, nblk, nv, region;
@@ -71,6 +73,8 @@ struct sepk_fin {
 	int ntask;
 	long region_words, region_rows;
 	sepdev_sys *sys;
+	double *l2;       // SEPK_MAXTASK x SEPK_MAXNV x SEPK_FIN_BLOCKS block results
+	unsigned *count;  // arrivals of the current launch; 0 between launches
 };
 
 struct sepk_timer {
@@ -90,7 +94,7 @@ struct sepdev {
 	// cell binning
 	long ncell_cap;
 	int *cell_count, *cell_start, *cell_fill;
-	int *cellid_old, *cellid_new, *order_tmp, *neworder;
+	int *cellid_old, *order_tmp, *neworder;
 
 	// full neighbour list: entry k of atom t at slot
 	// ((k/SEPK_SUB)*npad + t)*SEPK_SUB + k%SEPK_SUB, nbr_count[t] entries,
@@ -107,6 +111,8 @@ struct sepdev {
 	long partials_words, region_words, region_rows;
 	sepk_task pending[SEPK_MAXTASK];
 	int npending;
+	double *fin_l2;
+	unsigned *fin_count;
 
 	// return and scalar slots
 	double *ret;

@@ -9,9 +9,9 @@
 //                  index (so the device order and every list are independent
 //                  of atomic timing) and move each member's state to its new
 //                  cell-sorted index in the spare copy
-//   k_build        SEPK_SUB lanes per atom: copy the atom's state home and
-//                  scan the stencil cells' contiguous segments, appending
-//                  neighbours within the list cutoff to the lane's own slots
+//   k_build        one workgroup per cell: copy its members' state home,
+//                  stage the stencil cells' segments in LDS and append the
+//                  candidates within the list cutoff, one wave per atom
 // and a BUILD task for the step's finisher.  With conditional = 1 every
 // kernel returns at once unless the device rebuild flag is set, so the
 // sequence can sit in a captured graph.
@@ -68,6 +68,9 @@ __global__ void k_bin(const sepk_pos *pos, long n, sepdev_grid g, int *cellid,
 {
 	if (conditional && sys->neighb_flag == 0)
 		return;
+	// the build's atomic maximum starts here
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		sys->max_full = 0;
 	for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
 	     t += (long)gridDim.x * blockDim.x) {
 		const sepk_pos p = pos[t];
@@ -150,7 +153,7 @@ __global__ void k_scatter(const int *cellid, const int *start, int *fill,
 // One wave per cell (grid-stride over cells): rank each member by its
 // original index and move its state to start + rank in the spare copy.
 __global__ void k_sort_permute(const int *order, const int *start, sepk_state src,
-			       sepk_state dst, int *iperm, int *cellid_new, long ncell,
+			       sepk_state dst, int *iperm, long ncell,
 			       long npad, sepdev_sys *sys, int conditional,
 			       int has_partners)
 {
@@ -168,7 +171,6 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 			for (int j = s; j < e; j++)
 				rank += src.orig[order[j]] < key;
 			const long t = s + rank;
-			cellid_new[t] = (int)c;
 			const sepk_pos p = src.pos[o];
 			dst.pos[t] = p;
 			const double xyz[3] = {p.x, p.y, p.z};
@@ -230,125 +232,284 @@ __device__ __forceinline__ void sepk_copy_home(const sepk_state &src, const sepk
 			dst.partners[k * npad + t] = src.partners[k * npad + t];
 }
 
-// SEPK_SUB lanes per atom scan each stencil cell's contiguous segment
-// together, SEPK_SUB consecutive candidates per round; a wave ballot gives
-// every accepting lane its position, so each atom's list comes out in
-// stencil-cell order and, within a cell, in device order -- the same order
-// whatever the lane count.  Entry k goes to lane k % SEPK_SUB's slots.
+// One workgroup per home cell at a time (cells split into one contiguous
+// range per XCD, walked with a stride of the range's workgroup count).
+// The stencil cells' segments are staged in LDS as SoA chunks of
+// SEPK_BCHUNK candidates (position, device index, stencil offset, molecule)
+// and the cell's members ("home" atoms, SEPK_BHOME at a time) beside them.
+// Each wave owns every fourth home atom: it takes 64 staged candidates into
+// registers and tests them against each of its atoms, whose position is an
+// LDS broadcast; a wave ballot gives every accepted candidate its place in
+// that atom's list.  Lists therefore come out in stencil-offset order and,
+// within a cell, in device order.  Entry k of atom t goes to slot
+// sepk_slot(k / SEPK_SUB, t, npad, k % SEPK_SUB).
 //
 // SHIFT: with at least 5 cells on every axis and every atom inside its own
 // cell, |x_i - x_j| < 2l <= 0.4 L for a target cell reached without
 // crossing the box and > L - 2l >= 0.6 L for one reached across it, so the
 // reference's single-image wrap (subtract or add L exactly when |d| > L/2)
 // is the constant shift of the target cell; d + shift rounds exactly like
-// its d - L / d + L.  Otherwise the wrap is evaluated per candidate.
-template <bool SHIFT>
-__device__ __forceinline__ void sepk_build_atom(
-	const sepk_state &s, long t, int sub, int gshift, const int *cellid,
-	const int *start, const sepdev_grid &g, long npad, int cap, int *nbr, int &m_out)
+// its d - L / d + L, and d + 0 is d.  Otherwise the wrap is evaluated per
+// candidate.
+#define SEPK_BT 256      // threads of a build workgroup (4 waves)
+#define SEPK_BCHUNK 768  // candidates staged at a time
+#define SEPK_BHOME 64    // home atoms staged at a time
+#define SEPK_BGRID 1280  // build workgroups at most (a multiple of 8)
+
+struct sepk_build_lds {
+	double cx[SEPK_BCHUNK], cy[SEPK_BCHUNK], cz[SEPK_BCHUNK];
+	int cj[SEPK_BCHUNK], cm[SEPK_BCHUNK];
+	unsigned char co[SEPK_BCHUNK];
+	double hx[SEPK_BHOME], hy[SEPK_BHOME], hz[SEPK_BHOME];
+	int horig[SEPK_BHOME], hmol[SEPK_BHOME];
+	int segjs[SEPDEV_MAX_OFF], segpre[SEPDEV_MAX_OFF + 1];
+	int segrule[SEPDEV_MAX_OFF]; // rule when the target is the home cell, else -1
+	double segsh[SEPDEV_MAX_OFF][3];
+	int hs, h, anyshift;
+};
+
+// Tests the 64 staged candidates [q0, q0 + 64) of a chunk of kn against
+// the wave's home atoms i = wid, wid + 4, ... < hn.  The wave's running
+// list lengths live in mreg, lane kk holding that of its kk-th atom.
+// SH: some stencil target of the cell lies across the box (shift adds).
+// EXCL: 0 = SEP_ALL, 2 = SEP_EXCL_BONDED, 3 = SEP_EXCL_SAME_MOL.
+template <bool SHIFT, bool SH, int EXCL>
+__device__ __forceinline__ void sepk_build_chunk(
+	const sepk_build_lds &L, const sepk_state &s, const sepdev_grid &g, int q0, int kn,
+	long tbase, int hn, long npad, int cap, int *nbr, int &mreg)
 {
-	const unsigned long long below = (1ull << sub) - 1ull;
-	const int c = cellid[t];
-	const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
-	const sepk_pos pi = s.pos[t];
-	const int oi = s.orig[t];
-	const int mi = s.mol[t];
-	int m = 0; // entries so far; equal in the atom's lanes
-	for (int o = 0; o < g.noff; o++) {
-		const int ux = cx + g.off[o][0], uy = cy + g.off[o][1], uz = cz + g.off[o][2];
-		const int tx = sepk_mod(ux, g.n[0]);
-		const int ty = sepk_mod(uy, g.n[1]);
-		const int tz = sepk_mod(uz, g.n[2]);
-		const double sx = ux >= g.n[0] ? -g.L[0] : ux < 0 ? g.L[0] : 0.0;
-		const double sy = uy >= g.n[1] ? -g.L[1] : uy < 0 ? g.L[1] : 0.0;
-		const double sz = uz >= g.n[2] ? -g.L[2] : uz < 0 ? g.L[2] : 0.0;
-		const int c2 = tx + ty * g.n[0] + tz * g.n[0] * g.n[1];
-		const int rule = g.off[o][3];
-		const bool same = c2 == c;
-		const int js = start[c2], je = start[c2 + 1];
-		for (int base = js; base < je; base += SEPK_SUB) {
-			const int j = base + sub;
-			bool take = j < je;
-			if (same && take) {
-				if (rule == SEPDEV_RULE_NE) {
-					take = j != t;
-				} else {
-					const int oj = s.orig[j];
-					take = rule == SEPDEV_RULE_GT ? oj > oi : oj < oi;
-				}
+	const int lane = threadIdx.x & 63;
+	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
+	const int q = min(q0 + lane, kn - 1);
+	const bool valid = q0 + lane < kn;
+	const double px = L.cx[q], py = L.cy[q], pz = L.cz[q];
+	const int j = L.cj[q];
+	const int o = L.co[q];
+	const int rule = L.segrule[o];
+	const int mj = EXCL == 3 ? L.cm[q] : -1;
+	double sx = 0.0, sy = 0.0, sz = 0.0;
+	if (SH) {
+		sx = L.segsh[o][0];
+		sy = L.segsh[o][1];
+		sz = L.segsh[o][2];
+	}
+	const size_t stride = (size_t)npad * SEPK_SUB;
+	int kk = 0;
+	for (int i = wid; i < hn; i += SEPK_BT / 64, kk++) {
+		const long t = tbase + i;
+		bool take = valid;
+		if (SHIFT) {
+			// the 27-cell stencil reaches the home cell only through
+			// offset 0, whose rule is NE
+			take = take && !(rule >= 0 && j == t);
+		} else if (rule == SEPDEV_RULE_NE) {
+			take = take && j != t;
+		} else if (rule >= 0 && take) {
+			const int oj = s.orig[j];
+			const int oi = L.horig[i];
+			take = rule == SEPDEV_RULE_GT ? oj > oi : oj < oi;
+		}
+		if (EXCL == 3 /* SEP_EXCL_SAME_MOL */) {
+			const int mi = L.hmol[i];
+			take = take && (mi == -1 || mi != mj);
+		}
+		double dx, dy, dz;
+		if (SHIFT) {
+			dx = L.hx[i] - px;
+			dy = L.hy[i] - py;
+			dz = L.hz[i] - pz;
+			if (SH) {
+				dx += sx;
+				dy += sy;
+				dz += sz;
 			}
-			if (take && g.opt == 3 /* SEP_EXCL_SAME_MOL */)
-				take = mi == -1 || mi != s.mol[j];
+		} else {
+			dx = sepk_wrap(L.hx[i] - px, g.L[0]);
+			dy = sepk_wrap(L.hy[i] - py, g.L[1]);
+			dz = sepk_wrap(L.hz[i] - pz, g.L[2]);
+		}
+		const double r2 = dx * dx + dy * dy + dz * dz;
+		take = take && r2 < g.rlist2;
+		if (EXCL == 2 /* SEP_EXCL_BONDED */ && take) {
+			const int oi = L.horig[i];
+			const int oj = s.orig[j];
+			int sh = sepk_share(s.partners, npad, t, j, 0, 10, oi, oj);
+			if (g.bonded_all)
+				sh += sepk_share(s.partners, npad, t, j, 10, 10, oi, oj) +
+				      sepk_share(s.partners, npad, t, j, 20, 20, oi, oj);
+			// rule 2 is the Coulomb brute-force test of
+			// source/sepcoulomb.c:44, which excludes only when exactly
+			// one kind of partnership holds
+			take = g.bonded_all == 2 ? sh != 1 : sh == 0;
+		}
+		const unsigned long long vote = __ballot(take);
+		if (vote) {
+			const int m = __builtin_amdgcn_readlane(mreg, kk);
 			if (take) {
-				const sepk_pos pj = s.pos[j];
-				double dx, dy, dz;
-				if (SHIFT) {
-					dx = (pi.x - pj.x) + sx;
-					dy = (pi.y - pj.y) + sy;
-					dz = (pi.z - pj.z) + sz;
-				} else {
-					dx = sepk_wrap(pi.x - pj.x, g.L[0]);
-					dy = sepk_wrap(pi.y - pj.y, g.L[1]);
-					dz = sepk_wrap(pi.z - pj.z, g.L[2]);
-				}
-				const double r2 = dx * dx + dy * dy + dz * dz;
-				take = r2 < g.rlist2;
+				const unsigned below = __builtin_amdgcn_mbcnt_hi(
+					(unsigned)(vote >> 32),
+					__builtin_amdgcn_mbcnt_lo((unsigned)vote, 0u));
+				const unsigned e = (unsigned)m + below;
+				if (e < (unsigned)cap)
+					nbr[(size_t)t * SEPK_SUB + (size_t)(e >> 2) * stride + (e & 3u)] = j;
 			}
-			if (take && g.opt == 2 /* SEP_EXCL_BONDED */) {
-				const int oj = s.orig[j];
-				int sh = sepk_share(s.partners, npad, t, j, 0, 10, oi, oj);
-				if (g.bonded_all)
-					sh += sepk_share(s.partners, npad, t, j, 10, 10, oi, oj) +
-					      sepk_share(s.partners, npad, t, j, 20, 20, oi, oj);
-				// rule 2 is the Coulomb brute-force test of
-				// source/sepcoulomb.c:44, which excludes only when
-				// exactly one kind of partnership holds
-				take = g.bonded_all == 2 ? sh != 1 : sh == 0;
-			}
-			const unsigned long long vote = __ballot(take);
-			const unsigned long long mine = (vote >> gshift) & ((1ull << SEPK_SUB) - 1ull);
-			if (take) {
-				const int k = m + __popcll(mine & below);
-				if (k < cap)
-					nbr[sepk_slot(k / SEPK_SUB, t, npad, k % SEPK_SUB)] = j;
-			}
-			m += __popcll(mine);
+			if (lane == kk)
+				mreg = m + __popcll(vote);
 		}
 	}
-	m_out = m;
 }
 
-// One group of SEPK_SUB lanes per atom; lane 0 also copies the atom's
-// permuted state home.
-__global__ __launch_bounds__(SEPK_BLOCK) void k_build(
-	sepk_state s, sepk_state home, const int *cellid, const int *start, sepdev_grid g,
-	long n, long npad, int cap, int *nbr, int *cnt, double *partial, long rows,
-	sepdev_sys *sys, int conditional, int has_partners)
+template <bool SHIFT, int EXCL>
+__device__ __forceinline__ void sepk_build_cell(
+	sepk_build_lds &L, const sepk_state &s, long c, const int *start,
+	const sepdev_grid &g, long npad, int cap, int *nbr, int *cnt, sepdev_sys *sys,
+	int &mymax)
+{
+	const int lane = threadIdx.x & 63;
+	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
+	const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+	__syncthreads(); // the previous cell's readers are done with the LDS
+	if (wid == 0) {
+		// stencil segments, their prefix sums and whether any is shifted
+		int len = 0;
+		bool shifted = false;
+		if (lane < g.noff) {
+			const int o = lane;
+			const int cxi = (int)(c % n0), cyi = (int)((c / n0) % n1);
+			const int czi = (int)(c / ((long)n0 * n1));
+			const int ux = cxi + g.off[o][0], uy = cyi + g.off[o][1], uz = czi + g.off[o][2];
+			const int tx = sepk_mod(ux, n0), ty = sepk_mod(uy, n1), tz = sepk_mod(uz, n2);
+			const long c2 = tx + (long)ty * n0 + (long)tz * n0 * n1;
+			const double sx = ux >= n0 ? -g.L[0] : ux < 0 ? g.L[0] : 0.0;
+			const double sy = uy >= n1 ? -g.L[1] : uy < 0 ? g.L[1] : 0.0;
+			const double sz = uz >= n2 ? -g.L[2] : uz < 0 ? g.L[2] : 0.0;
+			L.segsh[o][0] = sx;
+			L.segsh[o][1] = sy;
+			L.segsh[o][2] = sz;
+			shifted = sx != 0.0 || sy != 0.0 || sz != 0.0;
+			const int js = start[c2];
+			L.segjs[o] = js;
+			len = start[c2 + 1] - js;
+			L.segrule[o] = c2 == c ? g.off[o][3] : -1;
+		}
+		int x = len; // inclusive scan over the segments
+		for (int d = 1; d < 64; d <<= 1) {
+			const int y = __shfl_up(x, d, 64);
+			if (lane >= d)
+				x += y;
+		}
+		if (lane == 0)
+			L.segpre[0] = 0;
+		if (lane < g.noff)
+			L.segpre[lane + 1] = x;
+		const unsigned long long sv = __ballot(shifted);
+		if (lane == 0) {
+			L.anyshift = sv != 0ull;
+			L.hs = start[c];
+			L.h = start[c + 1] - L.hs;
+		}
+	}
+	__syncthreads();
+	const int hs = L.hs, h = L.h, T = L.segpre[g.noff];
+	const bool anyshift = L.anyshift;
+	for (int hb = 0; hb < h; hb += SEPK_BHOME) {
+		const int hn = min(h - hb, SEPK_BHOME);
+		int mreg = 0;
+		for (int k0 = 0; k0 < T; k0 += SEPK_BCHUNK) {
+			const int kn = min(T - k0, SEPK_BCHUNK);
+			if (k0 > 0)
+				__syncthreads(); // the previous chunk is no longer read
+			// stage: wave w copies the segments w, w + 4, ... that overlap
+			// [k0, k0 + kn)
+			for (int o = wid; o < g.noff; o += SEPK_BT / 64) {
+				const int p0 = L.segpre[o], p1 = L.segpre[o + 1];
+				const int a = max(p0, k0), b = min(p1, k0 + kn);
+				for (int k = a + lane; k < b; k += 64) {
+					const int j = L.segjs[o] + (k - p0);
+					const sepk_pos p = s.pos[j];
+					const int q = k - k0;
+					L.cx[q] = p.x;
+					L.cy[q] = p.y;
+					L.cz[q] = p.z;
+					L.cj[q] = j;
+					L.co[q] = (unsigned char)o;
+					if (EXCL == 3)
+						L.cm[q] = s.mol[j];
+				}
+			}
+			if (k0 == 0)
+				for (int i = threadIdx.x; i < hn; i += SEPK_BT) {
+					const long t = hs + hb + i;
+					const sepk_pos p = s.pos[t];
+					L.hx[i] = p.x;
+					L.hy[i] = p.y;
+					L.hz[i] = p.z;
+					L.horig[i] = s.orig[t];
+					L.hmol[i] = EXCL == 3 ? s.mol[t] : -1;
+				}
+			__syncthreads();
+			for (int q0 = 0; q0 < kn; q0 += 64) {
+				if (SHIFT && anyshift)
+					sepk_build_chunk<SHIFT, true, EXCL>(L, s, g, q0, kn, hs + hb, hn, npad,
+									   cap, nbr, mreg);
+				else
+					sepk_build_chunk<SHIFT, false, EXCL>(L, s, g, q0, kn, hs + hb, hn,
+									    npad, cap, nbr, mreg);
+			}
+		}
+		// lane kk of wave w holds the length of home atom w + 4 kk
+		const int i = wid + (SEPK_BT / 64) * lane;
+		if (lane < SEPK_BHOME / (SEPK_BT / 64) && i < hn) {
+			cnt[hs + hb + i] = mreg < cap ? mreg : cap;
+			if (mreg > cap)
+				sepk_set_error(sys, SEP_GPU_ENEIGHB);
+			mymax = max(mymax, mreg);
+		}
+		if (hb + SEPK_BHOME < h)
+			__syncthreads(); // the next batch restages the candidates
+	}
+}
+
+// Persistent over cells; the largest list length goes to sys->max_full by an
+// atomic maximum (order-independent, so the result is deterministic).  The
+// permuted state of the workgroup's cells goes home at the end, in one
+// batch, so its memory round trip is paid once.
+template <int EXCL>
+__global__ __launch_bounds__(SEPK_BT) void k_build(
+	sepk_state s, sepk_state home, const int *start, sepdev_grid g, long ncell, long npad,
+	int cap, int *nbr, int *cnt, sepdev_sys *sys, int conditional, int has_partners)
 {
 	if (conditional && sys->neighb_flag == 0)
 		return;
-	const long lb = sepk_xcd_block();
-	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
-	const int sub = threadIdx.x % SEPK_SUB;
-	const int gshift = (threadIdx.x & (SEPK_WAVE - 1)) & ~(SEPK_SUB - 1);
+	__shared__ sepk_build_lds L;
 	const bool shift = g.shift_ok && sys->unclean == 0;
-	double mx = 0.0;
-	if (t < n) {
-		if (sub == 0)
-			sepk_copy_home(s, home, t, npad, has_partners);
-		int m;
+	const long per = gridDim.x / SEPK_NXCD;
+	const long xcd = blockIdx.x % SEPK_NXCD, k = blockIdx.x / SEPK_NXCD;
+	const long cper = (ncell + SEPK_NXCD - 1) / SEPK_NXCD;
+	const long lo = xcd * cper, hi = min(lo + cper, ncell);
+	int mymax = 0;
+	for (long c = lo + k; c < hi; c += per) {
 		if (shift)
-			sepk_build_atom<true>(s, t, sub, gshift, cellid, start, g, npad, cap, nbr, m);
+			sepk_build_cell<true, EXCL>(L, s, c, start, g, npad, cap, nbr, cnt, sys, mymax);
 		else
-			sepk_build_atom<false>(s, t, sub, gshift, cellid, start, g, npad, cap, nbr, m);
-		if (m > cap)
-			sepk_set_error(sys, SEP_GPU_ENEIGHB);
-		if (sub == 0)
-			cnt[t] = m < cap ? m : cap;
-		mx = (double)m;
+			sepk_build_cell<false, EXCL>(L, s, c, start, g, npad, cap, nbr, cnt, sys, mymax);
 	}
-	double row[1] = {mx};
-	sepk_block_partial<1>(row, 1u, partial, lb, rows);
+	for (long c = lo + k; c < hi; c += per) {
+		const int a = start[c], b = start[c + 1];
+		for (int t = a + threadIdx.x; t < b; t += SEPK_BT)
+			sepk_copy_home(s, home, t, npad, has_partners);
+	}
+	for (int o = 32; o > 0; o >>= 1)
+		mymax = max(mymax, __shfl_xor(mymax, o, 64));
+	__shared__ int bmax;
+	if (threadIdx.x == 0)
+		bmax = 0;
+	__syncthreads();
+	if ((threadIdx.x & 63) == 0)
+		atomicMax(&bmax, mymax);
+	__syncthreads();
+	if (threadIdx.x == 0 && bmax > 0)
+		atomicMax(&sys->max_full, bmax);
 }
 
 // ---- host side --------------------------------------------------------------
@@ -447,23 +608,22 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 	k_scatter<<<nb, SEPK_BLOCK, 0, d->st>>>(d->cellid_old, d->cell_start, d->cell_fill,
 						d->order_tmp, d->n, d->sys, conditional);
 	k_sort_permute<<<(unsigned)sb, SEPK_BLOCK, 0, d->st>>>(
-		d->order_tmp, d->cell_start, cur, spare, d->iperm, d->cellid_new, ncell, d->npad,
+		d->order_tmp, d->cell_start, cur, spare, d->iperm, ncell, d->npad,
 		d->sys, conditional, hp);
 	SEPK_LAUNCH_CHECK();
-	const unsigned bb = sepk_blocks_xcd(d->n * SEPK_SUB);
-	k_build<<<bb, SEPK_BLOCK, 0, d->st>>>(spare, cur, d->cellid_new, d->cell_start, g, d->n,
-					      d->npad, d->cap, d->nbr, d->nbr_count,
-					      sepk_region(d, SEPK_R_BUILD), d->region_rows, d->sys,
-					      conditional, hp);
+	long bb = (ncell + SEPK_NXCD - 1) / SEPK_NXCD * SEPK_NXCD;
+	if (bb > SEPK_BGRID)
+		bb = SEPK_BGRID;
+	auto kb = g.opt == 2 ? k_build<2> : g.opt == 3 ? k_build<3> : k_build<0>;
+	kb<<<(unsigned)bb, SEPK_BT, 0, d->st>>>(spare, cur, d->cell_start, g, ncell, d->npad,
+						      d->cap, d->nbr, d->nbr_count, d->sys,
+						      conditional, hp);
 	SEPK_LAUNCH_CHECK();
-	// largest list; clears the rebuild flag and the owed xn reset
+	// clears the rebuild flag and the owed xn reset
 	sepk_task f;
 	memset(&f, 0, sizeof f);
 	f.mode = SEPK_FIN_BUILD;
 	f.region = SEPK_R_BUILD;
-	f.nblk = (int)bb;
-	f.nv = 1;
-	f.maxmask = 1u;
 	f.conditional = conditional;
 	if ((rc = sepk_queue_task(d, f)))
 		return rc;

@@ -1,22 +1,22 @@
-// sepdev_reduce.hip -- the one-block finisher that turns per-block partial
+// sepdev_reduce.hip -- the finisher that turns per-block partial
 // rows into the energies, stress tensors, thermostat state and rebuild
 // bookkeeping of a step, and the deferred-work helpers around it.
 //
 // Kernels leave one row per block in a region of the partials buffer and
 // queue a task; queued tasks run together in one launch of k_finish at the
 // next flush (each leapfrog, each host read, the end of a graph capture).
-// Every reduction is fixed-order -- each thread sums a strided set of rows
-// in row order, then a shared-memory tree of fixed shape -- so repeated runs
-// give identical bits.
+// Every reduction is fixed-order, so repeated runs give identical bits.
 #include <string.h>
 
 #include "sepdev_impl.h"
 
-#define FIN_THREADS 1024
+#define FIN_THREADS 256
 
-__device__ static void fin_apply(const sepk_task &k, const double *tot, sepdev_sys *sys)
+// ret, alpha and sys point at the finisher's LDS copies of the task's
+// return slot, thermostat variable and the system scalars.
+__device__ static void fin_apply(const sepk_task &k, const double *tot, double *ret,
+				 double *alpha, sepdev_sys *sys)
 {
-	double *ret = k.ret;
 	if (ret && k.zero_first)
 		for (int w = 0; w < SEPDEV_RET_WORDS; w++)
 			ret[w] = 0.0;
@@ -44,12 +44,12 @@ __device__ static void fin_apply(const sepk_task &k, const double *tot, sepdev_s
 		// nblk == 0: the sum of m v^2 is already in sys->sumv2
 		const double sum = k.nblk > 0 ? tot[0] : sys->sumv2;
 		sys->sumv2 = sum;
-		*k.alpha = sepk_nh_alpha(*k.alpha, sum, k.npart, k.Td, k.dt_tau2);
+		*alpha = sepk_nh_alpha(*alpha, sum, k.npart, k.Td, k.dt_tau2);
 		break;
 	}
 	case SEPK_FIN_LF: {
 		if (k.nh_fused)
-			*k.alpha = sepk_nh_alpha(*k.alpha, sys->sumv2, k.npart, k.Td, k.dt_tau2);
+			*alpha = sepk_nh_alpha(*alpha, sys->sumv2, k.npart, k.Td, k.dt_tau2);
 		ret[SEPDEV_RET_EKIN] += 0.5 * tot[0];
 		sepk_add_sym9(ret + SEPDEV_RET_KINP, tot + 1, 1.0);
 		// source/sepintgr.c:61,72-84: running maximum since sep_reset_force
@@ -68,7 +68,7 @@ __device__ static void fin_apply(const sepk_task &k, const double *tot, sepdev_s
 		break;
 	}
 	case SEPK_FIN_BUILD:
-		sys->max_full = (int)tot[0];
+		// sys->max_full was set by the build's atomic maximum
 		sys->neighb_flag = 0;
 		sys->xn_pending = 0; // applied by the permutation
 		sys->unclean = 0;    // judged afresh by the next binning
@@ -76,82 +76,134 @@ __device__ static void fin_apply(const sepk_task &k, const double *tot, sepdev_s
 	}
 }
 
-// One workgroup.  Per task: each thread sums rows tid, tid + 1024, ... of
-// every column (column-major rows: consecutive threads read consecutive
-// words), then a wave butterfly and a combine over the 16 waves, all in a
-// fixed order.
+// SEPK_FIN_BLOCKS workgroups.  Each takes one contiguous chunk of the rows
+// of every task (one thread per row, SEPK_MAXNV columns loaded at once with
+// clamped addresses) and reduces it in a fixed order -- thread-strided rows,
+// wave butterfly, waves in index order -- to one value per column.  The
+// last workgroup to arrive combines the SEPK_FIN_BLOCKS values of each
+// column in block order and applies the tasks in queue order.  Neither
+// step depends on timing, so repeated runs give identical bits.
+__device__ static double fin_comb(bool mx, double a, double b)
+{
+	return mx ? fmax(a, b) : a + b;
+}
+
 __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, sepk_fin f)
 {
 	__shared__ double wred[SEPK_MAXNV][FIN_THREADS / 64];
 	__shared__ double tot[SEPK_MAXNV];
+	__shared__ int last;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-	sepdev_sys *sys = f.sys;
+	const int G = gridDim.x;
 	for (int ti = 0; ti < f.ntask; ti++) {
 		const sepk_task &k = f.t[ti];
-		// a conditional build that did not run leaves nothing to finish
-		if (k.mode == SEPK_FIN_BUILD && k.conditional && sys->neighb_flag == 0)
-			continue;
 		const double *p = partials + (size_t)k.region * f.region_words;
-		// every load of the task is issued before any is used (addresses are
-		// clamped to a valid row instead of branched around): one memory
-		// round trip per task instead of one per column
+		const long chunk = (k.nblk + G - 1) / G;
+		const long r0 = (long)blockIdx.x * chunk;
+		const long r1 = min(r0 + chunk, (long)k.nblk);
 		double a[SEPK_MAXNV];
-		if (k.nblk > 0) {
-			double v[SEPK_MAXNV][4];
 #pragma unroll
-			for (int i = 0; i < SEPK_MAXNV; i++) {
-				const double *col = p + (size_t)(i < k.nv ? i : 0) * f.region_rows;
-#pragma unroll
-				for (int r = 0; r < 4; r++) {
-					const int b = threadIdx.x + r * FIN_THREADS;
-					v[i][r] = col[b < k.nblk ? b : 0];
-				}
-			}
-#pragma unroll
-			for (int i = 0; i < SEPK_MAXNV; i++) {
-				double sum = 0.0, mx = 0.0;
-#pragma unroll
-				for (int r = 0; r < 4; r++) {
-					const double x = (threadIdx.x + r * FIN_THREADS < k.nblk) ? v[i][r] : 0.0;
-					sum += x;
-					mx = fmax(mx, x);
-				}
-				a[i] = ((k.maxmask >> i) & 1u) ? mx : sum;
-			}
-		} else {
+		for (int i = 0; i < SEPK_MAXNV; i++)
+			a[i] = 0.0;
+		for (long r = r0 + threadIdx.x; r < r1; r += FIN_THREADS) {
+			double v[SEPK_MAXNV];
 #pragma unroll
 			for (int i = 0; i < SEPK_MAXNV; i++)
-				a[i] = 0.0;
+				v[i] = p[(size_t)(i < k.nv ? i : 0) * f.region_rows + r];
+#pragma unroll
+			for (int i = 0; i < SEPK_MAXNV; i++)
+				a[i] = fin_comb((k.maxmask >> i) & 1u, a[i], v[i]);
 		}
-		for (int b = threadIdx.x + 4 * FIN_THREADS; b < k.nblk; b += FIN_THREADS)
-			for (int i = 0; i < k.nv; i++) {
-				const double v = p[(size_t)i * f.region_rows + b];
-				a[i] = ((k.maxmask >> i) & 1u) ? fmax(a[i], v) : a[i] + v;
-			}
 #pragma unroll
 		for (int i = 0; i < SEPK_MAXNV; i++) {
 			const bool mx = (k.maxmask >> i) & 1u;
 			double x = a[i];
-			for (int o = 32; o > 0; o >>= 1) {
-				const double y = __shfl_xor(x, o, 64);
-				x = mx ? fmax(x, y) : x + y;
-			}
-			if (lane == 0 && i < k.nv)
+			for (int o = 32; o > 0; o >>= 1)
+				x = fin_comb(mx, x, __shfl_xor(x, o, 64));
+			if (lane == 0)
 				wred[i][wid] = x;
 		}
 		__syncthreads();
-		if (threadIdx.x == 0) {
-			for (int i = 0; i < k.nv; i++) {
-				const bool mx = (k.maxmask >> i) & 1u;
-				double a = wred[i][0];
-				for (int w = 1; w < FIN_THREADS / 64; w++)
-					a = mx ? fmax(a, wred[i][w]) : a + wred[i][w];
-				tot[i] = a;
-			}
-			fin_apply(k, tot, sys);
+		if (threadIdx.x < k.nv) {
+			const int i = threadIdx.x;
+			const bool mx = (k.maxmask >> i) & 1u;
+			double x = wred[i][0];
+			for (int w = 1; w < FIN_THREADS / 64; w++)
+				x = fin_comb(mx, x, wred[i][w]);
+			f.l2[((size_t)ti * SEPK_MAXNV + i) * G + blockIdx.x] = x;
 		}
 		__syncthreads();
 	}
+	// publish this block's rows, then count it in
+	if (threadIdx.x == 0) {
+		__threadfence();
+		last = atomicAdd(f.count, 1u) == (unsigned)(G - 1);
+	}
+	__syncthreads();
+	if (!last)
+		return;
+	__threadfence();
+	// The tasks read and update the system scalars, their return slots and
+	// thermostat variables: one parallel load of each into LDS, the tasks
+	// applied there in queue order, one parallel store back -- instead of a
+	// chain of dependent global round trips in a single thread.
+	__shared__ double sret[SEPK_MAXTASK][SEPDEV_RET_WORDS];
+	__shared__ double salpha[SEPK_MAXTASK];
+	__shared__ sepdev_sys ssys;
+	int rmap[SEPK_MAXTASK], amap[SEPK_MAXTASK]; // first task sharing the pointer
+	for (int ti = 0; ti < f.ntask; ti++) {
+		rmap[ti] = amap[ti] = -1;
+		for (int tj = ti; tj >= 0; tj--) {
+			if (f.t[ti].ret && f.t[tj].ret == f.t[ti].ret)
+				rmap[ti] = tj;
+			if (f.t[ti].alpha && f.t[tj].alpha == f.t[ti].alpha)
+				amap[ti] = tj;
+		}
+	}
+	const int nsys = (int)(sizeof(sepdev_sys) / sizeof(int));
+	for (int w = threadIdx.x; w < f.ntask * SEPDEV_RET_WORDS; w += FIN_THREADS) {
+		const int ti = w / SEPDEV_RET_WORDS, word = w % SEPDEV_RET_WORDS;
+		if (rmap[ti] == ti)
+			sret[ti][word] = f.t[ti].ret[word];
+	}
+	if (threadIdx.x < f.ntask && amap[threadIdx.x] == (int)threadIdx.x)
+		salpha[threadIdx.x] = *f.t[threadIdx.x].alpha;
+	if (threadIdx.x >= 64 && threadIdx.x < 64 + nsys)
+		((int *)&ssys)[threadIdx.x - 64] = ((const int *)f.sys)[threadIdx.x - 64];
+	__syncthreads();
+	for (int ti = 0; ti < f.ntask; ti++) {
+		const sepk_task &k = f.t[ti];
+		// a conditional build that did not run leaves nothing to finish
+		const bool skip = k.mode == SEPK_FIN_BUILD && k.conditional && ssys.neighb_flag == 0;
+		if (!skip) {
+			// wave w combines columns w, w + 4, ...: lane b holds block b
+			for (int i = wid; i < k.nv; i += FIN_THREADS / 64) {
+				const bool mx = (k.maxmask >> i) & 1u;
+				const double *col = f.l2 + ((size_t)ti * SEPK_MAXNV + i) * G;
+				double x = lane < G ? col[lane] : 0.0;
+				for (int o = 32; o > 0; o >>= 1)
+					x = fin_comb(mx, x, __shfl_xor(x, o, 64));
+				if (lane == 0)
+					tot[i] = x;
+			}
+			__syncthreads();
+			if (threadIdx.x == 0)
+				fin_apply(k, tot, rmap[ti] >= 0 ? sret[rmap[ti]] : nullptr,
+					  amap[ti] >= 0 ? &salpha[amap[ti]] : nullptr, &ssys);
+		}
+		__syncthreads();
+	}
+	for (int w = threadIdx.x; w < f.ntask * SEPDEV_RET_WORDS; w += FIN_THREADS) {
+		const int ti = w / SEPDEV_RET_WORDS, word = w % SEPDEV_RET_WORDS;
+		if (rmap[ti] == ti)
+			f.t[ti].ret[word] = sret[ti][word];
+	}
+	if (threadIdx.x < f.ntask && amap[threadIdx.x] == (int)threadIdx.x)
+		*f.t[threadIdx.x].alpha = salpha[threadIdx.x];
+	if (threadIdx.x >= 64 && threadIdx.x < 64 + nsys)
+		((int *)f.sys)[threadIdx.x - 64] = ((const int *)&ssys)[threadIdx.x - 64];
+	if (threadIdx.x == 0)
+		atomicExch(f.count, 0u);
 }
 
 double *sepk_region(sepdev *d, int region)
@@ -171,8 +223,10 @@ int sepk_flush_tasks(sepdev *d)
 	f.region_words = d->region_words;
 	f.region_rows = d->region_rows;
 	f.sys = d->sys;
+	f.l2 = d->fin_l2;
+	f.count = d->fin_count;
 	d->npending = 0;
-	k_finish<<<1, FIN_THREADS, 0, d->st>>>(d->partials, f);
+	k_finish<<<SEPK_FIN_BLOCKS, FIN_THREADS, 0, d->st>>>(d->partials, f);
 	SEPK_LAUNCH_CHECK();
 	return 0;
 }
