@@ -633,6 +633,42 @@ int seph_build(seph_ctx *c, sepsys *sys, const sepdev_grid *g, int conditional)
 	return write_half_lists(c);
 }
 
+static int grid_checked(const seph_ctx *c, const sepdev_grid *g)
+{
+	return c->checked_valid && c->checked_n[0] == g->n[0] && c->checked_n[1] == g->n[1] &&
+	       c->checked_n[2] == g->n[2] && c->checked_opt == g->opt &&
+	       c->checked_rlist2 == g->rlist2;
+}
+
+int seph_build_resident(seph_ctx *c, const sepdev_grid *g, int conditional)
+{
+	int rc = seph_dev_fail(sepdev_build_list(c->dev, g, conditional));
+	if (rc || grid_checked(c, g))
+		return rc;
+	/* first build of this grid: the initial capacity comes from the mean
+	 * density, which grids of fewer than 3 cells per axis (whose stencil
+	 * lists pairs more than once) or dense clusters can exceed; confirm
+	 * it here, once, instead of letting SEP_GPU_ENEIGHB surface later */
+	sepdev_sys ds;
+	if ((rc = seph_dev_fail(sepdev_get_sys(c->dev, &ds))))
+		return rc;
+	if (ds.error == SEP_GPU_ENEIGHB) {
+		ds.error = 0;
+		if ((rc = seph_dev_fail(sepdev_put_sys(c->dev, &ds))) ||
+		    (rc = seph_dev_fail(sepdev_reserve_list(c->dev, ds.max_full + ds.max_full / 4 + 32))) ||
+		    (rc = seph_dev_fail(sepdev_build_list(c->dev, g, 0))))
+			return rc;
+	}
+	if ((rc = seph_check_device_error(c)))
+		return rc;
+	for (int k = 0; k < 3; k++)
+		c->checked_n[k] = g->n[k];
+	c->checked_opt = g->opt;
+	c->checked_rlist2 = g->rlist2;
+	c->checked_valid = 1;
+	return 0;
+}
+
 int seph_identify_potential(double (*fun)(double, char))
 {
 	if (fun == sep_lj)

@@ -108,7 +108,7 @@ static int run_pairs(seppart *ptr, sepsys *sys, sepret *ret, pair_call *pc)
 		/* every call rebuilds at the interaction cutoff, bonds-only
 		 * exclusion test (source/sepprfrc.c:35-42) */
 		seph_grid_own(sys, pc->cf, pc->opt, 0, &g);
-		rc = c->resident ? seph_dev_fail(sepdev_build_list(c->dev, &g, 0))
+		rc = c->resident ? seph_build_resident(c, &g, 0)
 				 : seph_build(c, sys, &g, 0);
 		if (rc)
 			return rc;
@@ -120,7 +120,7 @@ static int run_pairs(seppart *ptr, sepsys *sys, sepret *ret, pair_call *pc)
 		} else {
 			seph_grid_reference(sys, pc->opt, &g);
 		}
-		if ((rc = seph_dev_fail(sepdev_build_list(c->dev, &g, 1))))
+		if ((rc = seph_build_resident(c, &g, 1)))
 			return rc;
 	} else if (sys->neighb_flag == 1) {
 		if (sys->neighb_update == SEP_NEIGHBLIST) {
@@ -239,7 +239,7 @@ void sep_coulomb_sf(seppart *ptr, double cf, sepsys *sys, sepret *retval,
 			goto out;
 		sepdev_grid g;
 		seph_grid_own(sys, cf, opt, 2, &g);
-		rc = c->resident ? seph_dev_fail(sepdev_build_list(c->dev, &g, 0))
+		rc = c->resident ? seph_build_resident(c, &g, 0)
 				 : seph_build(c, sys, &g, 0);
 		if (rc)
 			goto out;

@@ -48,6 +48,9 @@ typedef struct {
 	 * while resident, where types change only through sep_gpu_upload */
 	char uniform_type;
 	int uniform_type_valid;
+	/* grid whose list capacity was confirmed on the host (resident mode) */
+	int checked_n[3], checked_opt, checked_valid;
+	double checked_rlist2;
 	/* host staging */
 	double *v3;
 	int *i3;
@@ -93,6 +96,9 @@ void seph_grid_own(const sepsys *sys, double rlist, unsigned opt, int bonded_rul
 
 /* list build + AoS half list; used by the pair functions */
 int seph_build(seph_ctx *c, sepsys *sys, const sepdev_grid *g, int conditional);
+/* device-only build (resident mode); the first build of a grid is checked
+ * for list overflow on the host and grown once */
+int seph_build_resident(seph_ctx *c, const sepdev_grid *g, int conditional);
 
 /* potentials served on the device */
 enum { SEPH_POT_LJ = 0, SEPH_POT_LJ_SHIFT = 1, SEPH_POT_WCA = 2, SEPH_POT_NONE = -1 };

@@ -52,6 +52,8 @@ def box_length(n, rho):
 
 def maxwell_boltzmann(n, temp, mass, seed):
     """Velocities with zero total momentum, scaled so sum m v^2 = 3 N T."""
+    if temp <= 0.0:
+        return np.zeros((n, 3))
     v = normal(seed, 3 * n).reshape(n, 3) * np.sqrt(temp / mass)[:, None]
     p = (mass[:, None] * v).sum(axis=0)
     v -= p / mass.sum()

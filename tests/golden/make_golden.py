@@ -1,0 +1,126 @@
+"""Writes the golden vectors of tests/golden/ from the reference compiled
+from its own sources (oracle/_ref/libsepref.so, strict FP flags; build it
+with ``make -C oracle``).  Run from the repository root:
+
+    python tests/golden/make_golden.py
+
+Fixtures (numpy .npz, no pickles; inputs and outputs together, so the tests
+never read the reference tree):
+
+  g1_lj_nve.npz    config 1: LJ N=1000 SC 10^3, rho 0.8442, T 0.728, rc 2.5,
+                   skin 0.5, dt 0.005, sep_force_pairs + sep_leapfrog for
+                   1000 steps.  Per step: epot, ekin, max_dist2, pot_P,
+                   kin_P, tnow; neighb_flag, nupdate_neighb; half-list size
+                   and a hash of the sorted pair set at every rebuild;
+                   forces at steps 0, 1, 500, 999.
+  g4_lj_nvt.npz    the same start with sep_nosehoover(T0=0.728, Q=10) before
+                   each leapfrog, 200 steps: alpha, ekin, epot per step.
+  g2_water.npz     the reference's own water input (test/prg2.xyz/.top,
+                   216 SPC/Fw molecules, L=6.888, 2 cells per axis): one call
+                   each of sep_force_pairs("OO", EXCL_SAME_MOL),
+                   sep_stretch_harmonic, sep_angle_harmonic, sep_angle_cossq
+                   and sep_coulomb_sf(EXCL_SAME_MOL) with prgs/prg3.c's
+                   parameters; the force of each term and the sepret fields
+                   after it, with the linked-list path and with SEP_BRUTE.
+  g3_butane.npz    the first 100 molecules of the reference's butane input
+                   (test/prg1.xyz/.top, L=13.99) with prgs/prg2.c's terms.
+  g5_ka.npz        Kob-Andersen 80:20 N=500 at rho 1.2: sep_force_lj for
+                   AA, AB and BB.
+  abi_layout.json  sizes and field offsets of the structs of sepstrct.h as
+                   compiled from the reference's own header (abi_probe.c).
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle.checkers import OracleAPI, ReferenceAPI  # noqa: E402
+from seplib_amd import sep as S  # noqa: E402
+from tests.golden import runs as R  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = "/root/reference"
+G1_FSTEPS = (0, 1, 500, 999)
+
+
+def g1(ref, orc):
+    w = R.lj_cfg1()
+    scal, flags, hashes, npairs, forces = R.run_nve(ref, orc.lib, w, 1000, G1_FSTEPS)
+    np.savez_compressed(os.path.join(HERE, "g1_lj_nve.npz"), x=w["x"], v=w["v"], m=w["m"],
+                        L=w["L"], scal=scal, flags=flags, hashes=hashes, npairs=npairs,
+                        fsteps=np.array(G1_FSTEPS), forces=forces)
+
+
+def g4(ref):
+    w = R.lj_cfg1()
+    out = R.run_nvt(ref, w, 200)
+    np.savez_compressed(os.path.join(HERE, "g4_lj_nvt.npz"), x=w["x"], v=w["v"], m=w["m"],
+                        L=w["L"], alpha_ekin_epot=out)
+
+
+def g2(ref):
+    w = R.read_xyz(os.path.join(REF, "test", "prg2.xyz"))
+    w.update(R.read_top(os.path.join(REF, "test", "prg2.top")))
+    out = {}
+    for tag, update in (("llist", S.SEP_LLIST_NEIGHBLIST), ("brute", S.SEP_BRUTE)):
+        f, r = R.run_water(ref, w, update)
+        out["forces_" + tag] = f
+        out["rets_" + tag] = r
+    np.savez_compressed(os.path.join(HERE, "g2_water.npz"), x=w["x"], v=w["v"], m=w["m"],
+                        q=w["q"], type=w["type"], L=w["L"], bonds=w["bonds"],
+                        angles=w["angles"], **out)
+
+
+def g3(ref):
+    w = R.read_xyz(os.path.join(REF, "test", "prg1.xyz"))
+    top = R.read_top(os.path.join(REF, "test", "prg1.top"))
+    w = R.subset(w, top, 100, 4)
+    f, r = R.run_butane(ref, w)
+    np.savez_compressed(os.path.join(HERE, "g3_butane.npz"), x=w["x"], v=w["v"], m=w["m"],
+                        q=w["q"], type=w["type"], L=w["L"], bonds=w["bonds"],
+                        angles=w["angles"], dihedrals=w["dihedrals"], forces=f, rets=r)
+
+
+def g5(ref):
+    w = R.ka_workload()
+    f, r = R.run_ka(ref, w)
+    np.savez_compressed(os.path.join(HERE, "g5_ka.npz"), x=w["x"], v=w["v"], m=w["m"],
+                        type=w["type"], L=w["L"], forces=f, rets=r)
+
+
+def abi():
+    src = os.path.join(HERE, "abi_probe.c")
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "probe")
+        subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(REF, "include"), src,
+                               "-o", exe])
+        out = subprocess.check_output([exe]).decode()
+    with open(os.path.join(HERE, "abi_layout.json"), "w") as f:
+        json.dump(json.loads(out), f, indent=1, sort_keys=True)
+        f.write("\n")
+
+
+def main():
+    if not ReferenceAPI.available():
+        sys.exit("oracle/_ref/libsepref.so is missing: run make -C oracle")
+    ref = ReferenceAPI()
+    orc = OracleAPI()
+    abi()
+    g1(ref, orc)
+    g4(ref)
+    g2(ref)
+    g3(ref)
+    g5(ref)
+    for name in sorted(os.listdir(HERE)):
+        if name.endswith((".npz", ".json")):
+            print("%-20s %8d bytes" % (name, os.path.getsize(os.path.join(HERE, name))))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,140 @@
+"""The drop-in boundary and the host logic of libsep.so, on the CPU (no
+call here reaches the device: sep_init registers the array and the device
+state is created by the first force call).
+
+  * struct layouts of include/sepstrct.h equal the reference's (sizes and
+    field offsets recorded from the reference header by make_golden.py);
+  * libsep.so loads and exports every function include/*.h declares;
+  * sep_sys_setup, sep_set_skin, the pressure/temperature helpers and the
+    topology reader agree with the oracle and with their closed forms.
+"""
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from seplib_amd import sep as S
+from tests.golden import runs as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def test_struct_layout_matches_reference():
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "probe")
+        subprocess.check_call(["gcc", "-std=c99", "-I", INCLUDE,
+                               os.path.join(GOLD, "abi_probe.c"), "-o", exe])
+        own = json.loads(subprocess.check_output([exe]).decode())
+    with open(os.path.join(GOLD, "abi_layout.json")) as f:
+        ref = json.load(f)
+    assert own == ref
+    assert ref["sizeof.seppart"] == 568
+
+
+def _declared_functions():
+    names = set()
+    for h in sorted(os.listdir(INCLUDE)):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(sep_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_function():
+    names = _declared_functions()
+    assert {"sep_force_pairs", "sep_force_lj", "sep_coulomb_sf", "sep_leapfrog",
+            "sep_nosehoover", "sep_stretch_harmonic", "sep_angle_harmonic",
+            "sep_angle_cossq", "sep_torsion_Ryckaert", "sep_gpu_run"} <= names
+    lib = C.CDLL(S.PRODUCT_LIB)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.check_output(["nm", "-D", "--defined-only", S.PRODUCT_LIB]).decode()
+    defined = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    assert names <= defined
+
+
+@pytest.mark.parametrize("L,cf", [(10.580793, 2.5), (67.183848, 2.5), (6.888070, 2.9)])
+def test_sys_setup_matches_oracle_and_rule(product, oracle, L, cf):
+    n = 1000
+    a = product.sys_setup([L, L, L], cf, 0.005, n, S.SEP_LLIST_NEIGHBLIST)
+    b = oracle.sys_setup([L, L, L], cf, 0.005, n, S.SEP_LLIST_NEIGHBLIST)
+    for f, _ in S.sepsys._fields_:
+        if f in ("molptr",):
+            continue
+        va, vb = getattr(a, f), getattr(b, f)
+        if hasattr(va, "__len__"):
+            va, vb = list(va), list(vb)
+        assert va == vb, f
+    # source/sepinit.c:246-305: nsubbox = (int)(L / (cf + 0.25)), skin 0.25
+    assert a.nsubbox[0] == int(L / (cf + 0.25))
+    assert a.lsubbox[0] == L / a.nsubbox[0]
+    assert a.skin == 0.25 and a.neighb_flag == 1 and a.ndof == 3 * n - 3
+    product.lib.sep_set_skin(C.byref(a), C.c_double(0.3))
+    assert a.skin == 0.3 and a.nsubbox[0] == int(L / (cf + 0.25))
+    product.free_sys(C.byref(a))
+    oracle.free_sys(C.byref(b))
+
+
+def test_pressure_and_temperature_helpers(product):
+    lib = product.lib
+    n = 512
+    sysv = product.sys_setup([9.0, 9.5, 10.0], 2.5, 0.005, n, S.SEP_LLIST_NEIGHBLIST)
+    ret = S.sepret()
+    rng = np.random.default_rng(3)
+    kin = rng.normal(size=(3, 3))
+    pot = rng.normal(size=(3, 3))
+    for k in range(3):
+        for kk in range(3):
+            ret.kin_P[k][kk] = kin[k, kk]
+            ret.pot_P[k][kk] = pot[k, kk]
+    ret.ekin = 700.0
+    lib.sep_pressure_tensor(C.byref(ret), C.byref(sysv))
+    V = 9.0 * 9.5 * 10.0
+    P = (kin + pot) / V
+    assert np.allclose(S.mat(ret.P), P, rtol=1e-15, atol=0)
+    assert abs(ret.p - np.trace(P) / 3.0) <= 1e-15 * abs(np.trace(P))
+    lib.sep_get_pressure.restype = C.c_double
+    lib.sep_get_temperature.restype = C.c_double
+    p = lib.sep_get_pressure(C.byref(ret), C.byref(sysv))
+    assert abs(p - np.trace(P) / 3.0) <= 1e-15 * abs(p)
+    T = lib.sep_get_temperature(C.byref(ret), C.byref(sysv))
+    assert abs(T - 2.0 / (3.0 * (3 * n - 3)) * 700.0) <= 1e-15 * T
+    product.free_sys(C.byref(sysv))
+
+
+def _topology_arrays(api, w):
+    from seplib_amd.sim import SepSystem
+    s = SepSystem.from_workload(api, w, cf=2.9, dt=0.001)
+    try:
+        v = s.view
+        out = {k: np.array(v[k]) for k in ("bond", "angle", "dihed", "molindex")}
+        mp = s.sys.molptr.contents
+        out["num"] = (mp.num_mols, mp.num_bonds, mp.num_angles, mp.num_dihedrals)
+        out["blist"] = np.ctypeslib.as_array(mp.blist, (3 * mp.num_bonds,)).copy()
+        out["alist"] = np.ctypeslib.as_array(mp.alist, (4 * mp.num_angles,)).copy()
+        if mp.num_dihedrals:
+            out["dlist"] = np.ctypeslib.as_array(mp.dlist, (5 * mp.num_dihedrals,)).copy()
+    finally:
+        s.close()
+    return out
+
+
+@pytest.mark.parametrize("fixture", ["g2_water.npz", "g3_butane.npz"])
+def test_topology_reader_matches_oracle(product, oracle, fixture):
+    g = np.load(os.path.join(GOLD, fixture), allow_pickle=False)
+    w = R.workload_from(g)
+    a, b = _topology_arrays(product, w), _topology_arrays(oracle, w)
+    assert a["num"] == b["num"] and a["num"][1] == len(w["bonds"])
+    for k in b:
+        if k != "num":
+            assert np.array_equal(a[k], b[k]), k

@@ -1,0 +1,73 @@
+"""The C oracle against the golden vectors written by the compiled reference
+(tests/golden/make_golden.py).  CPU only.
+
+The oracle restates the reference's hot path operation by operation, so the
+bar is bit-identical output wherever the reference's own arithmetic is
+deterministic (every fixture here): identical pair sets, rebuild flags and
+counters, and floating-point results equal to the last bit over 1000 steps
+of a chaotic trajectory, where any dropped term, wrong sign, swapped index
+or reordered sum would show within a few steps.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from seplib_amd import sep as S
+from tests.golden import runs as R
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def test_g1_lj_nve_1000_steps(oracle):
+    g = _load("g1_lj_nve.npz")
+    w = R.workload_from(g)
+    scal, flags, hashes, npairs, forces = R.run_nve(oracle, oracle.lib, w, 1000,
+                                                    tuple(g["fsteps"]))
+    assert np.array_equal(npairs, g["npairs"])
+    assert np.array_equal(hashes, g["hashes"])
+    assert np.array_equal(flags[:, :2], g["flags"][:, :2])
+    assert (hashes >= 0).sum() > 50  # the list was rebuilt many times
+    assert np.array_equal(scal, g["scal"])
+    assert np.array_equal(forces, g["forces"])
+
+
+def test_g4_lj_nvt_200_steps(oracle):
+    g = _load("g4_lj_nvt.npz")
+    out = R.run_nvt(oracle, R.workload_from(g), 200)
+    assert np.array_equal(out, g["alpha_ekin_epot"])
+    assert np.abs(out[:, 0]).max() > 0.0
+
+
+@pytest.mark.parametrize("tag,update", [("llist", S.SEP_LLIST_NEIGHBLIST),
+                                        ("brute", S.SEP_BRUTE)])
+def test_g2_water_terms(oracle, tag, update):
+    g = _load("g2_water.npz")
+    f, r = R.run_water(oracle, R.workload_from(g), update)
+    gf, gr = g["forces_" + tag], g["rets_" + tag]
+    for k, name in enumerate(R.WATER_TERMS):
+        assert np.abs(gf[k]).max() > 0.0, name
+        assert np.array_equal(f[k], gf[k]), name
+        assert np.array_equal(r[k], gr[k]), name
+
+
+def test_g3_butane_terms(oracle):
+    g = _load("g3_butane.npz")
+    f, r = R.run_butane(oracle, R.workload_from(g))
+    for k, name in enumerate(R.BUTANE_TERMS):
+        assert np.abs(g["forces"][k]).max() > 0.0, name
+        assert np.array_equal(f[k], g["forces"][k]), name
+        assert np.array_equal(r[k], g["rets"][k]), name
+
+
+def test_g5_ka_force_lj(oracle):
+    g = _load("g5_ka.npz")
+    f, r = R.run_ka(oracle, R.workload_from(g))
+    for k in range(3):
+        assert np.abs(g["forces"][k]).max() > 0.0
+        assert np.array_equal(f[k], g["forces"][k]), k
+        assert np.array_equal(r[k], g["rets"][k]), k
