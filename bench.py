@@ -7,7 +7,7 @@ rc = 2.5, sep_lj_shift), Verlet list with skin 0.3 on a 23^3 cell grid
 (cell edge 2.921 >= rc + skin), rebuild when the largest displacement
 exceeds skin/2, Nose-Hoover at T0 = 1.0 with Q = 10, dt = 0.005.
 
-One step is the reference's call sequence (prgs/prg1.c:52-83):
+One step is the reference's call sequence (prgs/prg1.c:55-65):
 sep_reset_retval, sep_reset_force, sep_force_pairs (+ list rebuild when
 flagged), sep_nosehoover, sep_leapfrog -- all on the device in resident
 mode, replayed from a captured HIP graph, energies and stress reduced on
@@ -17,14 +17,21 @@ The timed region is K steps of the mixed form (FP32 pair arithmetic, FP64
 accumulation, FP64 positions and integrator) and K steps of the plain FP64
 form; ``value`` is the mixed form's atom-steps/s.  The mixed form's force
 error against FP64 is measured on the final state.  The pair-force kernel's
-average duration comes from HIP events on the library stream over a
-further instrumented segment of the same run.
+average launch duration (the roofline's ``achieved``) comes from HIP events
+on the library stream around 200 back-to-back launches of the step's
+pair-force kernel on the final state of the timed run.
+``traffic`` is the HBM bytes per launch of that kernel from the rocprofv3
+FETCH_SIZE / WRITE_SIZE counter summary committed under profiles/, scaled by
+the counters' reading for a plain device copy of known size.
 
 cpu_baseline: the reference's own C (oracle/_ref/libsepref_fast.so, built
 from its sources with the shipped -Ofast flags) with OpenMP on the host,
 over a bounded 200-step sample of the same workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-calib]
+
+--copy-calib first runs a plain 256 MiB device-to-device copy kernel (for
+the counter calibration of a profiled run).
 """
 import argparse
 import json
@@ -44,6 +51,7 @@ SKIN = 0.3
 DT = 0.005
 NSUB = 23
 Q_NH = 10.0
+KERNEL_REPS = 200
 
 
 def log(*a):
@@ -78,15 +86,11 @@ def run_gpu(api, w, precision, steps, warmup, time_kernel=True):
     kern_ms = None
     if time_kernel:
         import ctypes as C
-        # HIP events on the library stream around every pair-kernel launch
-        nk = max(50, min(steps, 500))
-        s.check(lib.sep_gpu_timing_enable(s.atoms, 1))
-        s.check(lib.sep_gpu_timing_reset(s.atoms))
-        s.gpu_run(ops, nk, graph=False)
-        tot, cnt = C.c_double(0), C.c_long(0)
-        s.check(lib.sep_gpu_timing_get(s.atoms, b"pair", C.byref(tot), C.byref(cnt)))
-        kern_ms = tot.value / max(cnt.value, 1)
-        s.check(lib.sep_gpu_timing_enable(s.atoms, 0))
+        # HIP events on the library stream around KERNEL_REPS back-to-back
+        # launches of the step's pair-force kernel on the final state
+        ms = C.c_double(0)
+        s.check(lib.sep_gpu_time_pair_kernel(s.atoms, KERNEL_REPS, C.byref(ms)))
+        kern_ms = ms.value
     import ctypes as C
     mean, mx = C.c_double(0), C.c_int(0)
     s.check(lib.sep_gpu_list_stats(s.atoms, C.byref(mean), C.byref(mx)))
@@ -151,6 +155,31 @@ def cpu_baseline(w, steps=200, warm=20):
                        "%.1f s" % (steps, warm, threads, secs))
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lj_lj256k_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary,
+    or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def copy_calibration(nbytes=256 << 20):
+    """A plain device copy of known size, for counter calibration."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy calibration: 3 x %d bytes device-to-device" % nbytes)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +187,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--copy-calib", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -168,6 +198,8 @@ def main():
     from seplib_amd import workloads as W
     from seplib_amd.sim import ProductAPI
 
+    if args.copy_calib:
+        copy_calibration()
     w = W.lj_fcc(N_SIDE, RHO, TEMP, seed=2)
     n = len(w["m"])
     api = ProductAPI()
@@ -193,9 +225,13 @@ def main():
     # entries, the list length, the atom's FP64 position, the FP64 force
     bytes_per_atom = 4.0 * m["nbar"] + 4 + 24 + 24
     achieved = bytes_per_atom * n / (m["kms"] * 1e-3) / 1e9
+    kname = "k_lj<false, true, true, true>"
+    traffic = measured_traffic(kname)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_lj (mixed)", kernel_us=round(m["kms"] * 1e3, 3),
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=None if traffic is None else round(traffic),
+                    kernel=kname + " (mixed pair forces)", kernel_us=round(m["kms"] * 1e3, 3),
+                    algorithmic_bytes=round(bytes_per_atom * n),
                     bytes_per_atom=round(bytes_per_atom, 2))
     cpu = None if args.no_cpu else cpu_baseline(w, steps=args.cpu_steps)
     if cpu:

@@ -102,13 +102,19 @@ int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret,
 int sep_gpu_neighbour_pairs(seppart *atoms, sepsys *sys, int *pairs,
 			    long cap, long *npairs);
 
-/* Kernel timing with HIP events on the library's stream.  kernel names:
- * "pair" (the dominant pair-force kernel), "step" (all device work of one
- * step).  sep_gpu_timing_reset starts a new window. */
+/* Kernel timing with HIP events on the library's stream around each eager
+ * launch of the pair-force kernel (kernel name "pair"; launches inside a
+ * captured graph are not timed).  sep_gpu_timing_reset starts a new window. */
 int sep_gpu_timing_enable(seppart *atoms, int on);
 int sep_gpu_timing_reset(seppart *atoms);
 int sep_gpu_timing_get(seppart *atoms, const char *kernel, double *total_ms,
 		       long *count);
+
+/* Mean duration of the pair-force kernel: the last configuration that a
+ * pair call launched, re-launched reps times back to back on the library
+ * stream between one pair of HIP events.  Afterwards f holds the pair forces
+ * of the current positions (as after sep_reset_force + that call). */
+int sep_gpu_time_pair_kernel(seppart *atoms, int reps, double *ms_per_launch);
 
 /* Mean full-list length of the current device list (entries per atom). */
 int sep_gpu_list_stats(seppart *atoms, double *mean_full, int *max_full);

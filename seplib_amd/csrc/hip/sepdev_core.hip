@@ -186,6 +186,7 @@ extern "C" void sepdev_destroy(sepdev *d)
 	if (d->hstage)
 		(void)hipHostFree(d->hstage);
 	free(d->ret_zero);
+	free(d->last_lj);
 	if (d->timing) {
 		for (int i = 0; i < 64; i++) {
 			(void)hipEventDestroy(d->t_pair.beg[i]);

@@ -252,6 +252,13 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 	d->f_zero_pending = 0;
 	if ((rc = sepk_timer_end(d, &d->t_pair)))
 		return rc;
+	if (!d->last_lj)
+		d->last_lj = malloc(sizeof(sepk_lj));
+	memcpy(d->last_lj, &c, sizeof c);
+	d->last_lj_kernel = (const void *)(a->typed ? (a->mixed ? pick_lj<true, true>(true, fast)
+							       : pick_lj<true, false>(true, fast))
+						    : (a->mixed ? pick_lj<false, true>(true, fast)
+							       : pick_lj<false, false>(true, fast)));
 	sepk_task f;
 	memset(&f, 0, sizeof f);
 	f.mode = SEPK_FIN_PAIR;
@@ -261,6 +268,51 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 	f.epot_assign = a->epot_assign;
 	sepk_task_ret(d, f, ret_slot);
 	return sepk_queue_task(d, f);
+}
+
+// Launch duration of the pair-force kernel: the last configuration launched
+// by sepdev_lj, in its overwriting form, `reps` times back to back on the
+// library stream between one pair of HIP events.  Leaves f holding the pair
+// forces of the current positions (what sep_reset_force + that call give)
+// and queues no reduction.
+extern "C" int sepdev_time_pair(sepdev *d, int reps, double *ms_per_launch)
+{
+	if (!d->last_lj || !d->list_built)
+		return sepk_fail(SEP_GPU_ESTATE, "no pair-force launch to time");
+	if (reps < 1 || d->capturing)
+		return sepk_fail(SEP_GPU_EARG, "time_pair: reps %d%s", reps,
+				 d->capturing ? " during capture" : "");
+	int rc = sepk_flush_nh(d);
+	if (rc || (rc = sepk_flush_tasks(d)))
+		return rc;
+	sepk_lj c;
+	memcpy(&c, d->last_lj, sizeof c);
+	const sepk_lj_kernel k = (sepk_lj_kernel)d->last_lj_kernel;
+	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
+	double *part = sepk_region(d, SEPK_R_PAIR);
+	hipEvent_t e0, e1;
+	SEPK_CHECK(hipEventCreate(&e0));
+	SEPK_CHECK(hipEventCreate(&e1));
+	hipError_t err = hipEventRecord(e0, d->st);
+	for (int r = 0; r < reps && err == hipSuccess; r++) {
+		k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c,
+						  part, d->region_rows);
+		err = hipGetLastError();
+	}
+	if (err == hipSuccess)
+		err = hipEventRecord(e1, d->st);
+	if (err == hipSuccess)
+		err = hipEventSynchronize(e1);
+	float ms = 0.0f;
+	if (err == hipSuccess)
+		err = hipEventElapsedTime(&ms, e0, e1);
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (err != hipSuccess)
+		return sepk_fail(SEP_GPU_EDEVICE, "time_pair: %s", hipGetErrorString(err));
+	d->f_zero_pending = 0;
+	*ms_per_launch = ms / reps;
+	return 0;
 }
 
 // ---- shifted-force Coulomb (source/sepcoulomb.c:96-160) ---------------------

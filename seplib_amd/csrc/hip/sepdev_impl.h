@@ -78,7 +78,7 @@ struct sepk_fin {
 };
 
 struct sepk_timer {
-	hipEvent_t beg[64], end[64];
+	hipEvent_t beg[64], end[64]; // eager launches, flushed in batches
 	int used;
 	double ms;
 	long count;
@@ -145,6 +145,10 @@ struct sepdev {
 	// timing
 	int timing;
 	sepk_timer t_pair;
+	// the last pair-force launch in its overwriting form, for
+	// sepdev_time_pair (kernel pointer and its sepk_lj argument block)
+	const void *last_lj_kernel;
+	void *last_lj;
 
 	// graph
 	hipGraph_t graph;

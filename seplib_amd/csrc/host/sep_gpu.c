@@ -201,6 +201,14 @@ int sep_gpu_timing_get(seppart *atoms, const char *kernel, double *total_ms, lon
 	return seph_dev_fail(sepdev_timing_get(c->dev, kernel, total_ms, count));
 }
 
+int sep_gpu_time_pair_kernel(seppart *atoms, int reps, double *ms_per_launch)
+{
+	seph_ctx *c = seph_find(atoms);
+	if (!c)
+		return seph_fail(SEP_GPU_ESTATE, "no device context for these atoms");
+	return seph_dev_fail(sepdev_time_pair(c->dev, reps, ms_per_launch));
+}
+
 int sep_gpu_list_stats(seppart *atoms, double *mean_full, int *max_full)
 {
 	seph_ctx *c = seph_find(atoms);

@@ -158,6 +158,9 @@ int sepdev_synchronize(sepdev *d);
 int sepdev_timing(sepdev *d, int on);
 int sepdev_timing_reset(sepdev *d);
 int sepdev_timing_get(sepdev *d, const char *name, double *ms, long *count);
+/* mean duration of the last pair-force configuration over reps back-to-back
+ * launches (replaces f with the pair forces; queues no reduction) */
+int sepdev_time_pair(sepdev *d, int reps, double *ms_per_launch);
 int sepdev_graph_begin(sepdev *d);
 int sepdev_graph_end(sepdev *d);
 int sepdev_graph_launch(sepdev *d, long times);

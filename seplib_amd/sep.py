@@ -192,6 +192,7 @@ def load_product(path=PRODUCT_LIB):
     f("sep_gpu_timing_reset", C.c_int, P_ATOMS)
     f("sep_gpu_timing_get", C.c_int, P_ATOMS, C.c_char_p, P_D, C.POINTER(C.c_long))
     f("sep_gpu_list_stats", C.c_int, P_ATOMS, P_D, C.POINTER(C.c_int))
+    f("sep_gpu_time_pair_kernel", C.c_int, P_ATOMS, C.c_int, P_D)
     f("sep_gpu_synchronize", C.c_int, P_ATOMS)
     return lib
 
