@@ -4,6 +4,8 @@
 #ifndef SEP_MI355X_SEPMISC_H
 #define SEP_MI355X_SEPMISC_H
 
+#include <stdio.h>
+
 #include "sepstrct.h"
 
 #ifdef __cplusplus
@@ -38,6 +40,9 @@ double sep_wca(double r2, char opt);
  * include/sepmisc.h:245 (source/sepmisc.c:393-400). */
 void sep_reset_force(seppart *ptr, sepsys *sys);
 
+/* Flushes stdout (include/sepmisc.h:39). */
+#define SEP_FLUSH fflush(stdout)
+
 /* Sets sys->omp_flag and sys->nthreads.  The device path ignores the thread
  * count.  Replaces include/sepmisc.h:303 (source/sepmisc.c:1116-1123). */
 void sep_set_omp(unsigned nthreads, sepsys *sys);
@@ -45,6 +50,13 @@ void sep_set_omp(unsigned nthreads, sepsys *sys);
 /* Sets the list skin; like the reference it does not recompute the cell
  * grid.  Replaces include/sepmisc.h:311 (source/sepmisc.c:1125-1129). */
 void sep_set_skin(sepsys *sys, double value);
+
+/* Writes the atoms whose type is one of the characters of partnames in the
+ * sep_init_xyz format ("N\nLx Ly Lz\n", then "type x y z vx vy vz m q"
+ * per atom, %.15f), fopen mode `mode`.  A resident system is synchronised
+ * first.  Replaces include/sepmisc.h:164 (source/sepmisc.c:536-573). */
+void sep_save_xyz(seppart *ptr, const char *partnames, const char *file, char *mode,
+		  sepsys *sys);
 
 /* Diagnostics printed as "sep-error -> ..." / "sep-warning -> ..." on
  * stdout.  sep_error does not exit (the reference's -DOCTAVE behaviour,

@@ -8,20 +8,25 @@
  * Built with -O2 -ffp-contract=off so that every product and sum is rounded
  * on its own, in the operation order of the reference source.
  *
- * How each part is pinned independently of this file (tests/test_oracle_*):
- *   neighbour lists     -> compiled reference (oracle/_ref) on the same input,
- *                          committed golden pair sets, O(N^2) enumeration
- *   LJ / WCA forces     -> golden vectors from the compiled reference,
- *                          two-atom closed forms, Newton's third law,
- *                          tr(pot_P) = sum r.f, O(N^2) brute-force sums
- *   shifted-force Coulomb -> golden vectors, two-charge closed form,
- *                          F(rc) = U(rc) = 0
- *   bond / cos^2 angle / RB torsion -> golden vectors, central finite
- *                          differences of their own energy
- *   harmonic angle      -> golden vectors only (its force is not -grad U)
- *   Nose-Hoover, leapfrog -> golden step sequences, free-particle and
- *                          constant-force closed forms, wrap/crossing cases
- *   topology reader     -> compiled reference on the reference's .top files
+ * How each part is pinned independently of this file (tests/
+ * test_golden_oracle.py against vectors written by the compiled reference,
+ * test_reference_live.py against that reference run live,
+ * test_oracle_math.py against the mathematics); every function below is
+ * covered, none is "parity unpinned":
+ *   neighbour lists     -> golden pair-set hashes of 1000 steps (bitwise),
+ *                          O(N^2) enumeration with the same r^2 op order
+ *   LJ / WCA forces     -> golden vectors (bitwise), two-atom closed forms
+ *                          (F(2^(1/6)) = 0, U(rc) = 0, the sigma^2 factor of
+ *                          sep_force_lj), O(N^2) numpy sums, Newton's third law
+ *   shifted-force Coulomb -> golden vectors (bitwise), two-charge closed
+ *                          form, F(rc) = U(rc) = 0, O(N^2) numpy sum
+ *   bond / cos^2 angle / RB torsion -> golden vectors (bitwise), central
+ *                          finite differences of their own energy
+ *   harmonic angle      -> golden vectors (bitwise); F = sin(theta) (-grad U)
+ *                          against central differences
+ *   Nose-Hoover, leapfrog -> golden step sequences (bitwise), free-particle
+ *                          recurrences, wrap/crossing cases, skin/2 trigger
+ *   topology reader     -> the reference's own .top inputs (golden vectors)
  */
 #include "seporacle.h"
 

@@ -293,3 +293,34 @@ double sep_wca(double r2, char opt)
 		return 4.0 * inv3 * (inv3 - 1.0) + 1.0;
 	return 0.0;
 }
+
+/* ---- configuration output: source/sepmisc.c:536-573 -------------------- */
+
+void sep_save_xyz(seppart *ptr, const char *partnames, const char *file, char *mode,
+		  sepsys *sys)
+{
+	seph_ctx *c = seph_find(ptr);
+	if (c && c->resident && seph_get_all(c, sys) != 0) {
+		sep_warning("sep_save_xyz: %s", (char *)sep_gpu_last_error_message());
+		return;
+	}
+	FILE *fout = fopen(file, mode);
+	if (fout == NULL) {
+		sep_error("%s: I couldn't open file %s", __func__, file);
+		return;
+	}
+	const size_t ntype = strlen(partnames);
+	unsigned long ntotal = 0;
+	for (long n = 0; n < sys->npart; n++)
+		for (size_t k = 0; k < ntype; k++)
+			if (ptr[n].type == partnames[k])
+				ntotal++;
+	fprintf(fout, "%lu\n%f %f %f\n", ntotal, sys->length[0], sys->length[1], sys->length[2]);
+	for (long n = 0; n < sys->npart; n++)
+		for (size_t k = 0; k < ntype; k++)
+			if (ptr[n].type == partnames[k])
+				fprintf(fout, "%c %.15f %.15f %.15f %.15f %.15f %.15f %.15f %.15f\n",
+					ptr[n].type, ptr[n].x[0], ptr[n].x[1], ptr[n].x[2], ptr[n].v[0],
+					ptr[n].v[1], ptr[n].v[2], ptr[n].m, ptr[n].z);
+	fclose(fout);
+}

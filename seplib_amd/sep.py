@@ -218,6 +218,7 @@ def bind_sep_names(lib):
     f("sep_read_topology_file", None, P_ATOMS, C.c_char_p, P_SYS, C.c_char)
     f("sep_get_pressure", C.c_double, P_RET, P_SYS)
     f("sep_get_temperature", C.c_double, P_RET, P_SYS)
+    f("sep_save_xyz", None, P_ATOMS, C.c_char_p, C.c_char_p, C.c_char_p, P_SYS)
 
 
 def fn_address(lib, name):

@@ -371,3 +371,29 @@ def test_rebuild_trigger_at_half_skin(oracle, disp, expect):
         assert rel(math.sqrt(s.sys.max_dist2), disp) <= 1e-12
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("fun,cf,shift", [("sep_lj", 2.5, 0.0), ("sep_lj_shift", 2.5, LJCF2),
+                                          ("sep_wca", 2.0 ** (1.0 / 6.0), 1.0)])
+def test_potential_functions_two_body(oracle, fun, cf, shift):
+    """sep_force_pairs with each served potential (source/sepmisc.c:115-164):
+    U = 4 (r^-12 - r^-6) + shift, F = 48 r^-2 (r^-12 - r^-6 / 2) r, zero
+    beyond the interaction cutoff."""
+    for r in (0.95, 1.05, 1.3, 2.4):
+        s = _two_atoms(oracle, r, cf=2.5)
+        try:
+            s.reset_retval()
+            s.reset_force()
+            s.force_pairs("AA", cf, fun)
+            f = np.array(s.view["f"])
+            if r * r < cf * cf:
+                rri = 1.0 / (r * r)
+                e = 4.0 * rri ** 3 * (rri ** 3 - 1.0) + shift
+                ft = 48.0 * rri ** 3 * (rri ** 3 - 0.5) * rri
+                assert abs(s.ret.epot - e) <= 1e-12 * max(1.0, abs(e)), (fun, r)
+                assert abs(f[1, 0] - ft * r) <= 1e-12 * max(1.0, abs(ft * r)), (fun, r)
+                assert f[0, 0] == -f[1, 0]
+            else:
+                assert s.ret.epot == 0.0 and np.abs(f).max() == 0.0, (fun, r)
+        finally:
+            s.close()
