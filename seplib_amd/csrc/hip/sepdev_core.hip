@@ -78,9 +78,12 @@ __global__ void k_init_identity(sepk_state s, int *iperm, long n, long npad)
 	s.orig[t] = t < n ? (int)t : -1;
 	if (t < n)
 		iperm[t] = (int)t;
-	s.pos[t] = sepk_pos{0.0, 0.0, 0.0, 0.0};
+	// slots past the last atom are phantoms: lists are padded with the
+	// first of them, and a NaN position fails every cutoff test
+	const double nan = __builtin_nan("");
+	s.pos[t] = t < n ? sepk_pos{0.0, 0.0, 0.0, 0.0} : sepk_pos{nan, nan, nan, 0.0};
 	s.m[t] = 1.0;
-	s.type[t] = 'A';
+	s.type[t] = t < n ? 'A' : 0;
 	s.mol[t] = -1;
 	for (int c = 0; c < 3; c++) {
 		s.v[c * npad + t] = 0.0;
@@ -101,7 +104,8 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 	}
 	sepdev *d = (sepdev *)calloc(1, sizeof(sepdev));
 	d->n = npart;
-	d->npad = (npart + 63) / 64 * 64;
+	// at least one slot past the last atom: the phantom that pads lists
+	d->npad = (npart + 1 + 63) / 64 * 64;
 	int rc = 0;
 	if (hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking) != hipSuccess) {
 		*err = sepk_fail(SEP_GPU_EDEVICE, "hipStreamCreate failed");
@@ -113,7 +117,7 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 	rc |= dalloc(&d->iperm, d->npad);
 	rc |= dalloc(&d->cellid_old, d->npad);
 	rc |= dalloc(&d->order_tmp, d->npad);
-	rc |= dalloc(&d->neworder, d->npad);
+	rc |= dalloc(&d->cell_slot, d->npad);
 	rc |= dalloc(&d->nbr_count, d->npad);
 	{
 		// rows per region: the widest launch is a list kernel (SEPK_SUB
@@ -173,8 +177,8 @@ extern "C" void sepdev_destroy(sepdev *d)
 	sepdev_graph_drop(d);
 	free_state(&d->s[0]);
 	free_state(&d->s[1]);
-	void *ptrs[] = {d->iperm, d->cell_count, d->cell_start, d->cell_fill,
-			d->cellid_old, d->order_tmp, d->neworder,
+	void *ptrs[] = {d->iperm, d->cell_count, d->cell_start,
+			d->cellid_old, d->order_tmp, d->cell_slot,
 			d->nbr, d->nbr_count, d->partials, d->fin_l2, d->fin_count, d->ret,
 			d->scal, d->sys, d->dstage, d->blist, d->alist, d->dlist,
 			d->bcontrib, d->acontrib, d->dcontrib, d->bcsr_off,

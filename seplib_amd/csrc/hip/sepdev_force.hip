@@ -46,10 +46,13 @@ extern "C" int sepdev_zero_forces(sepdev *d)
 
 // ---- LJ family ---------------------------------------------------------------
 
+// ft = eps48 q^3 (q^3 - aw/2) q and u = eps4 q^3 (q^3 - aw) + ushift with
+// q = sig2 / r^2, evaluated as ft = (A q^3 + nB) q^3 q, u = (C q^3 + nD) q^3 + ushift
+// (nB = -eps48 aw/2, nD = -eps4 aw)
 struct sepk_lj {
-	double cf2, eps48, eps4, awh, aw, sig2, ushift;
+	double cf2, A, nB, C, nD, sig2, ushift;
 	double L0, L1, L2, iL0, iL1, iL2;
-	float eps48f, eps4f, awhf, awf, sig2f, ushiftf;
+	float Af, nBf, Cf, nDf, sig2f, ushiftf;
 	signed char t0, t1;
 };
 
@@ -88,13 +91,50 @@ __device__ __forceinline__ double sepk_image(double d, double L, double iL)
 	return FAST ? sepk_wrap_fast(d, L, iL) : sepk_wrap(d, L);
 }
 
+// Adds one pair's force ft r to the atom, its energy, and ft r (x) r to the
+// six stress components {xx, xy, xz, yy, yz, zz}, with fused multiply-adds
+// (the diagonal reuses the squares of r^2).
+__device__ __forceinline__ void sepk_pair_accumulate(
+	double ft, double e, double dx, double dy, double dz, double xx, double yy, double zz,
+	double &fx, double &fy, double &fz, double (&acc)[7])
+{
+	fx = fma(ft, dx, fx);
+	fy = fma(ft, dy, fy);
+	fz = fma(ft, dz, fz);
+	acc[0] += e;
+	acc[1] = fma(ft, xx, acc[1]);
+	acc[2] = fma(ft, dx * dy, acc[2]);
+	acc[3] = fma(ft, dx * dz, acc[3]);
+	acc[4] = fma(ft, yy, acc[4]);
+	acc[5] = fma(ft, dy * dz, acc[5]);
+	acc[6] = fma(ft, zz, acc[6]);
+}
+
 // Entries of one lane are taken SEPK_BATCH at a time: every index and
 // every neighbour position of the batch is loaded before any is used, so a
-// wave keeps SEPK_BATCH gathers in flight instead of one.  Slots past the
-// lane's last entry are clamped to that entry (always a valid atom) and
-// masked out, so the loads need no branches; the accumulation order is the
-// list order, as in a one-at-a-time loop.
-#define SEPK_BATCH 4
+// wave keeps SEPK_BATCH gathers in flight instead of one.  Lists are padded
+// to whole batches with the phantom slot (NaN position, type 0, charge 0),
+// which fails every test, so the loads need neither clamps nor branches; the
+// accumulation order is the list order.
+//
+// Addressing: lane `sub` of atom t reads its e-th entry at
+// lp + e * stride with lp = nbr + 4 t + sub and stride = 4 npad; neighbour
+// records are fetched with 32-bit byte offsets from the uniform base.
+__device__ __forceinline__ const sepk_pos &sepk_rec(const char *base, int j)
+{
+	return *(const sepk_pos *)(base + (unsigned)j * (unsigned)sizeof(sepk_pos));
+}
+
+// 1 / r2 in FP64: hardware reciprocal and two Newton steps (within an ulp of
+// the division; the error bar of the FP64 path is 1e-12)
+__device__ __forceinline__ double sepk_recip(double r2)
+{
+	double y = __builtin_amdgcn_rcp(r2);
+	double e = fma(-r2, y, 1.0);
+	y = fma(y, e, y);
+	e = fma(-r2, y, 1.0);
+	return fma(y, e, y);
+}
 
 template <bool TYPED, bool MIXED, bool OVERWRITE, bool FAST>
 __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
@@ -114,57 +154,52 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 	}
 	if (act) {
 		const sepk_pos pi = s.pos[t];
-		// entries sub, sub + SEPK_SUB, ... of the atom belong to this lane
-		const int ne = (cnt[t] - sub + SEPK_SUB - 1) / SEPK_SUB;
-		for (int e0 = 0; e0 < ne; e0 += SEPK_BATCH) {
+		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
+		const size_t stride = (size_t)npad * SEPK_SUB;
+		const int *lp = nbr + (size_t)t * SEPK_SUB + sub;
+		const char *pb = (const char *)s.pos;
+		for (int b = 0; b < nbat; b++, lp += SEPK_BATCH * stride) {
 			int j[SEPK_BATCH];
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++)
-				j[u] = nbr[sepk_slot(min(e0 + u, ne - 1), t, npad, sub)];
-			sepk_pos pj[SEPK_BATCH];
+				j[u] = lp[u * stride];
+			double px[SEPK_BATCH], py[SEPK_BATCH], pz[SEPK_BATCH];
 			signed char tj[SEPK_BATCH];
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {
-				pj[u] = s.pos[j[u]];
+				const sepk_pos &r = sepk_rec(pb, j[u]);
+				px[u] = r.x;
+				py[u] = r.y;
+				pz[u] = r.z;
 				if (TYPED)
-					tj[u] = s.type[j[u]];
+					tj[u] = s.type[(unsigned)j[u]];
 			}
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {
-				if (e0 + u >= ne)
-					break;
 				if (TYPED && !((ti == c.t0 && tj[u] == c.t1) ||
 					       (ti == c.t1 && tj[u] == c.t0)))
 					continue;
-				const double dx = sepk_image<FAST>(pi.x - pj[u].x, c.L0, c.iL0);
-				const double dy = sepk_image<FAST>(pi.y - pj[u].y, c.L1, c.iL1);
-				const double dz = sepk_image<FAST>(pi.z - pj[u].z, c.L2, c.iL2);
-				const double r2 = dx * dx + dy * dy + dz * dz;
+				const double dx = sepk_image<FAST>(pi.x - px[u], c.L0, c.iL0);
+				const double dy = sepk_image<FAST>(pi.y - py[u], c.L1, c.iL1);
+				const double dz = sepk_image<FAST>(pi.z - pz[u], c.L2, c.iL2);
+				// r^2 in the reference's order; its squares feed the virial
+				const double xx = dx * dx, yy = dy * dy, zz = dz * dz;
+				const double r2 = xx + yy + zz;
 				if (!(r2 < c.cf2))
 					continue;
 				double ft, e;
 				if (MIXED) {
 					const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
 					const float q3 = q * q * q;
-					ft = (double)(c.eps48f * q3 * (q3 - c.awhf) * q);
-					e = (double)(c.eps4f * q3 * (q3 - c.awf) + c.ushiftf);
+					ft = (double)(fmaf(c.Af, q3, c.nBf) * q3 * q);
+					e = (double)fmaf(fmaf(c.Cf, q3, c.nDf), q3, c.ushiftf);
 				} else {
-					const double q = c.sig2 / r2;
+					const double q = c.sig2 * sepk_recip(r2);
 					const double q3 = q * q * q;
-					ft = c.eps48 * q3 * (q3 - c.awh) * q;
-					e = c.eps4 * q3 * (q3 - c.aw) + c.ushift;
+					ft = fma(c.A, q3, c.nB) * q3 * q;
+					e = fma(fma(c.C, q3, c.nD), q3, c.ushift);
 				}
-				const double gx = ft * dx, gy = ft * dy, gz = ft * dz;
-				fx += gx;
-				fy += gy;
-				fz += gz;
-				acc[0] += e;
-				acc[1] += gx * dx;
-				acc[2] += gx * dy;
-				acc[3] += gx * dz;
-				acc[4] += gy * dy;
-				acc[5] += gy * dz;
-				acc[6] += gz * dz;
+				sepk_pair_accumulate(ft, e, dx, dy, dz, xx, yy, zz, fx, fy, fz, acc);
 			}
 		}
 	}
@@ -210,10 +245,10 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 		return sepk_fail(SEP_GPU_EARG, "ret slot %d", ret_slot);
 	sepk_lj c;
 	c.cf2 = a->cf2;
-	c.eps48 = a->eps48;
-	c.eps4 = a->eps4;
-	c.awh = a->awh;
-	c.aw = a->aw;
+	c.A = a->eps48;
+	c.nB = -(a->eps48 * a->awh);
+	c.C = a->eps4;
+	c.nD = -(a->eps4 * a->aw);
 	c.sig2 = a->sig2;
 	c.ushift = a->ushift;
 	c.L0 = a->L[0];
@@ -222,10 +257,10 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 	c.iL0 = 1.0 / a->L[0];
 	c.iL1 = 1.0 / a->L[1];
 	c.iL2 = 1.0 / a->L[2];
-	c.eps48f = (float)a->eps48;
-	c.eps4f = (float)a->eps4;
-	c.awhf = (float)a->awh;
-	c.awf = (float)a->aw;
+	c.Af = (float)c.A;
+	c.nBf = (float)c.nB;
+	c.Cf = (float)c.C;
+	c.nDf = (float)c.nD;
 	c.sig2f = (float)a->sig2;
 	c.ushiftf = (float)a->ushift;
 	c.t0 = a->t0;
@@ -339,42 +374,36 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 		act = !(fabs(pi.q) < DBL_EPSILON);
 	}
 	if (act) {
-		const int ne = (cnt[t] - sub + SEPK_SUB - 1) / SEPK_SUB;
-		for (int e0 = 0; e0 < ne; e0 += SEPK_BATCH) {
+		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
+		const size_t stride = (size_t)npad * SEPK_SUB;
+		const int *lp = nbr + (size_t)t * SEPK_SUB + sub;
+		const char *pb = (const char *)s.pos;
+		for (int b = 0; b < nbat; b++, lp += SEPK_BATCH * stride) {
 			int j[SEPK_BATCH];
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++)
-				j[u] = nbr[sepk_slot(min(e0 + u, ne - 1), t, npad, sub)];
+				j[u] = lp[u * stride];
 			sepk_pos pj[SEPK_BATCH];
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++)
-				pj[u] = s.pos[j[u]];
+				pj[u] = sepk_rec(pb, j[u]);
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {
-				if (e0 + u >= ne)
-					break;
+				// the phantom (charge 0) is skipped here
 				if (fabs(pj[u].q) < DBL_EPSILON)
 					continue;
 				const double dx = sepk_image<FAST>(pi.x - pj[u].x, c.L0, c.iL0);
 				const double dy = sepk_image<FAST>(pi.y - pj[u].y, c.L1, c.iL1);
 				const double dz = sepk_image<FAST>(pi.z - pj[u].z, c.L2, c.iL2);
-				const double r2 = dx * dx + dy * dy + dz * dz;
+				const double xx = dx * dx, yy = dy * dy, zz = dz * dz;
+				const double r2 = xx + yy + zz;
 				if (!(r2 < c.cf2))
 					continue;
-				const double zz = pj[u].q * pi.q;
+				const double qq = pj[u].q * pi.q;
 				const double r = sqrt(r2);
-				const double ft = zz * (1.0 / r2 - c.icf2) / r;
-				const double gx = ft * dx, gy = ft * dy, gz = ft * dz;
-				fx += gx;
-				fy += gy;
-				fz += gz;
-				acc[0] += zz * (1.0 / r + (r - c.cf) * c.icf2 - c.icf);
-				acc[1] += gx * dx;
-				acc[2] += gx * dy;
-				acc[3] += gx * dz;
-				acc[4] += gy * dy;
-				acc[5] += gy * dz;
-				acc[6] += gz * dz;
+				const double ft = qq * (1.0 / r2 - c.icf2) / r;
+				const double e = qq * (1.0 / r + (r - c.cf) * c.icf2 - c.icf);
+				sepk_pair_accumulate(ft, e, dx, dy, dz, xx, yy, zz, fx, fy, fz, acc);
 			}
 		}
 	}

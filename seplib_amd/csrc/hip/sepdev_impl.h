@@ -16,6 +16,11 @@
 // its list reads is 256 contiguous bytes, and since every list is in cell
 // order the lanes of one atom gather neighbours from the same cache lines.
 #define SEPK_SUB SEPDEV_LANES
+// entries a lane of a pair kernel takes at a time; every atom's list is
+// padded with the phantom slot n to a multiple of SEPK_PAD entries, so a
+// lane's entries come in full batches
+#define SEPK_BATCH 4
+#define SEPK_PAD (SEPK_SUB * SEPK_BATCH)
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
 // L2); list kernels remap so that each XCD gets one contiguous range of
 // cell-sorted atoms, whose neighbours are mostly in the same range.
@@ -93,8 +98,8 @@ struct sepdev {
 
 	// cell binning
 	long ncell_cap;
-	int *cell_count, *cell_start, *cell_fill;
-	int *cellid_old, *order_tmp, *neworder;
+	int *cell_count, *cell_start;
+	int *cellid_old, *cell_slot, *order_tmp;
 
 	// full neighbour list: entry k of atom t at slot
 	// ((k/SEPK_SUB)*npad + t)*SEPK_SUB + k%SEPK_SUB, nbr_count[t] entries,

@@ -2,9 +2,9 @@
 //
 // A rebuild is a short kernel sequence on the library stream:
 //   k_bin          cell index per atom (reference formula or floor+wrap),
-//                  per-cell counts by atomics
+//                  per-cell counts and slots by wave-aggregated atomics
 //   k_scan         one workgroup: exclusive scan of the counts -> cell_start
-//   k_scatter      atom -> slot inside its cell segment (atomic fill)
+//   k_scatter      atom -> start of its cell + its slot
 //   k_sort_permute one wave per cell: order the segment by original atom
 //                  index (so the device order and every list are independent
 //                  of atomic timing) and move each member's state to its new
@@ -63,7 +63,11 @@ __device__ __forceinline__ long sepk_cell_of(const sepk_pos &p, const sepdev_gri
 
 #define SEPK_GRID_CAP 2048 // grid-stride launches: at most this many blocks
 
-__global__ void k_bin(const sepk_pos *pos, long n, sepdev_grid g, int *cellid,
+// Cell index of every atom and its slot among the cell's members.  The
+// lanes of a wave that share a cell take their slots with one atomic add per
+// distinct cell (atoms arrive in the last build's cell order, so a wave holds
+// a handful of runs), which keeps the device-wide atomics to a few per wave.
+__global__ void k_bin(const sepk_pos *pos, long n, sepdev_grid g, int *cellid, int *slot,
 		      int *count, sepdev_sys *sys, int conditional)
 {
 	if (conditional && sys->neighb_flag == 0)
@@ -71,45 +75,89 @@ __global__ void k_bin(const sepk_pos *pos, long n, sepdev_grid g, int *cellid,
 	// the build's atomic maximum starts here
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 		sys->max_full = 0;
-	for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-	     t += (long)gridDim.x * blockDim.x) {
-		const sepk_pos p = pos[t];
-		long c = sepk_cell_of(p, g);
-		if (c < 0) {
-			sepk_set_error(sys, SEP_GPU_ECELL);
-			c = 0;
+	const int lane = threadIdx.x & 63;
+	const unsigned long long below = (1ull << lane) - 1ull;
+	const long stride = (long)gridDim.x * blockDim.x;
+	// whole waves iterate together (the bound is rounded up to a wave)
+	const long nw = (n + 63) / 64 * 64;
+	for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nw; t += stride) {
+		const bool act = t < n;
+		long c = 0;
+		if (act) {
+			const sepk_pos p = pos[t];
+			c = sepk_cell_of(p, g);
+			if (c < 0) {
+				sepk_set_error(sys, SEP_GPU_ECELL);
+				c = 0;
+			}
+			// an atom outside [0, n*l) on some axis (allowed: it is
+			// binned by truncation or wrapping) rules out the per-cell
+			// image shifts
+			const double q[3] = {p.x / g.l[0], p.y / g.l[1], p.z / g.l[2]};
+			if (!(q[0] >= 0.0 && q[0] < g.n[0] && q[1] >= 0.0 && q[1] < g.n[1] &&
+			      q[2] >= 0.0 && q[2] < g.n[2]))
+				sys->unclean = 1;
+			cellid[t] = (int)c;
 		}
-		// an atom outside [0, n*l) on some axis (allowed: it is binned by
-		// truncation or wrapping) rules out the per-cell image shifts
-		const double q[3] = {p.x / g.l[0], p.y / g.l[1], p.z / g.l[2]};
-		if (!(q[0] >= 0.0 && q[0] < g.n[0] && q[1] >= 0.0 && q[1] < g.n[1] &&
-		      q[2] >= 0.0 && q[2] < g.n[2]))
-			sys->unclean = 1;
-		cellid[t] = (int)c;
-		atomicAdd(&count[c], 1);
+		unsigned long long todo = __ballot(act);
+		int my = 0;
+		while (todo) {
+			const int lead = __ffsll((long long)todo) - 1;
+			const int cl = __shfl((int)c, lead, 64);
+			const unsigned long long same = __ballot(act && (int)c == cl);
+			int base = 0;
+			if (lane == lead)
+				base = atomicAdd(&count[cl], __popcll(same));
+			base = __shfl(base, lead, 64);
+			if (act && (int)c == cl)
+				my = base + __popcll(same & below);
+			todo &= ~same;
+		}
+		if (act)
+			slot[t] = my;
 	}
 }
 
-// Exclusive scan of count[0..ncell) into start[0..ncell]; also zeroes count
-// and fill for the next rebuild.  One workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan(int *count, int *start, int *fill,
-					       long ncell, sepdev_sys *sys,
-					       int conditional)
+// Exclusive scan of count[0..ncell) into start[0..ncell], zeroing count for
+// the next rebuild.  One workgroup; each thread scans SEPK_SCAN_ITEMS
+// consecutive cells of a pass in registers (one memory round trip per pass).
+#define SEPK_SCAN_ITEMS 16
+__global__ __launch_bounds__(1024) void k_scan(int *count, int *start, long ncell,
+					       sepdev_sys *sys, int conditional)
 {
 	if (conditional && sys->neighb_flag == 0)
 		return;
 	__shared__ int wsum[16];
-	__shared__ int carry;
+	__shared__ int carry_s;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-	if (threadIdx.x == 0)
-		carry = 0;
-	__syncthreads();
-	for (long base = 0; base < ncell; base += 1024) {
-		const long c = base + threadIdx.x;
-		const int v = c < ncell ? count[c] : 0;
-		int x = v; // inclusive wave scan
+	int carry = 0;
+	const long pass = 1024L * SEPK_SCAN_ITEMS;
+	for (long base = 0; base < ncell; base += pass) {
+		const long c0 = base + (long)threadIdx.x * SEPK_SCAN_ITEMS;
+		int v[SEPK_SCAN_ITEMS];
+		if (c0 + SEPK_SCAN_ITEMS <= ncell) {
+			// 16-byte aligned (count comes from hipMalloc, c0 % 16 == 0)
+			const int4 *q = (const int4 *)(count + c0);
+#pragma unroll
+			for (int k = 0; k < SEPK_SCAN_ITEMS / 4; k++) {
+				const int4 w = q[k];
+				v[4 * k] = w.x;
+				v[4 * k + 1] = w.y;
+				v[4 * k + 2] = w.z;
+				v[4 * k + 3] = w.w;
+			}
+		} else {
+#pragma unroll
+			for (int k = 0; k < SEPK_SCAN_ITEMS; k++)
+				v[k] = c0 + k < ncell ? count[c0 + k] : 0;
+		}
+		int tot = 0;
+#pragma unroll
+		for (int k = 0; k < SEPK_SCAN_ITEMS; k++)
+			tot += v[k];
+		int x = tot; // inclusive wave scan of the thread totals
 		for (int o = 1; o < 64; o <<= 1) {
-			int y = __shfl_up(x, o, 64);
+			const int y = __shfl_up(x, o, 64);
 			if (lane >= o)
 				x += y;
 		}
@@ -119,39 +167,40 @@ __global__ __launch_bounds__(1024) void k_scan(int *count, int *start, int *fill
 		int off = carry;
 		for (int w = 0; w < wid; w++)
 			off += wsum[w];
-		if (c < ncell) {
-			start[c] = off + x - v;
-			count[c] = 0;
-			fill[c] = 0;
+		off += x - tot;
+#pragma unroll
+		for (int k = 0; k < SEPK_SCAN_ITEMS; k++) {
+			if (c0 + k < ncell) {
+				start[c0 + k] = off;
+				count[c0 + k] = 0;
+			}
+			off += v[k];
 		}
+		if (threadIdx.x == 1023)
+			carry_s = off;
 		__syncthreads();
-		if (threadIdx.x == 1023) {
-			int tot = carry;
-			for (int w = 0; w < 16; w++)
-				tot += wsum[w];
-			carry = tot;
-		}
+		carry = carry_s;
 		__syncthreads();
 	}
 	if (threadIdx.x == 0)
 		start[ncell] = carry;
 }
 
-__global__ void k_scatter(const int *cellid, const int *start, int *fill,
-			  int *order, long n, sepdev_sys *sys, int conditional)
+// Atom t -> its place in the cell-sorted order (slot from k_bin).
+__global__ void k_scatter(const int *cellid, const int *slot, const int *start, int *order,
+			  long n, sepdev_sys *sys, int conditional)
 {
 	if (conditional && sys->neighb_flag == 0)
 		return;
 	for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-	     t += (long)gridDim.x * blockDim.x) {
-		const int c = cellid[t];
-		const int slot = start[c] + atomicAdd(&fill[c], 1);
-		order[slot] = (int)t;
-	}
+	     t += (long)gridDim.x * blockDim.x)
+		order[start[cellid[t]] + slot[t]] = (int)t;
 }
 
 // One wave per cell (grid-stride over cells): rank each member by its
-// original index and move its state to start + rank in the spare copy.
+// original index -- the keys of the cell are held one per lane and compared
+// through wave shuffles -- and move its state to start + rank in the spare
+// copy.
 __global__ void k_sort_permute(const int *order, const int *start, sepk_state src,
 			       sepk_state dst, int *iperm, long ncell,
 			       long npad, sepdev_sys *sys, int conditional,
@@ -164,12 +213,21 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 	const long waves = (long)gridDim.x * (blockDim.x / 64);
 	for (long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64; c < ncell; c += waves) {
 		const int s = start[c], e = start[c + 1];
-		for (int i = s + lane; i < e; i += 64) {
-			const long o = order[i];
-			const int key = src.orig[o];
+		for (int i0 = s; i0 < e; i0 += 64) {
+			const int i = i0 + lane;
+			const long o = i < e ? order[i] : 0;
+			const int key = i < e ? src.orig[o] : 0;
 			int rank = 0;
-			for (int j = s; j < e; j++)
-				rank += src.orig[order[j]] < key;
+			for (int b = s; b < e; b += 64) {
+				// keys of members b .. b+63, one per lane
+				const int kj = b == i0 ? key
+						       : (b + lane < e ? src.orig[order[b + lane]] : 0);
+				const int m = min(64, e - b);
+				for (int jj = 0; jj < m; jj++)
+					rank += __shfl(kj, jj, 64) < key;
+			}
+			if (i >= e)
+				continue;
 			const long t = s + rank;
 			const sepk_pos p = src.pos[o];
 			dst.pos[t] = p;
@@ -362,8 +420,8 @@ __device__ __forceinline__ void sepk_build_chunk(
 template <bool SHIFT, int EXCL>
 __device__ __forceinline__ void sepk_build_cell(
 	sepk_build_lds &L, const sepk_state &s, long c, const int *start,
-	const sepdev_grid &g, long npad, int cap, int *nbr, int *cnt, sepdev_sys *sys,
-	int &mymax)
+	const sepdev_grid &g, long npad, int cap, int phantom, int *nbr, int *cnt,
+	sepdev_sys *sys, int &mymax)
 {
 	const int lane = threadIdx.x & 63;
 	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
@@ -457,13 +515,19 @@ __device__ __forceinline__ void sepk_build_cell(
 									    npad, cap, nbr, mreg);
 			}
 		}
-		// lane kk of wave w holds the length of home atom w + 4 kk
+		// lane kk of wave w holds the length of home atom w + 4 kk; the
+		// list is padded with the phantom slot to a multiple of SEPK_PAD
 		const int i = wid + (SEPK_BT / 64) * lane;
 		if (lane < SEPK_BHOME / (SEPK_BT / 64) && i < hn) {
-			cnt[hs + hb + i] = mreg < cap ? mreg : cap;
+			const long t = hs + hb + i;
+			cnt[t] = mreg < cap ? mreg : cap;
 			if (mreg > cap)
 				sepk_set_error(sys, SEP_GPU_ENEIGHB);
 			mymax = max(mymax, mreg);
+			const int pad = min((mreg + SEPK_PAD - 1) / SEPK_PAD * SEPK_PAD, cap);
+			const size_t stride = (size_t)npad * SEPK_SUB;
+			for (int e = mreg; e < pad; e++)
+				nbr[(size_t)t * SEPK_SUB + (size_t)(e >> 2) * stride + (e & 3)] = phantom;
 		}
 		if (hb + SEPK_BHOME < h)
 			__syncthreads(); // the next batch restages the candidates
@@ -476,8 +540,9 @@ __device__ __forceinline__ void sepk_build_cell(
 // batch, so its memory round trip is paid once.
 template <int EXCL>
 __global__ __launch_bounds__(SEPK_BT) void k_build(
-	sepk_state s, sepk_state home, const int *start, sepdev_grid g, long ncell, long npad,
-	int cap, int *nbr, int *cnt, sepdev_sys *sys, int conditional, int has_partners)
+	sepk_state s, sepk_state home, const int *start, sepdev_grid g, long ncell, long n,
+	long npad, int cap, int *nbr, int *cnt, sepdev_sys *sys, int conditional,
+	int has_partners)
 {
 	if (conditional && sys->neighb_flag == 0)
 		return;
@@ -490,9 +555,11 @@ __global__ __launch_bounds__(SEPK_BT) void k_build(
 	int mymax = 0;
 	for (long c = lo + k; c < hi; c += per) {
 		if (shift)
-			sepk_build_cell<true, EXCL>(L, s, c, start, g, npad, cap, nbr, cnt, sys, mymax);
+			sepk_build_cell<true, EXCL>(L, s, c, start, g, npad, cap, (int)n, nbr, cnt,
+						    sys, mymax);
 		else
-			sepk_build_cell<false, EXCL>(L, s, c, start, g, npad, cap, nbr, cnt, sys, mymax);
+			sepk_build_cell<false, EXCL>(L, s, c, start, g, npad, cap, (int)n, nbr, cnt,
+						     sys, mymax);
 	}
 	for (long c = lo + k; c < hi; c += per) {
 		const int a = start[c], b = start[c + 1];
@@ -566,12 +633,9 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 			return sepk_fail(SEP_GPU_ESTATE, "cell grid grew during graph capture");
 		(void)hipFree(d->cell_count);
 		(void)hipFree(d->cell_start);
-		(void)hipFree(d->cell_fill);
 		SEPK_CHECK(hipMalloc(&d->cell_count, ncell * sizeof(int)));
 		SEPK_CHECK(hipMalloc(&d->cell_start, (ncell + 1) * sizeof(int)));
-		SEPK_CHECK(hipMalloc(&d->cell_fill, ncell * sizeof(int)));
 		SEPK_CHECK(hipMemsetAsync(d->cell_count, 0, ncell * sizeof(int), d->st));
-		SEPK_CHECK(hipMemsetAsync(d->cell_fill, 0, ncell * sizeof(int), d->st));
 		d->ncell_cap = ncell;
 	}
 	if (!d->nbr) {
@@ -601,11 +665,12 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 	long sb = (ncell * 64 + SEPK_BLOCK - 1) / SEPK_BLOCK;
 	if (sb > SEPK_GRID_CAP)
 		sb = SEPK_GRID_CAP;
-	k_bin<<<nb, SEPK_BLOCK, 0, d->st>>>(cur.pos, d->n, g, d->cellid_old, d->cell_count, d->sys,
+	k_bin<<<nb, SEPK_BLOCK, 0, d->st>>>(cur.pos, d->n, g, d->cellid_old, d->cell_slot,
+					    d->cell_count, d->sys,
 					    conditional);
-	k_scan<<<1, 1024, 0, d->st>>>(d->cell_count, d->cell_start, d->cell_fill, ncell, d->sys,
+	k_scan<<<1, 1024, 0, d->st>>>(d->cell_count, d->cell_start, ncell, d->sys,
 				      conditional);
-	k_scatter<<<nb, SEPK_BLOCK, 0, d->st>>>(d->cellid_old, d->cell_start, d->cell_fill,
+	k_scatter<<<nb, SEPK_BLOCK, 0, d->st>>>(d->cellid_old, d->cell_slot, d->cell_start,
 						d->order_tmp, d->n, d->sys, conditional);
 	k_sort_permute<<<(unsigned)sb, SEPK_BLOCK, 0, d->st>>>(
 		d->order_tmp, d->cell_start, cur, spare, d->iperm, ncell, d->npad,
@@ -615,7 +680,7 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 	if (bb > SEPK_BGRID)
 		bb = SEPK_BGRID;
 	auto kb = g.opt == 2 ? k_build<2> : g.opt == 3 ? k_build<3> : k_build<0>;
-	kb<<<(unsigned)bb, SEPK_BT, 0, d->st>>>(spare, cur, d->cell_start, g, ncell, d->npad,
+	kb<<<(unsigned)bb, SEPK_BT, 0, d->st>>>(spare, cur, d->cell_start, g, ncell, d->n, d->npad,
 						      d->cap, d->nbr, d->nbr_count, d->sys,
 						      conditional, hp);
 	SEPK_LAUNCH_CHECK();

@@ -76,26 +76,32 @@ __device__ static void fin_apply(const sepk_task &k, const double *tot, double *
 	}
 }
 
-// SEPK_FIN_BLOCKS workgroups.  Each takes one contiguous chunk of the rows
-// of every task (one thread per row, SEPK_MAXNV columns loaded at once with
-// clamped addresses) and reduces it in a fixed order -- thread-strided rows,
-// wave butterfly, waves in index order -- to one value per column.  The
-// last workgroup to arrive combines the SEPK_FIN_BLOCKS values of each
-// column in block order and applies the tasks in queue order.  Neither
+// SEPK_FIN_BLOCKS workgroups of SEPK_MAXTASK waves.  Wave w of every
+// workgroup takes task w: a contiguous chunk of its rows, one lane per row
+// (SEPK_MAXNV columns loaded at once with clamped addresses, all tasks'
+// loads in flight together), reduced in a fixed order -- lane-strided rows,
+// then a wave butterfly -- to one value per column.  The last workgroup to
+// arrive combines the SEPK_FIN_BLOCKS values of each column in block order
+// (wave w again for task w) and applies the tasks in queue order.  Neither
 // step depends on timing, so repeated runs give identical bits.
 __device__ static double fin_comb(bool mx, double a, double b)
 {
 	return mx ? fmax(a, b) : a + b;
 }
 
+static_assert(FIN_THREADS == 64 * SEPK_MAXTASK, "one finisher wave per task");
+
 __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, sepk_fin f)
 {
-	__shared__ double wred[SEPK_MAXNV][FIN_THREADS / 64];
-	__shared__ double tot[SEPK_MAXNV];
+	__shared__ double tot[SEPK_MAXTASK][SEPK_MAXNV];
+	__shared__ double sret[SEPK_MAXTASK][SEPDEV_RET_WORDS];
+	__shared__ double salpha[SEPK_MAXTASK];
+	__shared__ sepdev_sys ssys;
 	__shared__ int last;
-	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63;
+	const int ti = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // this wave's task
 	const int G = gridDim.x;
-	for (int ti = 0; ti < f.ntask; ti++) {
+	if (ti < f.ntask) {
 		const sepk_task &k = f.t[ti];
 		const double *p = partials + (size_t)k.region * f.region_words;
 		const long chunk = (k.nblk + G - 1) / G;
@@ -105,7 +111,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, 
 #pragma unroll
 		for (int i = 0; i < SEPK_MAXNV; i++)
 			a[i] = 0.0;
-		for (long r = r0 + threadIdx.x; r < r1; r += FIN_THREADS) {
+		for (long r = r0 + lane; r < r1; r += 64) {
 			double v[SEPK_MAXNV];
 #pragma unroll
 			for (int i = 0; i < SEPK_MAXNV; i++)
@@ -120,83 +126,76 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, 
 			double x = a[i];
 			for (int o = 32; o > 0; o >>= 1)
 				x = fin_comb(mx, x, __shfl_xor(x, o, 64));
-			if (lane == 0)
-				wred[i][wid] = x;
+			if (lane == 0 && i < k.nv)
+				f.l2[((size_t)ti * SEPK_MAXNV + i) * G + blockIdx.x] = x;
 		}
-		__syncthreads();
-		if (threadIdx.x < k.nv) {
-			const int i = threadIdx.x;
-			const bool mx = (k.maxmask >> i) & 1u;
-			double x = wred[i][0];
-			for (int w = 1; w < FIN_THREADS / 64; w++)
-				x = fin_comb(mx, x, wred[i][w]);
-			f.l2[((size_t)ti * SEPK_MAXNV + i) * G + blockIdx.x] = x;
-		}
-		__syncthreads();
 	}
-	// publish this block's rows, then count it in
-	if (threadIdx.x == 0) {
-		__threadfence();
+	// publish this block's values (every wave that stored fences its own
+	// stores), then count the block in
+	__threadfence();
+	__syncthreads();
+	if (threadIdx.x == 0)
 		last = atomicAdd(f.count, 1u) == (unsigned)(G - 1);
-	}
 	__syncthreads();
 	if (!last)
 		return;
 	__threadfence();
 	// The tasks read and update the system scalars, their return slots and
-	// thermostat variables: one parallel load of each into LDS, the tasks
-	// applied there in queue order, one parallel store back -- instead of a
-	// chain of dependent global round trips in a single thread.
-	__shared__ double sret[SEPK_MAXTASK][SEPDEV_RET_WORDS];
-	__shared__ double salpha[SEPK_MAXTASK];
-	__shared__ sepdev_sys ssys;
+	// thermostat variables: one parallel load of each into LDS (issued with
+	// the level-2 loads), the tasks applied there in queue order, one
+	// parallel store back.
 	int rmap[SEPK_MAXTASK], amap[SEPK_MAXTASK]; // first task sharing the pointer
-	for (int ti = 0; ti < f.ntask; ti++) {
-		rmap[ti] = amap[ti] = -1;
-		for (int tj = ti; tj >= 0; tj--) {
-			if (f.t[ti].ret && f.t[tj].ret == f.t[ti].ret)
-				rmap[ti] = tj;
-			if (f.t[ti].alpha && f.t[tj].alpha == f.t[ti].alpha)
-				amap[ti] = tj;
+	for (int u = 0; u < f.ntask; u++) {
+		rmap[u] = amap[u] = -1;
+		for (int v = u; v >= 0; v--) {
+			if (f.t[u].ret && f.t[v].ret == f.t[u].ret)
+				rmap[u] = v;
+			if (f.t[u].alpha && f.t[v].alpha == f.t[u].alpha)
+				amap[u] = v;
 		}
 	}
 	const int nsys = (int)(sizeof(sepdev_sys) / sizeof(int));
 	for (int w = threadIdx.x; w < f.ntask * SEPDEV_RET_WORDS; w += FIN_THREADS) {
-		const int ti = w / SEPDEV_RET_WORDS, word = w % SEPDEV_RET_WORDS;
-		if (rmap[ti] == ti)
-			sret[ti][word] = f.t[ti].ret[word];
+		const int u = w / SEPDEV_RET_WORDS, word = w % SEPDEV_RET_WORDS;
+		if (rmap[u] == u)
+			sret[u][word] = f.t[u].ret[word];
 	}
 	if (threadIdx.x < f.ntask && amap[threadIdx.x] == (int)threadIdx.x)
 		salpha[threadIdx.x] = *f.t[threadIdx.x].alpha;
 	if (threadIdx.x >= 64 && threadIdx.x < 64 + nsys)
 		((int *)&ssys)[threadIdx.x - 64] = ((const int *)f.sys)[threadIdx.x - 64];
-	__syncthreads();
-	for (int ti = 0; ti < f.ntask; ti++) {
+	if (ti < f.ntask) {
 		const sepk_task &k = f.t[ti];
-		// a conditional build that did not run leaves nothing to finish
-		const bool skip = k.mode == SEPK_FIN_BUILD && k.conditional && ssys.neighb_flag == 0;
-		if (!skip) {
-			// wave w combines columns w, w + 4, ...: lane b holds block b
-			for (int i = wid; i < k.nv; i += FIN_THREADS / 64) {
-				const bool mx = (k.maxmask >> i) & 1u;
-				const double *col = f.l2 + ((size_t)ti * SEPK_MAXNV + i) * G;
-				double x = lane < G ? col[lane] : 0.0;
-				for (int o = 32; o > 0; o >>= 1)
-					x = fin_comb(mx, x, __shfl_xor(x, o, 64));
-				if (lane == 0)
-					tot[i] = x;
-			}
-			__syncthreads();
-			if (threadIdx.x == 0)
-				fin_apply(k, tot, rmap[ti] >= 0 ? sret[rmap[ti]] : nullptr,
-					  amap[ti] >= 0 ? &salpha[amap[ti]] : nullptr, &ssys);
+		double x[SEPK_MAXNV];
+#pragma unroll
+		for (int i = 0; i < SEPK_MAXNV; i++) {
+			const double *col = f.l2 + ((size_t)ti * SEPK_MAXNV + (i < k.nv ? i : 0)) * G;
+			x[i] = lane < G ? col[lane] : 0.0;
 		}
-		__syncthreads();
+#pragma unroll
+		for (int i = 0; i < SEPK_MAXNV; i++) {
+			const bool mx = (k.maxmask >> i) & 1u;
+			for (int o = 32; o > 0; o >>= 1)
+				x[i] = fin_comb(mx, x[i], __shfl_xor(x[i], o, 64));
+			if (lane == 0)
+				tot[ti][i] = x[i];
+		}
 	}
+	__syncthreads();
+	if (threadIdx.x == 0)
+		for (int u = 0; u < f.ntask; u++) {
+			const sepk_task &k = f.t[u];
+			// a conditional build that did not run leaves nothing to finish
+			if (k.mode == SEPK_FIN_BUILD && k.conditional && ssys.neighb_flag == 0)
+				continue;
+			fin_apply(k, tot[u], rmap[u] >= 0 ? sret[rmap[u]] : nullptr,
+				  amap[u] >= 0 ? &salpha[amap[u]] : nullptr, &ssys);
+		}
+	__syncthreads();
 	for (int w = threadIdx.x; w < f.ntask * SEPDEV_RET_WORDS; w += FIN_THREADS) {
-		const int ti = w / SEPDEV_RET_WORDS, word = w % SEPDEV_RET_WORDS;
-		if (rmap[ti] == ti)
-			f.t[ti].ret[word] = sret[ti][word];
+		const int u = w / SEPDEV_RET_WORDS, word = w % SEPDEV_RET_WORDS;
+		if (rmap[u] == u)
+			f.t[u].ret[word] = sret[u][word];
 	}
 	if (threadIdx.x < f.ntask && amap[threadIdx.x] == (int)threadIdx.x)
 		*f.t[threadIdx.x].alpha = salpha[threadIdx.x];
