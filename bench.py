@@ -155,6 +155,26 @@ def cpu_baseline(w, steps=200, warm=20):
                        "%.1f s" % (steps, warm, threads, secs))
 
 
+def energy_drift(api, w, precision, nvt_steps, nve_steps, every=100):
+    """NVT equilibration, then NVE (no thermostat) with the total energy
+    epot + ekin read every `every` steps: relative drift |E_end - E_0| / |E_0|
+    and the largest excursion max_t |E(t) - E_0| / |E_0|."""
+    s = make_system(api, w, precision)
+    s.gpu_resident(True)
+    s.gpu_run(step_ops(s), nvt_steps, graph=True)
+    nve = [s.op_pairs("AA", RC, "sep_lj_shift"), s.op_leapfrog()]
+    energies = []
+    for k in range(nve_steps // every):
+        s.gpu_run(nve, every, graph=True)
+        s.gpu_sync()
+        energies.append(s.ret.epot + s.ret.ekin)
+    s.close()
+    e0 = energies[0]
+    drift = abs(energies[-1] - e0) / abs(e0)
+    excursion = max(abs(e - e0) for e in energies) / abs(e0)
+    return drift, excursion, e0 / len(w["m"])
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lj_lj256k_traffic.json")
 
 
@@ -188,6 +208,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--copy-calib", action="store_true")
+    ap.add_argument("--drift", action="store_true",
+                    help="also measure the NVE energy drift (1e4 NVT + 1e4 NVE steps per form)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -233,6 +255,14 @@ def main():
                     kernel=kname + " (mixed pair forces)", kernel_us=round(m["kms"] * 1e3, 3),
                     algorithmic_bytes=round(bytes_per_atom * n),
                     bytes_per_atom=round(bytes_per_atom, 2))
+    drift = None
+    if args.drift:
+        drift = {}
+        for name, prec in (("fp64", S.SEP_GPU_FP64), ("mixed", S.SEP_GPU_MIXED)):
+            d, x, e0 = energy_drift(api, w, prec, 10000, 10000)
+            drift[name] = dict(drift=d, max_excursion=x, e0_per_atom=e0)
+            log("%s NVE over 1e4 steps after 1e4 NVT: drift %.3e, max excursion %.3e "
+                "(E0/N %.6f)" % (name, d, x, e0))
     cpu = None if args.no_cpu else cpu_baseline(w, steps=args.cpu_steps)
     if cpu:
         log("cpu baseline: %.4g atom-steps/s on %d threads" % (cpu["value"], cpu["cores"]))
@@ -261,6 +291,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if drift is not None:
+        out["nve_energy_drift"] = drift
     print(json.dumps(out), flush=True)
 
 
