@@ -95,6 +95,25 @@ typedef struct {
 int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret,
 		const sep_gpu_op *ops, int nops, long nsteps, int flags);
 
+/* ---- multi-type LJ and force matching --------------------------------- */
+/* The pair forces of sep_force_lj (include/sepprfrc.h:69, source/
+ * sepprfrc.c:743-923) for every pair of the ntypes labels in `types` in one
+ * list pass: params[(i * ntypes + k) * 4 + {0,1,2,3}] = {rc, eps, sigma, aw}
+ * of type pair (types[i], types[k]) (symmetric), at most 3 types.  Adds to
+ * f, ret.epot and ret.pot_P as the reference's three calls would. */
+int sep_gpu_force_lj_table(seppart *atoms, int ntypes, const char *types,
+			   const double *params, sepsys *sys, sepret *ret,
+			   const unsigned opt);
+
+/* Adjoint of a force-matching loss L(F) with F the forces above, on the
+ * current positions and neighbour list (call sep_gpu_force_lj_table on the
+ * same positions first): dLdF (3n, original order, dL/dF_i) in; out dL/dx
+ * (3n) and dLdtheta[(i * ntypes + k) * 2 + {0,1}] = dL/d eps, dL/d sigma of
+ * pair (i, k).  Cutoffs are held fixed.  No reference counterpart. */
+int sep_gpu_lj_adjoint(seppart *atoms, sepsys *sys, int ntypes, const char *types,
+		       const double *params, const double *dLdF, double *dLdx,
+		       double *dLdtheta);
+
 /* ---- diagnostics ----------------------------------------------------- */
 /* Unordered neighbour pairs (i < j, original atom indices) of the current
  * device list, written as pairs[2k], pairs[2k+1]; *npairs receives the

@@ -191,6 +191,10 @@ extern "C" void sepdev_destroy(sepdev *d)
 		(void)hipHostFree(d->hstage);
 	free(d->ret_zero);
 	free(d->last_lj);
+	void *adj[] = {d->adj_g, d->adj_x, d->adj_out, d->adj_part};
+	for (void *p : adj)
+		if (p)
+			(void)hipFree(p);
 	if (d->timing) {
 		for (int i = 0; i < 64; i++) {
 			(void)hipEventDestroy(d->t_pair.beg[i]);
@@ -361,6 +365,25 @@ extern "C" int sepdev_get_vec(sepdev *d, int which, double *dst3n)
 			return sepk_fail(SEP_GPU_EARG, "get_vec: field %d", which);
 		k_gather_vec<<<nb, SEPK_BLOCK, 0, d->st>>>((double *)d->dstage, src, d->iperm, d->n, d->npad);
 	}
+	SEPK_LAUNCH_CHECK();
+	return stage_out(d, dst3n, (size_t)d->n * 3 * sizeof(double));
+}
+
+// A host 3n array in original order into / out of any 3 x npad device block
+// in device order (adjoint inputs and outputs).
+int sepk_put_vec_to(sepdev *d, double *dst, const double *src3n)
+{
+	int rc = stage_in(d, src3n, (size_t)d->n * 3 * sizeof(double));
+	if (rc)
+		return rc;
+	k_scatter_vec<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(dst, (const double *)d->dstage, d->iperm, d->n, d->npad);
+	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+int sepk_get_vec_from(sepdev *d, double *dst3n, const double *src)
+{
+	k_gather_vec<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>((double *)d->dstage, src, d->iperm, d->n, d->npad);
 	SEPK_LAUNCH_CHECK();
 	return stage_out(d, dst3n, (size_t)d->n * 3 * sizeof(double));
 }

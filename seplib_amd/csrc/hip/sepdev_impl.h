@@ -155,6 +155,10 @@ struct sepdev {
 	const void *last_lj_kernel;
 	void *last_lj;
 
+	// force-matching adjoint: g and dL/dx (3 x npad), parameter-gradient
+	// partial rows and sums (allocated on first use)
+	double *adj_g, *adj_x, *adj_out, *adj_part;
+
 	// graph
 	hipGraph_t graph;
 	hipGraphExec_t exec;
@@ -298,4 +302,6 @@ __device__ __forceinline__ void sepk_add_sym9(double *dst9, const double *w6,
 // deferred-work helpers (sepdev_force.hip, sepdev_reduce.hip)
 int sepk_materialize_forces(sepdev *d);
 int sepk_flush_nh(sepdev *d);
+int sepk_put_vec_to(sepdev *d, double *dst, const double *src3n);
+int sepk_get_vec_from(sepdev *d, double *dst3n, const double *src);
 void sepk_task_ret(sepdev *d, sepk_task &t, int slot);

@@ -57,6 +57,7 @@ typedef struct {
 	unsigned opt;       /* exclusion rule                        */
 	sepdev_lj_args lj;  /* constants; cf2/types filled here      */
 	int epot_assign_list;
+	const sepdev_ljtab_args *table; /* multi-type table instead of lj */
 } pair_call;
 
 /* True when the type filter of source/sepprfrc.c:116-121 keeps every pair:
@@ -135,24 +136,34 @@ static int run_pairs(seppart *ptr, sepsys *sys, sepret *ret, pair_call *pc)
 		sys->neighb_flag = 0;
 	}
 
-	pc->lj.cf2 = pc->cf * pc->cf;
-	for (int k = 0; k < 3; k++)
-		pc->lj.L[k] = sys->length[k];
-	pc->lj.t0 = pc->types[0];
-	pc->lj.t1 = pc->types[1];
-	pc->lj.typed = !types_cover_all(c, pc->types);
-	pc->lj.epot_assign = brute ? 0 : pc->epot_assign_list;
-	pc->lj.mixed = c->precision == SEP_GPU_MIXED;
 	const int slot = ret_slot_for(c, ret);
 	if (slot < 0)
 		return sep_gpu_last_error();
-	if ((rc = seph_dev_fail(sepdev_lj(c->dev, &pc->lj, slot))))
-		return rc;
+	int assign = 0;
+	if (pc->table) {
+		sepdev_ljtab_args ta = *pc->table;
+		for (int k = 0; k < 3; k++)
+			ta.L[k] = sys->length[k];
+		if ((rc = seph_dev_fail(sepdev_lj_table(c->dev, &ta, slot))))
+			return rc;
+	} else {
+		pc->lj.cf2 = pc->cf * pc->cf;
+		for (int k = 0; k < 3; k++)
+			pc->lj.L[k] = sys->length[k];
+		pc->lj.t0 = pc->types[0];
+		pc->lj.t1 = pc->types[1];
+		pc->lj.typed = !types_cover_all(c, pc->types);
+		pc->lj.epot_assign = brute ? 0 : pc->epot_assign_list;
+		pc->lj.mixed = c->precision == SEP_GPU_MIXED;
+		assign = pc->lj.epot_assign;
+		if ((rc = seph_dev_fail(sepdev_lj(c->dev, &pc->lj, slot))))
+			return rc;
+	}
 	if (c->resident)
 		return 0;
 	if ((rc = seph_get_f(c)))
 		return rc;
-	return sync_ret_end(c, ret, pc->lj.epot_assign, 0);
+	return sync_ret_end(c, ret, assign, 0);
 }
 
 int sep_force_pairs(seppart *ptr, const char *types, double cf,
@@ -459,4 +470,49 @@ void sep_leapfrog(seppart *ptr, sepsys *sys, sepret *retval)
 out:
 	if (rc)
 		sep_warning("sep_leapfrog: %s", (char *)sep_gpu_last_error_message());
+}
+
+/* ---- multi-type LJ table and its force-matching adjoint (extension) ---- */
+
+int sep_gpu_force_lj_table(seppart *ptr, int ntypes, const char *types,
+			   const double *params, sepsys *sys, sepret *retval,
+			   const unsigned opt)
+{
+	double cmax = 0.0;
+	for (int i = 0; i < ntypes * ntypes; i++)
+		if (params[4 * i] > cmax)
+			cmax = params[4 * i];
+	if (cmax > sys->cf)
+		return seph_fail(SEP_GPU_EARG, "a table cutoff exceeds the maximum cutoff %g", sys->cf);
+	sepdev_ljtab_args ta;
+	memset(&ta, 0, sizeof ta);
+	ta.ntypes = ntypes;
+	ta.types = types;
+	ta.params = params;
+	pair_call pc;
+	memset(&pc, 0, sizeof pc);
+	pc.types = "\0\0";
+	pc.cf = cmax;
+	pc.opt = opt;
+	pc.table = &ta;
+	return run_pairs(ptr, sys, retval, &pc);
+}
+
+int sep_gpu_lj_adjoint(seppart *ptr, sepsys *sys, int ntypes, const char *types,
+		       const double *params, const double *dLdF, double *dLdx, double *dLdtheta)
+{
+	seph_ctx *c = seph_ctx_for(ptr, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	int rc;
+	if (!c->resident && ((rc = seph_put_x(c)) || (rc = seph_put_scalars(c))))
+		return rc;
+	sepdev_ljtab_args ta;
+	memset(&ta, 0, sizeof ta);
+	ta.ntypes = ntypes;
+	ta.types = types;
+	ta.params = params;
+	for (int k = 0; k < 3; k++)
+		ta.L[k] = sys->length[k];
+	return seph_dev_fail(sepdev_lj_adjoint(c->dev, &ta, dLdF, dLdx, dLdtheta));
 }

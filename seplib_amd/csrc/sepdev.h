@@ -133,6 +133,22 @@ typedef struct {
 } sepdev_coul_args;
 int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot);
 
+/* multi-type LJ from a type-pair table (one list pass) and the adjoint of a
+ * force-matching loss.  params[(i * ntypes + k) * 4 + {0,1,2,3}] =
+ * {rc, eps, sigma, aw} of type pair (types[i], types[k]), symmetric. */
+#define SEPDEV_MAXT 3
+typedef struct {
+	int ntypes;
+	const char *types;
+	const double *params;
+	double L[3];
+} sepdev_ljtab_args;
+int sepdev_lj_table(sepdev *d, const sepdev_ljtab_args *a, int ret_slot);
+/* g3n: dL/dF per atom (original order, 3n); out: dL/dx (3n) and
+ * dldtheta[(i * ntypes + k) * 2 + {0,1}] = dL/d eps, dL/d sigma */
+int sepdev_lj_adjoint(sepdev *d, const sepdev_ljtab_args *a, const double *g3n,
+		      double *dldx3n, double *dldtheta);
+
 /* bonded terms; lists hold ORIGINAL atom indices as in sepmolinfo */
 int sepdev_put_topology(sepdev *d, const unsigned *blist, int nb,
 			const unsigned *alist, int na, const unsigned *dlist,

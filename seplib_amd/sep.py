@@ -193,6 +193,9 @@ def load_product(path=PRODUCT_LIB):
     f("sep_gpu_timing_get", C.c_int, P_ATOMS, C.c_char_p, P_D, C.POINTER(C.c_long))
     f("sep_gpu_list_stats", C.c_int, P_ATOMS, P_D, C.POINTER(C.c_int))
     f("sep_gpu_time_pair_kernel", C.c_int, P_ATOMS, C.c_int, P_D)
+    f("sep_gpu_force_lj_table", C.c_int, P_ATOMS, C.c_int, C.c_char_p, P_D, P_SYS, P_RET,
+      C.c_uint)
+    f("sep_gpu_lj_adjoint", C.c_int, P_ATOMS, P_SYS, C.c_int, C.c_char_p, P_D, P_D, P_D, P_D)
     f("sep_gpu_synchronize", C.c_int, P_ATOMS)
     return lib
 
