@@ -141,6 +141,55 @@ int sep_gpu_list_stats(seppart *atoms, double *mean_full, int *max_full);
 /* Device synchronisation barrier for timing. */
 int sep_gpu_synchronize(seppart *atoms);
 
+/* ---- batches of independent systems ---------------------------------- */
+/* No counterpart in the reference, whose API advances one system per call
+ * sequence.  A batch holds nsys systems, each the caller's own seppart
+ * array and sepsys as set up for the reference's step loop
+ * (prgs/prg1.c:55-65): sep_reset_retval, sep_reset_force,
+ * sep_force_pairs(atoms, types, cf, sep_lj_shift, sys, ret, SEP_ALL),
+ * sep_nosehoover(atoms, Td[s], &alpha[s], tau[s], sys), sep_leapfrog.
+ * Every atom must carry the type types[0] (== types[1]); sys->cf and
+ * sys->skin give each system's list cutoff, which must stay below half its
+ * box; cells per axis = floor(L / (cf + skin)) and, unlike the reference's
+ * builder, a box with fewer than three cells on an axis still lists every
+ * pair once from each end.  The systems are advanced together, one
+ * persistent launch per step (replayed from a HIP graph) after the list
+ * rebuild launches, which return at once for systems whose rebuild flag is
+ * clear.  precision: SEP_GPU_FP64 or SEP_GPU_MIXED; cap: per-atom full-list
+ * capacity (0: 256).  Returns NULL on failure (sep_gpu_last_error()). */
+typedef struct sep_gpu_batch sep_gpu_batch;
+sep_gpu_batch *sep_gpu_batch_create(int nsys, seppart **atoms, sepsys **sys,
+				    const char *types, double cf, const double *Td,
+				    const double *tau, const double *alpha, int precision,
+				    int cap);
+void sep_gpu_batch_free(sep_gpu_batch *b);
+/* nsteps steps of every system, replayed from a captured graph
+ * (run_eager: the same launches without the graph). */
+int sep_gpu_batch_run(sep_gpu_batch *b, long nsteps);
+int sep_gpu_batch_run_eager(sep_gpu_batch *b, long nsteps);
+/* Pair forces, energy and stress at the current positions in the given
+ * precision (nothing moves; f holds the pair forces afterwards). */
+int sep_gpu_batch_eval(sep_gpu_batch *b, int precision);
+/* Copy every system's x, v, f, a, crossings and sys->{max_dist2, tnow,
+ * nupdate_neighb, neighb_flag} back; rets[s] gets epot, ekin, pot_P and
+ * kin_P of the last step (or eval) and alpha[s] the thermostat state
+ * (either may be NULL). */
+int sep_gpu_batch_sync(sep_gpu_batch *b, sepret *rets, double *alpha);
+int sep_gpu_batch_synchronize(sep_gpu_batch *b);
+/* Tile shape: lanes per atom of each system (1, 2, 4, 8 or 16; NULL keeps
+ * them) and the persistent grid in workgroups (<= 0 keeps it). */
+int sep_gpu_batch_set_shape(sep_gpu_batch *b, const int *lanes, int grid);
+int sep_gpu_batch_get_shape(sep_gpu_batch *b, int *lanes, int *grid);
+/* Mean duration (ms) of a force evaluation over the systems with
+ * mask[s] != 0 (NULL: all), over reps back-to-back launches. */
+int sep_gpu_batch_time_eval(sep_gpu_batch *b, const int *mask, int reps, double *ms);
+/* Autotuner: for each size bucket (N < 2048, N < 8192, larger) the lanes
+ * per atom with the fastest force evaluation of that bucket's systems, then
+ * the fastest persistent grid; reps launches per candidate.  The chosen
+ * shape stays set and is returned in lanes_out[nsys] and grid_out (either
+ * may be NULL). */
+int sep_gpu_batch_tune(sep_gpu_batch *b, int reps, int *lanes_out, int *grid_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,1261 @@
+// sepdev_batch.hip -- many independent Lennard-Jones systems advanced
+// together: one persistent launch per step for all of them.
+//
+// Each system is the reference's prg1 loop (prgs/prg1.c:55-65) on its own
+// box: sep_reset_retval, sep_reset_force, sep_force_pairs("AA", cf,
+// sep_lj_shift) with the list rebuilt when flagged, sep_nosehoover and
+// sep_leapfrog.  All systems' atoms are concatenated on the device (system s
+// owns slots off[s] .. off[s] + npad[s], npad = a multiple of 64 above n,
+// slot n being a phantom with a NaN position that pads lists).  Per step:
+//
+//   k_bbin        cell of every atom of a flagged system and its slot among
+//                 the cell's members; the last wave of a system to finish
+//                 scans the system's cell counts into cell starts
+//   k_bscatter    the members of every cell, in slot order
+//   k_bsort       one wave per cell of a flagged system: rank the members
+//                 by original index (the new order, independent of the
+//                 atomic order that listed them) and move their state to it
+//                 in the spare copy
+//   k_bbuild      one wave per cell of a flagged system: the candidates of
+//                 the distinct neighbour cells, 64 at a time in the lanes,
+//                 tested against each home atom in turn and appended by
+//                 ballot in candidate order; the last cell of a system swaps
+//                 the copies
+//   k_bstep       persistent: the grid walks a table of tiles (a system and
+//                 a run of its atoms, lanes per atom chosen per system);
+//                 pair forces, Nose-Hoover and leapfrog for the tile's
+//                 atoms, a row of block sums, and the last tile of a
+//                 system to finish reduces that system's rows in tile order
+//                 and applies its finishing step (energies, stress,
+//                 thermostat, rebuild flag).
+//
+// The first four return at once for systems whose flag is clear, so the
+// five launches replay from one captured graph.  Positions are double
+// buffered inside a state copy (a tile reads slot p and writes slot 1-p;
+// other atoms' tiles still read slot p), every other per-atom field is
+// written only by its own atom.  Unlike the reference's cell-list builder,
+// a system with fewer than three cells on an axis visits each distinct
+// neighbour cell once, so every pair is listed once from each end.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sepdev_impl.h"
+
+#define SEPB_BLOCK 256
+#define SEPB_NV 16
+#define SEPB_BATCH 4
+
+// One system on the device.  Read by every kernel; written only by the
+// finishing tile of k_bstep, the last cell of k_bbuild and the host.
+struct sepb_sys {
+	long off;         // first slot
+	long list_off;    // first list slot (cap * off)
+	int n, npad;
+	int sub;          // lanes per atom of its tiles and its list layout
+	int ntile, row0;  // its tiles' partial rows: row0 .. row0 + ntile - 1
+	int ncell[3], cell_off;  // cell_count/cell_start slots: cell_off ..
+	double lcell[3], L[3], iL[3];
+	double rlist2;
+	double dt, Td, dt_tau2, skin;
+	int cur, p;       // state copy in use, position slot in use
+	unsigned neighb_flag, nupdate;
+	int error, max_full;
+	double alpha, sumv2, max_dist2, tnow;
+	unsigned arrive_step, arrive_build, arrive_bin;
+	double ret[SEPDEV_RET_WORDS];
+};
+
+// One copy of every system's per-atom state, concatenated.
+struct sepb_state {
+	sepk_pos *pos[2];
+	double *v, *f, *a, *xn; // x, y, z blocks of `slots` doubles
+	int *cn, *cr;           // x, y, z blocks of `slots` ints
+	double *m;
+	int *orig;              // original index within the system
+};
+
+struct sepb_lj {
+	double cf2, A, nB, C, nD, ushift;
+	float Af, nBf, Cf, nDf, ushiftf;
+};
+
+struct sepb_view {
+	sepb_sys *sys;
+	const sepb_state *st; // two copies, device memory
+	long slots;
+	int nsys;
+	const int *chunk_sys; // system of every 64-slot chunk
+	int *cellid, *cslot, *otmp;
+	int *cell_count, *cell_start;
+	const int *cell_sys;  // system of every global cell index
+	long ncell_tot;
+	int *nbr, *nbr_count;
+	int cap;
+	double *partial;
+	long prow;
+};
+
+struct sepdev_batch {
+	int nsys;
+	long slots, ncell_tot;
+	hipStream_t st;
+	sepb_state S[2];
+	sepb_state *dS;
+	sepb_sys *dsys;
+	sepb_sys *hsys; // host copy of the descriptors as last uploaded
+	int *chunk_sys, *cellid, *cslot, *otmp, *cell_count, *cell_start, *cell_sys;
+	int *nbr, *nbr_count;
+	int cap;
+	double *partial;
+	long prow;
+	int4 *tiles; // full table
+	int ntiles;
+	int4 *sub_tiles; // a subset (autotuner)
+	int grid;
+	sepb_lj lj;
+	hipGraph_t graph;
+	hipGraphExec_t exec;
+	int graph_mixed;
+};
+
+__device__ __forceinline__ int sepb_mod(int a, int n)
+{
+	const int r = a % n;
+	return r < 0 ? r + n : r;
+}
+
+// ---- rebuild ------------------------------------------------------------------
+
+// Exclusive scan of a system's cell counts into cell_start (one extra
+// entry: n) by one wave, and the counts back to zero for the next build.
+__device__ void sepb_scan_cells(sepb_view &b, sepb_sys *y)
+{
+	const int lane = threadIdx.x & 63;
+	const int nc = y->ncell[0] * y->ncell[1] * y->ncell[2];
+	int *cnt = b.cell_count + y->cell_off;
+	int *st = b.cell_start + y->cell_off;
+	int carry = 0;
+	for (int base = 0; base < nc; base += 64) {
+		const int c = base + lane;
+		const int v = c < nc ? cnt[c] : 0;
+		int incl = v;
+		for (int o = 1; o < 64; o <<= 1) {
+			const int u = __shfl_up(incl, o, 64);
+			if (lane >= o)
+				incl += u;
+		}
+		if (c < nc) {
+			st[c] = carry + incl - v;
+			cnt[c] = 0;
+		}
+		carry += __shfl(incl, 63, 64);
+	}
+	if (lane == 0)
+		st[nc] = carry;
+}
+
+__global__ __launch_bounds__(SEPB_BLOCK) void k_bbin(sepb_view b)
+{
+	const long stride = (long)gridDim.x * blockDim.x;
+	const int lane = threadIdx.x & 63;
+	const unsigned long long below = (1ull << lane) - 1ull;
+	// whole waves iterate together (slots is a multiple of 64)
+	for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < b.slots; g += stride) {
+		const int s = b.chunk_sys[g >> 6];
+		sepb_sys *y = b.sys + s;
+		if (!y->neighb_flag)
+			continue; // uniform over the wave: a chunk is one system
+		const long t = g - y->off;
+		const bool act = t < y->n;
+		int c = 0;
+		if (act) {
+			const sepk_pos q = b.st[y->cur].pos[y->p][g];
+			const double xs[3] = {q.x, q.y, q.z};
+			int ci[3];
+			for (int k = 0; k < 3; k++) {
+				const double u = xs[k] / y->lcell[k];
+				if (!(fabs(u) < 1.0e9)) {
+					atomicCAS(&y->error, 0, SEP_GPU_ECELL);
+					ci[k] = 0;
+				} else {
+					ci[k] = sepb_mod((int)floor(u), y->ncell[k]);
+				}
+			}
+			c = ci[0] + y->ncell[0] * (ci[1] + y->ncell[1] * ci[2]);
+			b.cellid[g] = c;
+		}
+		// one atomic per distinct cell of the wave
+		unsigned long long todo = __ballot(act);
+		int my = 0;
+		while (todo) {
+			const int lead = __ffsll((long long)todo) - 1;
+			const int cl = __shfl(c, lead, 64);
+			const unsigned long long same = __ballot(act && c == cl);
+			int base = 0;
+			if (lane == lead)
+				base = atomicAdd(&b.cell_count[y->cell_off + cl], __popcll(same));
+			base = __shfl(base, lead, 64);
+			if (act && c == cl)
+				my = base + __popcll(same & below);
+			todo &= ~same;
+		}
+		if (act)
+			b.cslot[g] = my;
+		// the system's last wave to finish scans its counts
+		__threadfence();
+		unsigned prev = 0;
+		if (lane == 0)
+			prev = atomicAdd(&y->arrive_bin, 1u);
+		prev = __shfl(prev, 0, 64);
+		if (prev == (unsigned)(y->npad / 64 - 1)) {
+			__threadfence();
+			sepb_scan_cells(b, y);
+			if (lane == 0)
+				y->arrive_bin = 0u;
+		}
+	}
+}
+
+// otmp[off + start of its cell + slot] = local index, for every atom of a
+// flagged system.
+__global__ __launch_bounds__(SEPB_BLOCK) void k_bscatter(sepb_view b)
+{
+	const long stride = (long)gridDim.x * blockDim.x;
+	for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < b.slots; g += stride) {
+		const sepb_sys *y = b.sys + b.chunk_sys[g >> 6];
+		if (!y->neighb_flag)
+			continue;
+		const long t = g - y->off;
+		if (t < y->n)
+			b.otmp[y->off + b.cell_start[y->cell_off + b.cellid[g]] + b.cslot[g]] = (int)t;
+	}
+}
+
+// Rank of `key` among the original indices of the m members seg[0..m) of a
+// cell (all distinct), by the whole wave: the members' new order within the
+// cell, independent of the atomic order that listed them.
+__device__ __forceinline__ int sepb_rank(const int *seg, const int *orig, int m, int key)
+{
+	const int lane = threadIdx.x & 63;
+	int r = 0;
+	for (int cb = 0; cb < m; cb += 64) {
+		const int kk = cb + lane < m ? orig[seg[cb + lane]] : 0x7fffffff;
+		const int lim = min(m - cb, 64);
+		for (int j = 0; j < lim; j++)
+			r += __shfl(kk, j, 64) < key;
+	}
+	return r;
+}
+
+// The distinct neighbour-cell offsets of an axis with `nc` cells.
+__device__ __forceinline__ int sepb_offsets(int nc, int *o)
+{
+	if (nc >= 3) {
+		o[0] = -1;
+		o[1] = 0;
+		o[2] = 1;
+		return 3;
+	}
+	o[0] = 0;
+	o[1] = 1;
+	return nc;
+}
+
+// One wave per cell (grid-stride): the new order of the cell's members and
+// their state moved to it.
+__global__ __launch_bounds__(SEPB_BLOCK) void k_bsort(sepb_view b)
+{
+	const int lane = threadIdx.x & 63;
+	const long nw = ((long)gridDim.x * blockDim.x) >> 6;
+	for (long gc = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gc < b.ncell_tot;
+	     gc += nw) {
+		const int s = b.cell_sys[gc];
+		if (s < 0)
+			continue; // the scan's extra entry of a system
+		const sepb_sys *y = b.sys + s;
+		if (!y->neighb_flag)
+			continue;
+		const int c = (int)(gc - y->cell_off);
+		const long off = y->off, slots = b.slots;
+		const int *st = b.cell_start + y->cell_off;
+		const int h0 = st[c], m = st[c + 1] - h0;
+		const sepb_state &So = b.st[y->cur];
+		const sepb_state &Sn = b.st[1 - y->cur];
+		const int *seg = b.otmp + off + h0;
+		const int *korig = So.orig + off;
+		for (int hb = 0; hb < m; hb += 64) {
+			const bool v = hb + lane < m;
+			const int o = v ? seg[hb + lane] : 0;
+			const int r = sepb_rank(seg, korig, m, v ? korig[o] : 0x7fffffff);
+			if (!v)
+				continue;
+			// xn = x and cross_neighb = 0 go with a rebuild
+			// (source/sepintgr.c:72-84)
+			const long go = off + o, gn = off + h0 + r;
+			const sepk_pos q = So.pos[y->p][go];
+			Sn.pos[0][gn] = q;
+			for (int k = 0; k < 3; k++) {
+				Sn.v[k * slots + gn] = So.v[k * slots + go];
+				Sn.f[k * slots + gn] = So.f[k * slots + go];
+				Sn.a[k * slots + gn] = So.a[k * slots + go];
+				Sn.cr[k * slots + gn] = So.cr[k * slots + go];
+				Sn.cn[k * slots + gn] = 0;
+			}
+			Sn.xn[gn] = q.x;
+			Sn.xn[slots + gn] = q.y;
+			Sn.xn[2 * slots + gn] = q.z;
+			Sn.m[gn] = So.m[go];
+			Sn.orig[gn] = So.orig[go];
+		}
+	}
+}
+
+// Per wave: the home atoms of the cell being built (64 at a time), their
+// list lengths, and the candidate segments (one per distinct neighbour
+// cell: first new index and prefix of lengths).
+struct sepb_build_lds {
+	double hx[64], hy[64], hz[64];
+	int hcnt[64];
+	int segjs[27], segpre[28];
+};
+
+__global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
+{
+	__shared__ sepb_build_lds Lall[SEPB_BLOCK / 64];
+	const int lane = threadIdx.x & 63;
+	sepb_build_lds &L = Lall[threadIdx.x >> 6];
+	const unsigned long long below = (1ull << lane) - 1ull;
+	const long nw = ((long)gridDim.x * blockDim.x) >> 6;
+	for (long gc = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gc < b.ncell_tot;
+	     gc += nw) {
+		const int s = b.cell_sys[gc];
+		if (s < 0)
+			continue;
+		sepb_sys *y = b.sys + s;
+		if (!y->neighb_flag)
+			continue;
+		const int c = (int)(gc - y->cell_off);
+		const int nx = y->ncell[0], ny = y->ncell[1], nz = y->ncell[2];
+		const int cxi = c % nx, cyi = (c / nx) % ny, czi = c / (nx * ny);
+		const long off = y->off;
+		const int n = y->n, npad = y->npad, sub = y->sub, cap = b.cap;
+		const int lg = __ffs(sub) - 1;
+		const int *st = b.cell_start + y->cell_off;
+		// the new order, written by k_bsort
+		const sepk_pos *pn = b.st[1 - y->cur].pos[0] + off;
+		const double L0 = y->L[0], L1 = y->L[1], L2 = y->L[2];
+		const double rl2 = y->rlist2;
+		const int h0 = st[c], m = st[c + 1] - h0;
+		int *lst = b.nbr + y->list_off;
+		const long lstride = (long)npad * sub;
+		// the distinct neighbour cells, in a fixed order
+		int ox[3], oy[3], oz[3];
+		const int nox = sepb_offsets(nx, ox), noy = sepb_offsets(ny, oy),
+			  noz = sepb_offsets(nz, oz);
+		const int nseg = nox * noy * noz;
+		int len = 0;
+		if (lane < nseg) {
+			const int dx = ox[lane % nox], dy = oy[(lane / nox) % noy],
+				  dz = oz[lane / (nox * noy)];
+			const int c2 = sepb_mod(cxi + dx, nx) +
+				       nx * (sepb_mod(cyi + dy, ny) + ny * sepb_mod(czi + dz, nz));
+			L.segjs[lane] = st[c2];
+			len = st[c2 + 1] - st[c2];
+		}
+		int incl = len;
+		for (int o = 1; o < 64; o <<= 1) {
+			const int u = __shfl_up(incl, o, 64);
+			if (lane >= o)
+				incl += u;
+		}
+		if (lane < nseg)
+			L.segpre[lane + 1] = incl;
+		if (lane == 0)
+			L.segpre[0] = 0;
+		const int T = __shfl(incl, nseg - 1, 64);
+		__builtin_amdgcn_wave_barrier();
+		for (int hb = 0; hb < m; hb += 64) {
+			const int hn = min(m - hb, 64);
+			if (lane < hn) {
+				const sepk_pos q = pn[h0 + hb + lane];
+				L.hx[lane] = q.x;
+				L.hy[lane] = q.y;
+				L.hz[lane] = q.z;
+				L.hcnt[lane] = 0;
+			}
+			__builtin_amdgcn_wave_barrier();
+			for (int q0 = 0; q0 < T; q0 += 64) {
+				const int q = q0 + lane;
+				const bool valid = q < T;
+				int j = 0;
+				double cx = 0.0, cy = 0.0, cz = 0.0;
+				if (valid) {
+					int r = 0;
+					while (r + 1 < nseg && L.segpre[r + 1] <= q)
+						r++;
+					j = L.segjs[r] + (q - L.segpre[r]);
+					const sepk_pos cq = pn[j];
+					cx = cq.x;
+					cy = cq.y;
+					cz = cq.z;
+				}
+				for (int i = 0; i < hn; i++) {
+					const int h = h0 + hb + i;
+					const double ddx = sepk_wrap(L.hx[i] - cx, L0);
+					const double ddy = sepk_wrap(L.hy[i] - cy, L1);
+					const double ddz = sepk_wrap(L.hz[i] - cz, L2);
+					const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+					const bool hit = valid && r2 < rl2 && j != h;
+					const unsigned long long bal = __ballot(hit);
+					if (bal) {
+						const int base = L.hcnt[i];
+						const int e = base + __popcll(bal & below);
+						if (hit && e < cap)
+							lst[(long)(e >> lg) * lstride + (long)h * sub + (e & (sub - 1))] = j;
+						__builtin_amdgcn_wave_barrier();
+						if (lane == 0)
+							L.hcnt[i] = base + __popcll(bal);
+						__builtin_amdgcn_wave_barrier();
+					}
+				}
+			}
+			if (lane < hn) {
+				const int h = h0 + hb + lane;
+				int cnt = L.hcnt[lane];
+				if (cnt > cap) {
+					atomicCAS(&y->error, 0, SEP_GPU_ENEIGHB);
+					cnt = cap;
+				}
+				// pad to whole batches of every lane with the phantom
+				const int qq = sub * SEPB_BATCH;
+				const int pad = min((cnt + qq - 1) / qq * qq, cap);
+				for (int e = cnt; e < pad; e++)
+					lst[(long)(e >> lg) * lstride + (long)h * sub + (e & (sub - 1))] = n;
+				b.nbr_count[off + h] = cnt;
+				atomicMax(&y->max_full, cnt);
+			}
+			__builtin_amdgcn_wave_barrier();
+		}
+		// the last cell of the system swaps the copies
+		__threadfence();
+		unsigned prev = 0;
+		if (lane == 0)
+			prev = atomicAdd(&y->arrive_build, 1u);
+		prev = __shfl(prev, 0, 64);
+		if (prev == (unsigned)(nx * ny * nz - 1) && lane == 0) {
+			__threadfence();
+			y->cur = 1 - y->cur;
+			y->p = 0;
+			y->neighb_flag = 0;
+			y->arrive_build = 0u;
+		}
+	}
+}
+
+// ---- the step ------------------------------------------------------------------
+
+// Columns of a tile's row: 0 pair energy, 1-6 pair stress (full-list
+// sums), 7 sum m vh^2, 8-13 kinetic stress, 14 sum m v^2, 15 max d^2.
+template <bool MIXED, bool INTEGRATE>
+__global__ __launch_bounds__(SEPB_BLOCK) void k_bstep(sepb_view b, sepb_lj c, const int4 *tiles,
+						      int ntiles)
+{
+	__shared__ double tot[SEPB_NV];
+	__shared__ double red[16][SEPB_NV];
+	__shared__ int last;
+	for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+		const int4 td = tiles[tile]; // system, first atom, row
+		sepb_sys *y = b.sys + td.x;
+		const long off = y->off;
+		const int n = y->n, npad = y->npad, sub = y->sub;
+		const sepb_state &S = b.st[y->cur];
+		const int p = y->p;
+		const long slots = b.slots;
+		const double L0 = y->L[0], L1 = y->L[1], L2 = y->L[2];
+		const int t = td.y + (int)threadIdx.x / sub;
+		const int l = (int)threadIdx.x % sub;
+		const bool act = t < n;
+		double fx = 0.0, fy = 0.0, fz = 0.0;
+		// pair energy and the crossing part of the pair stress
+		double pacc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+		sepk_pos pi = {0.0, 0.0, 0.0, 0.0};
+		const sepk_pos *pos = S.pos[p] + off;
+		if (act) {
+			const double iL0 = y->iL[0], iL1 = y->iL[1], iL2 = y->iL[2];
+			pi = pos[t];
+			const int q = sub * SEPB_BATCH;
+			const int nbat = (b.nbr_count[off + t] + q - 1) / q;
+			const long stride = (long)npad * sub;
+			const int *lp = b.nbr + y->list_off + (long)t * sub + l;
+			for (int bb = 0; bb < nbat; bb++, lp += SEPB_BATCH * stride) {
+				int j[SEPB_BATCH];
+#pragma unroll
+				for (int u = 0; u < SEPB_BATCH; u++)
+					j[u] = lp[u * stride];
+				double px[SEPB_BATCH], py[SEPB_BATCH], pz[SEPB_BATCH];
+#pragma unroll
+				for (int u = 0; u < SEPB_BATCH; u++) {
+					const sepk_pos &r = pos[j[u]];
+					px[u] = r.x;
+					py[u] = r.y;
+					pz[u] = r.z;
+				}
+#pragma unroll
+				for (int u = 0; u < SEPB_BATCH; u++) {
+					// minimum image d - L k, k = rint(d / L): the
+					// single-image rule for every listed pair, whose
+					// separation stays below 1.5 L
+					const double kx = rint((pi.x - px[u]) * iL0);
+					const double ky = rint((pi.y - py[u]) * iL1);
+					const double kz = rint((pi.z - pz[u]) * iL2);
+					const double wx = fma(-L0, kx, pi.x - px[u]);
+					const double wy = fma(-L1, ky, pi.y - py[u]);
+					const double wz = fma(-L2, kz, pi.z - pz[u]);
+					const double r2 = wx * wx + wy * wy + wz * wz;
+					if (!(r2 < c.cf2))
+						continue;
+					double ft, e;
+					if (MIXED) {
+						const float q1 = __builtin_amdgcn_rcpf((float)r2);
+						const float q3 = q1 * q1 * q1;
+						ft = (double)(fmaf(c.Af, q3, c.nBf) * q3 * q1);
+						e = (double)fmaf(fmaf(c.Cf, q3, c.nDf), q3, c.ushiftf);
+					} else {
+						const double q1 = 1.0 / r2;
+						const double q3 = q1 * q1 * q1;
+						ft = fma(c.A, q3, c.nB) * q3 * q1;
+						e = fma(fma(c.C, q3, c.nD), q3, c.ushift);
+					}
+					fx = fma(ft, wx, fx);
+					fy = fma(ft, wy, fy);
+					fz = fma(ft, wz, fz);
+					pacc[0] += e;
+					if (kx != 0.0 || ky != 0.0 || kz != 0.0) {
+						// the crossing term of the stress: -f (x) L k
+						const double sx = kx * L0, sy = ky * L1, sz = kz * L2;
+						const double gx = -ft * wx, gy = -ft * wy, gz = -ft * wz;
+						pacc[1] = fma(gx, sx, pacc[1]);
+						pacc[2] = fma(gx, sy, pacc[2]);
+						pacc[3] = fma(gx, sz, pacc[3]);
+						pacc[4] = fma(gy, sy, pacc[4]);
+						pacc[5] = fma(gy, sz, pacc[5]);
+						pacc[6] = fma(gz, sz, pacc[6]);
+					}
+				}
+			}
+		}
+		double acc[SEPB_NV];
+#pragma unroll
+		for (int k = 0; k < 7; k++)
+			acc[k] = pacc[k];
+#pragma unroll
+		for (int k = 7; k < SEPB_NV; k++)
+			acc[k] = 0.0;
+		for (int o = 1; o < sub; o <<= 1) {
+			fx += __shfl_xor(fx, o, SEPK_WAVE);
+			fy += __shfl_xor(fy, o, SEPK_WAVE);
+			fz += __shfl_xor(fz, o, SEPK_WAVE);
+		}
+		double al = 0.0;
+		if (INTEGRATE)
+			al = sepk_nh_alpha(y->alpha, y->sumv2, (double)n, y->Td, y->dt_tau2);
+		if (act && l == 0) {
+			// stress: 2 F_i (x) x_i per atom (the full-list identity)
+			const double gx = 2.0 * fx, gy = 2.0 * fy, gz = 2.0 * fz;
+			acc[1] = fma(gx, pi.x, acc[1]);
+			acc[2] = fma(gx, pi.y, acc[2]);
+			acc[3] = fma(gx, pi.z, acc[3]);
+			acc[4] = fma(gy, pi.y, acc[4]);
+			acc[5] = fma(gy, pi.z, acc[5]);
+			acc[6] = fma(gz, pi.z, acc[6]);
+			const long g = off + t;
+			if (!INTEGRATE) {
+				S.f[g] = fx;
+				S.f[slots + g] = fy;
+				S.f[2 * slots + g] = fz;
+			} else {
+				// sep_nosehoover then sep_leapfrog
+				// (source/sepintgr.c:149-168, :42-88)
+				const double m = S.m[g];
+				const double Lk[3] = {L0, L1, L2};
+				const double F[3] = {fx, fy, fz};
+				double x[3] = {pi.x, pi.y, pi.z};
+				double vh[3];
+				double d2 = 0.0;
+				for (int k = 0; k < 3; k++) {
+					const long i = k * slots + g;
+					double v = S.v[i];
+					const double f = F[k] - al * m * v;
+					const double a = f / m;
+					v += a * y->dt;
+					x[k] += v * y->dt;
+					vh[k] = v - 0.5 * a * y->dt;
+					acc[7] += vh[k] * vh[k] * m;
+					acc[14] += v * v * m;
+					int cn = S.cn[i], cr = S.cr[i];
+					if (x[k] > Lk[k]) {
+						x[k] -= Lk[k];
+						cn++;
+						cr++;
+					} else if (x[k] < 0.0) {
+						x[k] += Lk[k];
+						cn--;
+						cr--;
+					}
+					const double dr = (x[k] + cn * Lk[k]) - S.xn[i];
+					d2 += dr * dr;
+					S.f[i] = f;
+					S.a[i] = a;
+					S.v[i] = v;
+					S.cn[i] = cn;
+					S.cr[i] = cr;
+				}
+				sepk_pos np = {x[0], x[1], x[2], pi.q};
+				S.pos[1 - p][g] = np;
+				acc[8] = vh[0] * vh[0] * m;
+				acc[9] = vh[0] * vh[1] * m;
+				acc[10] = vh[0] * vh[2] * m;
+				acc[11] = vh[1] * vh[1] * m;
+				acc[12] = vh[1] * vh[2] * m;
+				acc[13] = vh[2] * vh[2] * m;
+				acc[15] = d2;
+			}
+		}
+		sepk_block_partial<SEPB_NV>(acc, 1u << 15, b.partial, td.z, b.prow);
+		// the last tile of the system to finish combines its rows
+		__threadfence();
+		if (threadIdx.x == 0)
+			last = atomicAdd(&y->arrive_step, 1u) == (unsigned)(y->ntile - 1);
+		__syncthreads();
+		if (last) {
+			__threadfence();
+			{
+				// 16 threads per column over strided rows, then the 16
+				// part sums in order (a fixed order: deterministic)
+				const int col = threadIdx.x & 15, part = threadIdx.x >> 4;
+				const double *r = b.partial + (size_t)col * b.prow + y->row0;
+				double sacc = 0.0;
+				for (int k = part; k < y->ntile; k += 16)
+					sacc = col == 15 ? fmax(sacc, r[k]) : sacc + r[k];
+				red[part][col] = sacc;
+			}
+			__syncthreads();
+			if (threadIdx.x < SEPB_NV) {
+				const int col = threadIdx.x;
+				double sacc = red[0][col];
+				for (int k = 1; k < 16; k++)
+					sacc = col == 15 ? fmax(sacc, red[k][col]) : sacc + red[k][col];
+				tot[col] = sacc;
+			}
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				double *ret = y->ret;
+				for (int k = 0; k < SEPDEV_RET_WORDS; k++)
+					ret[k] = 0.0;
+				// each pair was seen from both ends
+				ret[SEPDEV_RET_EPOT] = 0.5 * tot[0];
+				sepk_add_sym9(ret + SEPDEV_RET_POTP, tot + 1, 0.5);
+				if (INTEGRATE) {
+					ret[SEPDEV_RET_EKIN] = 0.5 * tot[7];
+					sepk_add_sym9(ret + SEPDEV_RET_KINP, tot + 8, 1.0);
+					y->alpha = al;
+					y->sumv2 = tot[14];
+					y->max_dist2 = tot[15];
+					if (sqrt(tot[15]) > 0.5 * y->skin) {
+						y->neighb_flag = 1u;
+						y->nupdate++;
+					}
+					y->tnow += y->dt;
+					y->p = 1 - p;
+				}
+				y->arrive_step = 0u;
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// ---- host side ---------------------------------------------------------------------
+
+template <typename T>
+static int balloc(T **p, size_t count)
+{
+	*p = nullptr;
+	if (count == 0)
+		count = 1;
+	if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess)
+		return sepk_fail(SEP_GPU_EDEVICE, "batch: hipMalloc of %zu bytes failed",
+				 count * sizeof(T));
+	return 0;
+}
+
+static void free_state(sepb_state &s)
+{
+	for (int k = 0; k < 2; k++)
+		(void)hipFree(s.pos[k]);
+	(void)hipFree(s.v);
+	(void)hipFree(s.f);
+	(void)hipFree(s.a);
+	(void)hipFree(s.xn);
+	(void)hipFree(s.cn);
+	(void)hipFree(s.cr);
+	(void)hipFree(s.m);
+	(void)hipFree(s.orig);
+}
+
+static int alloc_state(sepb_state &s, long slots)
+{
+	memset(&s, 0, sizeof s);
+	int rc = 0;
+	rc |= balloc(&s.pos[0], slots);
+	rc |= balloc(&s.pos[1], slots);
+	rc |= balloc(&s.v, 3 * slots);
+	rc |= balloc(&s.f, 3 * slots);
+	rc |= balloc(&s.a, 3 * slots);
+	rc |= balloc(&s.xn, 3 * slots);
+	rc |= balloc(&s.cn, 3 * slots);
+	rc |= balloc(&s.cr, 3 * slots);
+	rc |= balloc(&s.m, slots);
+	rc |= balloc(&s.orig, slots);
+	return rc;
+}
+
+extern "C" void sepdev_batch_destroy(sepdev_batch *b)
+{
+	if (!b)
+		return;
+	if (b->st)
+		(void)hipStreamSynchronize(b->st);
+	if (b->exec)
+		(void)hipGraphExecDestroy(b->exec);
+	if (b->graph)
+		(void)hipGraphDestroy(b->graph);
+	free_state(b->S[0]);
+	free_state(b->S[1]);
+	(void)hipFree(b->dS);
+	(void)hipFree(b->dsys);
+	(void)hipFree(b->chunk_sys);
+	(void)hipFree(b->cellid);
+	(void)hipFree(b->cslot);
+	(void)hipFree(b->otmp);
+	(void)hipFree(b->cell_count);
+	(void)hipFree(b->cell_start);
+	(void)hipFree(b->cell_sys);
+	(void)hipFree(b->nbr);
+	(void)hipFree(b->nbr_count);
+	(void)hipFree(b->partial);
+	(void)hipFree(b->tiles);
+	(void)hipFree(b->sub_tiles);
+	if (b->st)
+		(void)hipStreamDestroy(b->st);
+	free(b->hsys);
+	free(b);
+}
+
+static sepb_view view_of(const sepdev_batch *b)
+{
+	sepb_view v;
+	memset(&v, 0, sizeof v);
+	v.sys = b->dsys;
+	v.st = b->dS;
+	v.slots = b->slots;
+	v.nsys = b->nsys;
+	v.chunk_sys = b->chunk_sys;
+	v.cellid = b->cellid;
+	v.cslot = b->cslot;
+	v.otmp = b->otmp;
+	v.cell_count = b->cell_count;
+	v.cell_start = b->cell_start;
+	v.cell_sys = b->cell_sys;
+	v.ncell_tot = b->ncell_tot;
+	v.nbr = b->nbr;
+	v.nbr_count = b->nbr_count;
+	v.cap = b->cap;
+	v.partial = b->partial;
+	v.prow = b->prow;
+	return v;
+}
+
+// Tiles of every system at its lanes per atom; rows numbered in table order.
+static int make_tiles(sepdev_batch *b)
+{
+	long nt = 0;
+	for (int s = 0; s < b->nsys; s++) {
+		const int apt = SEPB_BLOCK / b->hsys[s].sub;
+		nt += (b->hsys[s].n + apt - 1) / apt;
+	}
+	if (nt > b->prow)
+		return sepk_fail(SEP_GPU_EARG, "batch: %ld tiles exceed %ld rows", nt, b->prow);
+	int4 *h = (int4 *)malloc(nt * sizeof(int4));
+	long k = 0;
+	for (int s = 0; s < b->nsys; s++) {
+		sepb_sys &y = b->hsys[s];
+		const int apt = SEPB_BLOCK / y.sub;
+		y.row0 = (int)k;
+		y.ntile = (y.n + apt - 1) / apt;
+		for (int i = 0; i < y.ntile; i++, k++)
+			h[k] = make_int4(s, i * apt, (int)k, 0);
+	}
+	b->ntiles = (int)nt;
+	hipError_t e = hipMemcpy(b->tiles, h, nt * sizeof(int4), hipMemcpyHostToDevice);
+	free(h);
+	if (e != hipSuccess)
+		return sepk_fail(SEP_GPU_EDEVICE, "batch: tile upload: %s", hipGetErrorString(e));
+	return 0;
+}
+
+static int get_sys(sepdev_batch *b)
+{
+	SEPK_CHECK(hipStreamSynchronize(b->st));
+	SEPK_CHECK(hipMemcpy(b->hsys, b->dsys, b->nsys * sizeof(sepb_sys), hipMemcpyDeviceToHost));
+	return 0;
+}
+
+static int put_sys(sepdev_batch *b)
+{
+	SEPK_CHECK(hipStreamSynchronize(b->st));
+	SEPK_CHECK(hipMemcpy(b->dsys, b->hsys, b->nsys * sizeof(sepb_sys), hipMemcpyHostToDevice));
+	return 0;
+}
+
+static void drop_graph(sepdev_batch *b)
+{
+	if (b->exec)
+		(void)hipGraphExecDestroy(b->exec);
+	if (b->graph)
+		(void)hipGraphDestroy(b->graph);
+	b->exec = nullptr;
+	b->graph = nullptr;
+}
+
+extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *in, double cf,
+					     int cap, int *err)
+{
+	*err = 0;
+	if (nsys <= 0 || !in) {
+		*err = sepk_fail(SEP_GPU_EARG, "batch: %d systems", nsys);
+		return nullptr;
+	}
+	if (cap <= 0)
+		cap = 256;
+	cap = (cap + 63) / 64 * 64;
+	sepdev_batch *b = (sepdev_batch *)calloc(1, sizeof(sepdev_batch));
+	b->nsys = nsys;
+	b->cap = cap;
+	b->hsys = (sepb_sys *)calloc(nsys, sizeof(sepb_sys));
+	long slots = 0, cells = 0;
+	for (int s = 0; s < nsys; s++) {
+		const sepdev_batch_sys &a = in[s];
+		sepb_sys &y = b->hsys[s];
+		if (a.n <= 0 || a.n > (1L << 26)) {
+			*err = sepk_fail(SEP_GPU_EARG, "batch: system %d has %ld atoms", s, a.n);
+			sepdev_batch_destroy(b);
+			return nullptr;
+		}
+		const double rl = sqrt(a.rlist2);
+		y.off = slots;
+		y.n = (int)a.n;
+		y.npad = (int)((a.n + 1 + 63) / 64 * 64);
+		y.sub = SEPDEV_LANES;
+		for (int k = 0; k < 3; k++) {
+			// single image: the list cutoff stays below half the box
+			if (!(0.5 * a.L[k] >= rl) || !(cf * cf <= a.rlist2)) {
+				*err = sepk_fail(SEP_GPU_EARG,
+						 "batch: system %d: box %g too small for list cutoff %g",
+						 s, a.L[k], rl);
+				sepdev_batch_destroy(b);
+				return nullptr;
+			}
+			int nc = (int)(a.L[k] / rl);
+			if (nc < 1)
+				nc = 1;
+			y.ncell[k] = nc;
+			y.lcell[k] = a.L[k] / nc;
+			y.L[k] = a.L[k];
+			y.iL[k] = 1.0 / a.L[k];
+		}
+		y.cell_off = (int)cells;
+		cells += (long)y.ncell[0] * y.ncell[1] * y.ncell[2] + 1;
+		y.rlist2 = a.rlist2;
+		y.dt = a.dt;
+		y.Td = a.Td;
+		y.dt_tau2 = a.dt / (a.tau * a.tau);
+		y.skin = a.skin;
+		y.alpha = a.alpha;
+		y.tnow = a.tnow;
+		y.neighb_flag = 1u; // the first step builds
+		slots += y.npad;
+	}
+	b->slots = slots;
+	b->ncell_tot = cells;
+	b->prow = slots / 16 + nsys + 64; // tiles hold at least 16 atoms
+	int rc = 0;
+	if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess)
+		rc = sepk_fail(SEP_GPU_EDEVICE, "batch: hipStreamCreate failed");
+	rc |= alloc_state(b->S[0], slots);
+	rc |= alloc_state(b->S[1], slots);
+	rc |= balloc(&b->dS, 2);
+	rc |= balloc(&b->dsys, nsys);
+	rc |= balloc(&b->chunk_sys, slots / 64);
+	rc |= balloc(&b->cellid, slots);
+	rc |= balloc(&b->cslot, slots);
+	rc |= balloc(&b->otmp, slots);
+	rc |= balloc(&b->cell_count, cells);
+	rc |= balloc(&b->cell_start, cells);
+	rc |= balloc(&b->cell_sys, cells);
+	rc |= balloc(&b->nbr, (size_t)slots * cap);
+	rc |= balloc(&b->nbr_count, slots);
+	rc |= balloc(&b->partial, (size_t)b->prow * SEPB_NV);
+	rc |= balloc(&b->tiles, b->prow);
+	rc |= balloc(&b->sub_tiles, b->prow);
+	if (rc) {
+		*err = rc;
+		sepdev_batch_destroy(b);
+		return nullptr;
+	}
+	for (int s = 0; s < nsys; s++)
+		b->hsys[s].list_off = b->hsys[s].off * (long)cap;
+	// LJ constants of sep_lj_shift (source/sepmisc.c:132-147)
+	b->lj.cf2 = cf * cf;
+	b->lj.A = 48.0;
+	b->lj.nB = -24.0;
+	b->lj.C = 4.0;
+	b->lj.nD = -4.0;
+	b->lj.ushift = 0.016316891136;
+	b->lj.Af = 48.0f;
+	b->lj.nBf = -24.0f;
+	b->lj.Cf = 4.0f;
+	b->lj.nDf = -4.0f;
+	b->lj.ushiftf = (float)0.016316891136;
+	b->grid = 1024;
+
+	// host images of the per-slot and per-cell tables, then the state
+	int *chunk = (int *)malloc((slots / 64) * sizeof(int));
+	int *csys = (int *)malloc(cells * sizeof(int));
+	for (int s = 0; s < nsys; s++) {
+		const sepb_sys &y = b->hsys[s];
+		for (long k = y.off / 64; k < (y.off + y.npad) / 64; k++)
+			chunk[k] = s;
+		const int nc = y.ncell[0] * y.ncell[1] * y.ncell[2];
+		for (int k = 0; k < nc; k++)
+			csys[y.cell_off + k] = s;
+		csys[y.cell_off + nc] = -1;
+	}
+	const size_t dbytes = (size_t)slots * sizeof(double);
+	sepk_pos *hp = (sepk_pos *)malloc(slots * sizeof(sepk_pos));
+	double *hv = (double *)calloc(3 * slots, sizeof(double));
+	double *hm = (double *)calloc(slots, sizeof(double));
+	int *ho = (int *)calloc(slots, sizeof(int));
+	int *hz = (int *)calloc(3 * slots, sizeof(int));
+	const double nan = __builtin_nan("");
+	for (long k = 0; k < slots; k++)
+		hp[k] = sepk_pos{nan, nan, nan, 0.0};
+	for (int s = 0; s < nsys; s++) {
+		const sepdev_batch_sys &a = in[s];
+		sepb_sys &y = b->hsys[s];
+		double sv2 = 0.0;
+		for (long t = 0; t < a.n; t++) {
+			const long g = y.off + t;
+			hp[g] = sepk_pos{a.x[3 * t], a.x[3 * t + 1], a.x[3 * t + 2], 0.0};
+			for (int k = 0; k < 3; k++) {
+				hv[k * slots + g] = a.v[3 * t + k];
+				sv2 += a.v[3 * t + k] * a.v[3 * t + k] * a.m[t];
+			}
+			hm[g] = a.m[t];
+			ho[g] = (int)t;
+			if (a.cr)
+				for (int k = 0; k < 3; k++)
+					hz[k * slots + g] = a.cr[3 * t + k];
+		}
+		// sum m v^2 of the starting velocities, for the first thermostat
+		// update
+		y.sumv2 = sv2;
+	}
+	hipError_t e = hipSuccess;
+	for (int c = 0; c < 2 && e == hipSuccess; c++) {
+		for (int k = 0; k < 2 && e == hipSuccess; k++)
+			e = hipMemcpy(b->S[c].pos[k], hp, slots * sizeof(sepk_pos), hipMemcpyHostToDevice);
+		if (e == hipSuccess)
+			e = hipMemcpy(b->S[c].v, hv, 3 * dbytes, hipMemcpyHostToDevice);
+		if (e == hipSuccess)
+			e = hipMemcpy(b->S[c].m, hm, dbytes, hipMemcpyHostToDevice);
+		if (e == hipSuccess)
+			e = hipMemcpy(b->S[c].orig, ho, slots * sizeof(int), hipMemcpyHostToDevice);
+		if (e == hipSuccess)
+			e = hipMemcpy(b->S[c].cr, hz, 3 * slots * sizeof(int), hipMemcpyHostToDevice);
+		if (e == hipSuccess)
+			e = hipMemset(b->S[c].cn, 0, 3 * slots * sizeof(int));
+		if (e == hipSuccess)
+			e = hipMemset(b->S[c].f, 0, 3 * dbytes);
+		if (e == hipSuccess)
+			e = hipMemset(b->S[c].a, 0, 3 * dbytes);
+		if (e == hipSuccess)
+			e = hipMemset(b->S[c].xn, 0, 3 * dbytes);
+	}
+	if (e == hipSuccess)
+		e = hipMemcpy(b->dS, b->S, 2 * sizeof(sepb_state), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = hipMemcpy(b->chunk_sys, chunk, (slots / 64) * sizeof(int), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = hipMemcpy(b->cell_sys, csys, cells * sizeof(int), hipMemcpyHostToDevice);
+	if (e == hipSuccess)
+		e = hipMemset(b->cell_count, 0, cells * sizeof(int));
+	if (e == hipSuccess)
+		e = hipMemset(b->nbr_count, 0, slots * sizeof(int));
+	free(chunk);
+	free(csys);
+	free(hp);
+	free(hv);
+	free(hm);
+	free(ho);
+	free(hz);
+	if (e != hipSuccess) {
+		*err = sepk_fail(SEP_GPU_EDEVICE, "batch: upload: %s", hipGetErrorString(e));
+		sepdev_batch_destroy(b);
+		return nullptr;
+	}
+	if ((rc = make_tiles(b)) || (rc = put_sys(b))) {
+		*err = rc;
+		sepdev_batch_destroy(b);
+		return nullptr;
+	}
+	return b;
+}
+
+static int launch_rebuild(sepdev_batch *b)
+{
+	const sepb_view v = view_of(b);
+	k_bbin<<<512, SEPB_BLOCK, 0, b->st>>>(v);
+	SEPK_LAUNCH_CHECK();
+	k_bscatter<<<512, SEPB_BLOCK, 0, b->st>>>(v);
+	SEPK_LAUNCH_CHECK();
+	k_bsort<<<512, SEPB_BLOCK, 0, b->st>>>(v);
+	SEPK_LAUNCH_CHECK();
+	k_bbuild<<<1024, SEPB_BLOCK, 0, b->st>>>(v);
+	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+static int launch_step(sepdev_batch *b, int mixed, int integrate, const int4 *tiles, int ntiles)
+{
+	const sepb_view v = view_of(b);
+	const int grid = ntiles < b->grid ? ntiles : b->grid;
+	if (grid <= 0)
+		return 0;
+	if (mixed) {
+		if (integrate)
+			k_bstep<true, true><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
+		else
+			k_bstep<true, false><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
+	} else {
+		if (integrate)
+			k_bstep<false, true><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
+		else
+			k_bstep<false, false><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
+	}
+	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+extern "C" int sepdev_batch_run(sepdev_batch *b, long nsteps, int mixed, int graph)
+{
+	if (nsteps <= 0)
+		return 0;
+	int rc;
+	if (!graph) {
+		for (long i = 0; i < nsteps; i++)
+			if ((rc = launch_rebuild(b)) || (rc = launch_step(b, mixed, 1, b->tiles, b->ntiles)))
+				return rc;
+		return 0;
+	}
+	if (!b->exec || b->graph_mixed != mixed) {
+		drop_graph(b);
+		SEPK_CHECK(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
+		rc = launch_rebuild(b);
+		if (!rc)
+			rc = launch_step(b, mixed, 1, b->tiles, b->ntiles);
+		hipGraph_t g = nullptr;
+		const hipError_t e = hipStreamEndCapture(b->st, &g);
+		if (rc) {
+			if (g)
+				(void)hipGraphDestroy(g);
+			return rc;
+		}
+		if (e != hipSuccess)
+			return sepk_fail(SEP_GPU_EDEVICE, "batch: capture: %s", hipGetErrorString(e));
+		b->graph = g;
+		SEPK_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+		b->graph_mixed = mixed;
+	}
+	for (long i = 0; i < nsteps; i++)
+		SEPK_CHECK(hipGraphLaunch(b->exec, b->st));
+	return 0;
+}
+
+// Forces, energy and stress at the current positions (rebuilding flagged
+// lists first); nothing moves.
+extern "C" int sepdev_batch_eval(sepdev_batch *b, int mixed)
+{
+	int rc = launch_rebuild(b);
+	if (!rc)
+		rc = launch_step(b, mixed, 0, b->tiles, b->ntiles);
+	return rc;
+}
+
+extern "C" int sepdev_batch_synchronize(sepdev_batch *b)
+{
+	SEPK_CHECK(hipStreamSynchronize(b->st));
+	return 0;
+}
+
+// State of system s in its original atom order (3n interleaved vectors;
+// any pointer may be null), its return slot (SEPDEV_RET_WORDS) and scalars
+// {alpha, max_dist2, tnow, nupdate, neighb_flag, max_full, error, sumv2}.
+extern "C" int sepdev_batch_get(sepdev_batch *b, int s, double *x, double *v, double *f,
+				double *a, int *cr, double *ret, double *scal)
+{
+	if (s < 0 || s >= b->nsys)
+		return sepk_fail(SEP_GPU_EARG, "batch: system %d", s);
+	int rc = get_sys(b);
+	if (rc)
+		return rc;
+	const sepb_sys &y = b->hsys[s];
+	const sepb_state &S = b->S[y.cur];
+	const long n = y.n, slots = b->slots;
+	sepk_pos *hp = (sepk_pos *)malloc(n * sizeof(sepk_pos));
+	double *hv = (double *)malloc(3 * n * sizeof(double));
+	int *ho = (int *)malloc(n * sizeof(int));
+	int *hi = (int *)malloc(3 * n * sizeof(int));
+	hipError_t e = hipMemcpy(ho, S.orig + y.off, n * sizeof(int), hipMemcpyDeviceToHost);
+	if (e == hipSuccess && x) {
+		e = hipMemcpy(hp, S.pos[y.p] + y.off, n * sizeof(sepk_pos), hipMemcpyDeviceToHost);
+		for (long t = 0; t < n && e == hipSuccess; t++) {
+			x[3 * ho[t]] = hp[t].x;
+			x[3 * ho[t] + 1] = hp[t].y;
+			x[3 * ho[t] + 2] = hp[t].z;
+		}
+	}
+	double *fields[3] = {v, f, a};
+	double *src[3] = {S.v, S.f, S.a};
+	for (int q = 0; q < 3 && e == hipSuccess; q++) {
+		if (!fields[q])
+			continue;
+		for (int k = 0; k < 3 && e == hipSuccess; k++)
+			e = hipMemcpy(hv + k * n, src[q] + k * slots + y.off, n * sizeof(double),
+				      hipMemcpyDeviceToHost);
+		for (long t = 0; t < n && e == hipSuccess; t++)
+			for (int k = 0; k < 3; k++)
+				fields[q][3 * ho[t] + k] = hv[k * n + t];
+	}
+	if (cr && e == hipSuccess) {
+		for (int k = 0; k < 3 && e == hipSuccess; k++)
+			e = hipMemcpy(hi + k * n, S.cr + k * slots + y.off, n * sizeof(int),
+				      hipMemcpyDeviceToHost);
+		for (long t = 0; t < n && e == hipSuccess; t++)
+			for (int k = 0; k < 3; k++)
+				cr[3 * ho[t] + k] = hi[k * n + t];
+	}
+	free(hp);
+	free(hv);
+	free(ho);
+	free(hi);
+	if (e != hipSuccess)
+		return sepk_fail(SEP_GPU_EDEVICE, "batch: download: %s", hipGetErrorString(e));
+	if (ret)
+		memcpy(ret, y.ret, SEPDEV_RET_WORDS * sizeof(double));
+	if (scal) {
+		scal[0] = y.alpha;
+		scal[1] = y.max_dist2;
+		scal[2] = y.tnow;
+		scal[3] = y.nupdate;
+		scal[4] = y.neighb_flag;
+		scal[5] = y.max_full;
+		scal[6] = y.error;
+		scal[7] = y.sumv2;
+	}
+	if (y.error)
+		return sepk_fail(y.error, "batch: system %d: device error %d%s", s, y.error,
+				 y.error == SEP_GPU_ENEIGHB ? " (neighbour list overflow)" : "");
+	return 0;
+}
+
+// Lanes per atom of each system (1, 2, 4, 8 or 16) and the persistent
+// grid; systems whose lanes change are rebuilt before the next step.
+extern "C" int sepdev_batch_set_shape(sepdev_batch *b, const int *sub, int grid)
+{
+	int rc = get_sys(b);
+	if (rc)
+		return rc;
+	if (sub)
+		for (int s = 0; s < b->nsys; s++) {
+			const int u = sub[s];
+			if (u != 1 && u != 2 && u != 4 && u != 8 && u != 16)
+				return sepk_fail(SEP_GPU_EARG, "batch: %d lanes per atom", u);
+			if (u * SEPB_BATCH > b->cap)
+				return sepk_fail(SEP_GPU_EARG, "batch: %d lanes exceed the list capacity", u);
+			if (u != b->hsys[s].sub) {
+				b->hsys[s].sub = u;
+				b->hsys[s].neighb_flag = 1u;
+			}
+		}
+	if (grid > 0)
+		b->grid = grid;
+	if ((rc = make_tiles(b)) || (rc = put_sys(b)))
+		return rc;
+	drop_graph(b);
+	return 0;
+}
+
+extern "C" int sepdev_batch_get_shape(sepdev_batch *b, int *sub, int *grid)
+{
+	for (int s = 0; s < b->nsys; s++)
+		sub[s] = b->hsys[s].sub;
+	*grid = b->grid;
+	return 0;
+}
+
+// Mean duration of one force evaluation (no integration) over the tiles of
+// the systems with mask[s] != 0 (null: all), reps launches back to back.
+extern "C" int sepdev_batch_time_eval(sepdev_batch *b, const int *mask, int mixed, int reps,
+				      double *ms)
+{
+	int rc = launch_rebuild(b);
+	if (rc || (rc = get_sys(b)))
+		return rc;
+	int4 *h = (int4 *)malloc(b->prow * sizeof(int4));
+	int nt = 0;
+	for (int s = 0; s < b->nsys; s++) {
+		if (mask && !mask[s])
+			continue;
+		const sepb_sys &y = b->hsys[s];
+		const int apt = SEPB_BLOCK / y.sub;
+		for (int i = 0; i < y.ntile; i++)
+			h[nt++] = make_int4(s, i * apt, y.row0 + i, 0);
+	}
+	hipError_t e = hipMemcpy(b->sub_tiles, h, nt * sizeof(int4), hipMemcpyHostToDevice);
+	free(h);
+	if (e != hipSuccess)
+		return sepk_fail(SEP_GPU_EDEVICE, "batch: tiles: %s", hipGetErrorString(e));
+	hipEvent_t e0, e1;
+	SEPK_CHECK(hipEventCreate(&e0));
+	SEPK_CHECK(hipEventCreate(&e1));
+	rc = launch_step(b, mixed, 0, b->sub_tiles, nt);
+	if (!rc)
+		(void)hipEventRecord(e0, b->st);
+	for (int r = 0; r < reps && !rc; r++)
+		rc = launch_step(b, mixed, 0, b->sub_tiles, nt);
+	if (!rc) {
+		(void)hipEventRecord(e1, b->st);
+		e = hipEventSynchronize(e1);
+		float f = 0.0f;
+		if (e == hipSuccess)
+			e = hipEventElapsedTime(&f, e0, e1);
+		if (e != hipSuccess)
+			rc = sepk_fail(SEP_GPU_EDEVICE, "batch: timing: %s", hipGetErrorString(e));
+		*ms = reps > 0 ? f / reps : 0.0;
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	return rc;
+}

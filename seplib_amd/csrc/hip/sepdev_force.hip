@@ -60,10 +60,19 @@ struct sepk_lj {
 // atom (fixed xor order), write or add its force from lane 0, and leave the
 // block's energy + stress row in the pair region.  acc = {e, wxx, wxy, wxz,
 // wyy, wyz, wzz}, each summed over the lane's share of the full list.
+// The stress of a pair term, sum over listed pairs of f (x) r, is not
+// accumulated pair by pair.  With r_ij = x_i - x_j - s_ij (s_ij = L k_ij,
+// the periodic shift taken off) and f_ji = -f_ij, the full-list sum is
+//     sum_{i,j} f_ij (x) r_ij = 2 sum_i F_i (x) x_i - sum_{i,j} f_ij (x) s_ij,
+// so a pair adds to the stress only when it crosses the box (k != 0), and
+// each atom adds 2 F_i (x) x_i once, here, from its summed force.  The six
+// components are {xx, xy, xz, yy, yz, zz} of f (x) r; the finisher halves
+// the full-list sums.
 template <bool OVERWRITE>
 __device__ __forceinline__ void sepk_pair_epilogue(
 	long t, int sub, long n, long npad, bool act, double fx, double fy, double fz,
-	double (&acc)[7], double *f, double *partial, long row, long rows)
+	double px, double py, double pz, double (&acc)[7], double *f, double *partial,
+	long row, long rows)
 {
 #pragma unroll
 	for (int o = 1; o < SEPK_SUB; o <<= 1) {
@@ -82,32 +91,58 @@ __device__ __forceinline__ void sepk_pair_epilogue(
 			f[2 * npad + t] += fz;
 		}
 	}
+	if (act && sub == 0) {
+		const double gx = 2.0 * fx, gy = 2.0 * fy, gz = 2.0 * fz;
+		acc[1] = fma(gx, px, acc[1]);
+		acc[2] = fma(gx, py, acc[2]);
+		acc[3] = fma(gx, pz, acc[3]);
+		acc[4] = fma(gy, py, acc[4]);
+		acc[5] = fma(gy, pz, acc[5]);
+		acc[6] = fma(gz, pz, acc[6]);
+	}
 	sepk_block_partial<7>(acc, 0u, partial, row, rows);
 }
 
+// Minimum image of d, returning d - L k with the image count k in
+// {-1, 0, 1} (the reference's single-image rule, or its FAST form).
 template <bool FAST>
-__device__ __forceinline__ double sepk_image(double d, double L, double iL)
+__device__ __forceinline__ double sepk_image(double d, double L, double iL, double &k)
 {
-	return FAST ? sepk_wrap_fast(d, L, iL) : sepk_wrap(d, L);
+	if (FAST) {
+		k = rint(d * iL);
+		return fma(-L, k, d);
+	}
+	k = 0.0;
+	if (d > 0.5 * L) {
+		k = 1.0;
+		d -= L;
+	} else if (d < -0.5 * L) {
+		k = -1.0;
+		d += L;
+	}
+	return d;
 }
 
-// Adds one pair's force ft r to the atom, its energy, and ft r (x) r to the
-// six stress components {xx, xy, xz, yy, yz, zz}, with fused multiply-adds
-// (the diagonal reuses the squares of r^2).
+// Adds one pair's force ft r to the atom and its energy; the stress gets
+// only the crossing term -f (x) s (see sepk_pair_epilogue).
 __device__ __forceinline__ void sepk_pair_accumulate(
-	double ft, double e, double dx, double dy, double dz, double xx, double yy, double zz,
-	double &fx, double &fy, double &fz, double (&acc)[7])
+	double ft, double e, double dx, double dy, double dz, double kx, double ky, double kz,
+	double L0, double L1, double L2, double &fx, double &fy, double &fz, double (&acc)[7])
 {
 	fx = fma(ft, dx, fx);
 	fy = fma(ft, dy, fy);
 	fz = fma(ft, dz, fz);
 	acc[0] += e;
-	acc[1] = fma(ft, xx, acc[1]);
-	acc[2] = fma(ft, dx * dy, acc[2]);
-	acc[3] = fma(ft, dx * dz, acc[3]);
-	acc[4] = fma(ft, yy, acc[4]);
-	acc[5] = fma(ft, dy * dz, acc[5]);
-	acc[6] = fma(ft, zz, acc[6]);
+	if (kx != 0.0 || ky != 0.0 || kz != 0.0) {
+		const double sx = kx * L0, sy = ky * L1, sz = kz * L2;
+		const double gx = -ft * dx, gy = -ft * dy, gz = -ft * dz;
+		acc[1] = fma(gx, sx, acc[1]);
+		acc[2] = fma(gx, sy, acc[2]);
+		acc[3] = fma(gx, sz, acc[3]);
+		acc[4] = fma(gy, sy, acc[4]);
+		acc[5] = fma(gy, sz, acc[5]);
+		acc[6] = fma(gz, sz, acc[6]);
+	}
 }
 
 // Entries of one lane are taken SEPK_BATCH at a time: every index and
@@ -152,8 +187,9 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 		ti = s.type[t];
 		act = ti == c.t0 || ti == c.t1;
 	}
+	sepk_pos pi = {0.0, 0.0, 0.0, 0.0};
 	if (act) {
-		const sepk_pos pi = s.pos[t];
+		pi = s.pos[t];
 		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 		const size_t stride = (size_t)npad * SEPK_SUB;
 		const int *lp = nbr + (size_t)t * SEPK_SUB + sub;
@@ -179,12 +215,12 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 				if (TYPED && !((ti == c.t0 && tj[u] == c.t1) ||
 					       (ti == c.t1 && tj[u] == c.t0)))
 					continue;
-				const double dx = sepk_image<FAST>(pi.x - px[u], c.L0, c.iL0);
-				const double dy = sepk_image<FAST>(pi.y - py[u], c.L1, c.iL1);
-				const double dz = sepk_image<FAST>(pi.z - pz[u], c.L2, c.iL2);
-				// r^2 in the reference's order; its squares feed the virial
-				const double xx = dx * dx, yy = dy * dy, zz = dz * dz;
-				const double r2 = xx + yy + zz;
+				double kx, ky, kz;
+				const double dx = sepk_image<FAST>(pi.x - px[u], c.L0, c.iL0, kx);
+				const double dy = sepk_image<FAST>(pi.y - py[u], c.L1, c.iL1, ky);
+				const double dz = sepk_image<FAST>(pi.z - pz[u], c.L2, c.iL2, kz);
+				// r^2 in the reference's order
+				const double r2 = dx * dx + dy * dy + dz * dz;
 				if (!(r2 < c.cf2))
 					continue;
 				double ft, e;
@@ -199,12 +235,13 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 					ft = fma(c.A, q3, c.nB) * q3 * q;
 					e = fma(fma(c.C, q3, c.nD), q3, c.ushift);
 				}
-				sepk_pair_accumulate(ft, e, dx, dy, dz, xx, yy, zz, fx, fy, fz, acc);
+				sepk_pair_accumulate(ft, e, dx, dy, dz, kx, ky, kz, c.L0, c.L1, c.L2,
+						     fx, fy, fz, acc);
 			}
 		}
 	}
-	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, acc, s.f, partial, lb,
-				      rows);
+	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, pi.x, pi.y, pi.z, acc,
+				      s.f, partial, lb, rows);
 }
 
 typedef void (*sepk_lj_kernel)(sepk_state, const int *, const int *, long, long,
@@ -223,7 +260,8 @@ static sepk_lj_kernel pick_lj(bool over, bool fast)
 }
 
 // The fast image is exact for listed pairs when half the box exceeds the
-// list cutoff plus the skin with room to spare (see sepk_wrap_fast).
+// list cutoff plus the skin with room to spare (see sepk_wrap_fast and
+// sepk_image).
 static bool fast_image_ok(const sepdev *d, const double *L)
 {
 	if (!d->list_built)
@@ -392,23 +430,24 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 				// the phantom (charge 0) is skipped here
 				if (fabs(pj[u].q) < DBL_EPSILON)
 					continue;
-				const double dx = sepk_image<FAST>(pi.x - pj[u].x, c.L0, c.iL0);
-				const double dy = sepk_image<FAST>(pi.y - pj[u].y, c.L1, c.iL1);
-				const double dz = sepk_image<FAST>(pi.z - pj[u].z, c.L2, c.iL2);
-				const double xx = dx * dx, yy = dy * dy, zz = dz * dz;
-				const double r2 = xx + yy + zz;
+				double kx, ky, kz;
+				const double dx = sepk_image<FAST>(pi.x - pj[u].x, c.L0, c.iL0, kx);
+				const double dy = sepk_image<FAST>(pi.y - pj[u].y, c.L1, c.iL1, ky);
+				const double dz = sepk_image<FAST>(pi.z - pj[u].z, c.L2, c.iL2, kz);
+				const double r2 = dx * dx + dy * dy + dz * dz;
 				if (!(r2 < c.cf2))
 					continue;
 				const double qq = pj[u].q * pi.q;
 				const double r = sqrt(r2);
 				const double ft = qq * (1.0 / r2 - c.icf2) / r;
 				const double e = qq * (1.0 / r + (r - c.cf) * c.icf2 - c.icf);
-				sepk_pair_accumulate(ft, e, dx, dy, dz, xx, yy, zz, fx, fy, fz, acc);
+				sepk_pair_accumulate(ft, e, dx, dy, dz, kx, ky, kz, c.L0, c.L1, c.L2,
+						     fx, fy, fz, acc);
 			}
 		}
 	}
-	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, acc, s.f, partial, lb,
-				      rows);
+	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, pi.x, pi.y, pi.z, acc,
+				      s.f, partial, lb, rows);
 }
 
 extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot)

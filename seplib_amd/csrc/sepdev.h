@@ -182,6 +182,33 @@ int sepdev_graph_end(sepdev *d);
 int sepdev_graph_launch(sepdev *d, long times);
 int sepdev_graph_drop(sepdev *d);
 
+/* ---- batches: many independent LJ systems (sep_force_pairs "AA" with
+ * sep_lj_shift, sep_nosehoover, sep_leapfrog each), one persistent launch
+ * per step.  Host arrays in original order, 3n interleaved. */
+typedef struct sepdev_batch sepdev_batch;
+typedef struct {
+	long n;
+	double L[3];
+	double rlist2;                 /* (sys.cf + skin)^2                 */
+	double dt, Td, tau, skin, alpha;
+	double tnow;
+	const double *x, *v, *m;
+	const int *cr;                 /* crossings (may be null)           */
+} sepdev_batch_sys;
+sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *sys, double cf,
+				  int cap, int *err);
+void sepdev_batch_destroy(sepdev_batch *b);
+int sepdev_batch_run(sepdev_batch *b, long nsteps, int mixed, int graph);
+int sepdev_batch_eval(sepdev_batch *b, int mixed);
+int sepdev_batch_synchronize(sepdev_batch *b);
+/* scal: alpha, max_dist2, tnow, nupdate, neighb_flag, max_full, error, sumv2 */
+int sepdev_batch_get(sepdev_batch *b, int s, double *x, double *v, double *f,
+		     double *a, int *cr, double *ret, double *scal);
+int sepdev_batch_set_shape(sepdev_batch *b, const int *sub, int grid);
+int sepdev_batch_get_shape(sepdev_batch *b, int *sub, int *grid);
+int sepdev_batch_time_eval(sepdev_batch *b, const int *mask, int mixed, int reps,
+			   double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -197,6 +197,19 @@ def load_product(path=PRODUCT_LIB):
       C.c_uint)
     f("sep_gpu_lj_adjoint", C.c_int, P_ATOMS, P_SYS, C.c_int, C.c_char_p, P_D, P_D, P_D, P_D)
     f("sep_gpu_synchronize", C.c_int, P_ATOMS)
+    P_I = C.POINTER(C.c_int)
+    f("sep_gpu_batch_create", C.c_void_p, C.c_int, C.POINTER(P_ATOMS), C.POINTER(P_SYS),
+      C.c_char_p, C.c_double, P_D, P_D, P_D, C.c_int, C.c_int)
+    f("sep_gpu_batch_free", None, C.c_void_p)
+    f("sep_gpu_batch_run", C.c_int, C.c_void_p, C.c_long)
+    f("sep_gpu_batch_run_eager", C.c_int, C.c_void_p, C.c_long)
+    f("sep_gpu_batch_eval", C.c_int, C.c_void_p, C.c_int)
+    f("sep_gpu_batch_sync", C.c_int, C.c_void_p, P_RET, P_D)
+    f("sep_gpu_batch_synchronize", C.c_int, C.c_void_p)
+    f("sep_gpu_batch_set_shape", C.c_int, C.c_void_p, P_I, C.c_int)
+    f("sep_gpu_batch_get_shape", C.c_int, C.c_void_p, P_I, P_I)
+    f("sep_gpu_batch_time_eval", C.c_int, C.c_void_p, P_I, C.c_int, P_D)
+    f("sep_gpu_batch_tune", C.c_int, C.c_void_p, C.c_int, P_I, P_I)
     return lib
 
 

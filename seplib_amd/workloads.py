@@ -157,6 +157,24 @@ def ka_mixture(n, rho, temp, seed=5, frac_a=0.8):
                 type=types, L=np.array([L, L, L]), mol=np.full(n, -1))
 
 
+def random_packed(n, rho, temp, seed=7):
+    """One-type LJ start: n of the sites of the smallest simple-cubic lattice
+    with at least n sites, chosen at random and jittered by up to 5 % of
+    the spacing (no two atoms closer than 0.9 spacings)."""
+    L = box_length(n, rho)
+    k = int(math.ceil(n ** (1.0 / 3.0) - 1e-9))
+    a = L / k
+    g = np.arange(k)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    sites = (np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1) + 0.5) * a
+    order = np.argsort(uniform(seed, len(sites)))[:n]
+    pos = sites[np.sort(order)] + (uniform(seed + 1, 3 * n).reshape(n, 3) - 0.5) * 0.1 * a
+    pos = np.mod(pos, L)
+    m = np.ones(n)
+    return dict(x=pos, v=maxwell_boltzmann(n, temp, m, seed + 3), m=m, q=np.zeros(n),
+                type=np.full(n, b"A"), L=np.array([L, L, L]), mol=np.full(n, -1))
+
+
 def write_xyz(path, w):
     """Writes the sep_init_xyz text format: N, box, type x y z vx vy vz m q."""
     n = len(w["m"])
