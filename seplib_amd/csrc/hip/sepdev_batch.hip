@@ -8,10 +8,9 @@
 // owns slots off[s] .. off[s] + npad[s], npad = a multiple of 64 above n,
 // slot n being a phantom with a NaN position that pads lists).  Per step:
 //
-//   k_bbin        cell of every atom of a flagged system and its slot among
-//                 the cell's members; the last wave of a system to finish
-//                 scans the system's cell counts into cell starts
-//   k_bscatter    the members of every cell, in slot order
+//   k_bbin        one workgroup per flagged system: cell of every atom
+//                 (LDS counters), scan of the counts into cell starts, and
+//                 the members of every cell in slot order
 //   k_bsort       one wave per cell of a flagged system: rank the members
 //                 by original index (the new order, independent of the
 //                 atomic order that listed them) and move their state to it
@@ -19,18 +18,20 @@
 //   k_bbuild      one wave per cell of a flagged system: the candidates of
 //                 the distinct neighbour cells, 64 at a time in the lanes,
 //                 tested against each home atom in turn and appended by
-//                 ballot in candidate order; the last cell of a system swaps
-//                 the copies
+//                 ballot in candidate order
 //   k_bstep       persistent: the grid walks a table of tiles (a system and
 //                 a run of its atoms, lanes per atom chosen per system);
 //                 pair forces, Nose-Hoover and leapfrog for the tile's
-//                 atoms, a row of block sums, and the last tile of a
-//                 system to finish reduces that system's rows in tile order
-//                 and applies its finishing step (energies, stress,
-//                 thermostat, rebuild flag).
+//                 atoms and one row of block sums per tile
+//   k_bfinish     one workgroup per system: its rows reduced in tile order,
+//                 then energies, stress, thermostat, the state copy and
+//                 position slot in use, and the rebuild flag.
 //
-// The first four return at once for systems whose flag is clear, so the
-// five launches replay from one captured graph.  Positions are double
+// The first three return at once for systems whose flag is clear, so the
+// five launches replay from one captured graph.  Workgroups never wait for
+// each other inside a launch: on this chip a device-scope fence writes back
+// and invalidates an XCD's L2, so every combination across workgroups
+// happens at a kernel boundary instead.  Positions are double
 // buffered inside a state copy (a tile reads slot p and writes slot 1-p;
 // other atoms' tiles still read slot p), every other per-atom field is
 // written only by its own atom.  Unlike the reference's cell-list builder,
@@ -54,15 +55,15 @@ struct sepb_sys {
 	int n, npad;
 	int sub;          // lanes per atom of its tiles and its list layout
 	int ntile, row0;  // its tiles' partial rows: row0 .. row0 + ntile - 1
-	int ncell[3], cell_off;  // cell_count/cell_start slots: cell_off ..
+	int ncell[3], cell_off;  // cell_start slots: cell_off .. + ncells
 	double lcell[3], L[3], iL[3];
 	double rlist2;
 	double dt, Td, dt_tau2, skin;
 	int cur, p;       // state copy in use, position slot in use
 	unsigned neighb_flag, nupdate;
 	int error, max_full;
+	int unclean;      // the last binning saw an atom outside [0, L)
 	double alpha, sumv2, max_dist2, tnow;
-	unsigned arrive_step, arrive_build, arrive_bin;
 	double ret[SEPDEV_RET_WORDS];
 };
 
@@ -85,9 +86,8 @@ struct sepb_view {
 	const sepb_state *st; // two copies, device memory
 	long slots;
 	int nsys;
-	const int *chunk_sys; // system of every 64-slot chunk
 	int *cellid, *cslot, *otmp;
-	int *cell_count, *cell_start;
+	int *cell_start;
 	const int *cell_sys;  // system of every global cell index
 	long ncell_tot;
 	int *nbr, *nbr_count;
@@ -104,7 +104,7 @@ struct sepdev_batch {
 	sepb_state *dS;
 	sepb_sys *dsys;
 	sepb_sys *hsys; // host copy of the descriptors as last uploaded
-	int *chunk_sys, *cellid, *cslot, *otmp, *cell_count, *cell_start, *cell_sys;
+	int *cellid, *cslot, *otmp, *cell_start, *cell_sys;
 	int *nbr, *nbr_count;
 	int cap;
 	double *partial;
@@ -127,17 +127,59 @@ __device__ __forceinline__ int sepb_mod(int a, int n)
 
 // ---- rebuild ------------------------------------------------------------------
 
-// Exclusive scan of a system's cell counts into cell_start (one extra
-// entry: n) by one wave, and the counts back to zero for the next build.
-__device__ void sepb_scan_cells(sepb_view &b, sepb_sys *y)
+#define SEPB_BIN_THREADS 1024
+#define SEPB_MAX_CELLS 8192 // cells per system (LDS counters)
+
+// One workgroup per flagged system: the cell of every atom, its slot among
+// the cell's members (LDS atomics), the exclusive scan of the counts into
+// cell_start (one extra entry: n), and otmp[off + start + slot] = atom.
+__global__ __launch_bounds__(SEPB_BIN_THREADS) void k_bbin(sepb_view b)
 {
-	const int lane = threadIdx.x & 63;
+	__shared__ int cnt[SEPB_MAX_CELLS];
+	__shared__ int wsum[SEPB_BIN_THREADS / 64];
+	sepb_sys *y = b.sys + blockIdx.x;
+	if (!y->neighb_flag)
+		return;
 	const int nc = y->ncell[0] * y->ncell[1] * y->ncell[2];
-	int *cnt = b.cell_count + y->cell_off;
+	const long off = y->off;
+	const int n = y->n;
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	__shared__ int outside;
+	for (int c = threadIdx.x; c < nc; c += SEPB_BIN_THREADS)
+		cnt[c] = 0;
+	if (threadIdx.x == 0)
+		outside = 0;
+	__syncthreads();
+	const sepk_pos *pos = b.st[y->cur].pos[y->p] + off;
+	for (int t = threadIdx.x; t < n; t += SEPB_BIN_THREADS) {
+		const sepk_pos q = pos[t];
+		const double xs[3] = {q.x, q.y, q.z};
+		int ci[3];
+		for (int k = 0; k < 3; k++) {
+			const double u = xs[k] / y->lcell[k];
+			if (!(fabs(u) < 1.0e9)) {
+				atomicCAS(&y->error, 0, SEP_GPU_ECELL);
+				ci[k] = 0;
+			} else {
+				const int f = (int)floor(u);
+				// an atom outside [0, L) on some axis (x == L after a
+				// wrap, or a starting position) rules out the
+				// per-segment image shifts of the build
+				if (f < 0 || f >= y->ncell[k])
+					outside = 1;
+				ci[k] = sepb_mod(f, y->ncell[k]);
+			}
+		}
+		const int c = ci[0] + y->ncell[0] * (ci[1] + y->ncell[1] * ci[2]);
+		b.cellid[off + t] = c;
+		b.cslot[off + t] = atomicAdd(&cnt[c], 1);
+	}
+	__syncthreads();
+	// exclusive scan of cnt in place, SEPB_BIN_THREADS cells at a time
 	int *st = b.cell_start + y->cell_off;
 	int carry = 0;
-	for (int base = 0; base < nc; base += 64) {
-		const int c = base + lane;
+	for (int base = 0; base < nc; base += SEPB_BIN_THREADS) {
+		const int c = base + threadIdx.x;
 		const int v = c < nc ? cnt[c] : 0;
 		int incl = v;
 		for (int o = 1; o < 64; o <<= 1) {
@@ -145,91 +187,37 @@ __device__ void sepb_scan_cells(sepb_view &b, sepb_sys *y)
 			if (lane >= o)
 				incl += u;
 		}
+		if (lane == 63)
+			wsum[wid] = incl;
+		__syncthreads();
+		int before = carry, tot = 0;
+		for (int w = 0; w < SEPB_BIN_THREADS / 64; w++) {
+			if (w < wid)
+				before += wsum[w];
+			tot += wsum[w];
+		}
 		if (c < nc) {
-			st[c] = carry + incl - v;
-			cnt[c] = 0;
+			cnt[c] = before + incl - v;
+			st[c] = before + incl - v;
 		}
-		carry += __shfl(incl, 63, 64);
+		carry += tot;
+		__syncthreads();
 	}
-	if (lane == 0)
+	if (threadIdx.x == 0) {
 		st[nc] = carry;
+		y->unclean = outside;
+	}
+	for (int t = threadIdx.x; t < n; t += SEPB_BIN_THREADS)
+		b.otmp[off + cnt[b.cellid[off + t]] + b.cslot[off + t]] = t;
 }
 
-__global__ __launch_bounds__(SEPB_BLOCK) void k_bbin(sepb_view b)
+// A lane's double as a uniform value.
+__device__ __forceinline__ double sepb_readlane(double v, int l)
 {
-	const long stride = (long)gridDim.x * blockDim.x;
-	const int lane = threadIdx.x & 63;
-	const unsigned long long below = (1ull << lane) - 1ull;
-	// whole waves iterate together (slots is a multiple of 64)
-	for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < b.slots; g += stride) {
-		const int s = b.chunk_sys[g >> 6];
-		sepb_sys *y = b.sys + s;
-		if (!y->neighb_flag)
-			continue; // uniform over the wave: a chunk is one system
-		const long t = g - y->off;
-		const bool act = t < y->n;
-		int c = 0;
-		if (act) {
-			const sepk_pos q = b.st[y->cur].pos[y->p][g];
-			const double xs[3] = {q.x, q.y, q.z};
-			int ci[3];
-			for (int k = 0; k < 3; k++) {
-				const double u = xs[k] / y->lcell[k];
-				if (!(fabs(u) < 1.0e9)) {
-					atomicCAS(&y->error, 0, SEP_GPU_ECELL);
-					ci[k] = 0;
-				} else {
-					ci[k] = sepb_mod((int)floor(u), y->ncell[k]);
-				}
-			}
-			c = ci[0] + y->ncell[0] * (ci[1] + y->ncell[1] * ci[2]);
-			b.cellid[g] = c;
-		}
-		// one atomic per distinct cell of the wave
-		unsigned long long todo = __ballot(act);
-		int my = 0;
-		while (todo) {
-			const int lead = __ffsll((long long)todo) - 1;
-			const int cl = __shfl(c, lead, 64);
-			const unsigned long long same = __ballot(act && c == cl);
-			int base = 0;
-			if (lane == lead)
-				base = atomicAdd(&b.cell_count[y->cell_off + cl], __popcll(same));
-			base = __shfl(base, lead, 64);
-			if (act && c == cl)
-				my = base + __popcll(same & below);
-			todo &= ~same;
-		}
-		if (act)
-			b.cslot[g] = my;
-		// the system's last wave to finish scans its counts
-		__threadfence();
-		unsigned prev = 0;
-		if (lane == 0)
-			prev = atomicAdd(&y->arrive_bin, 1u);
-		prev = __shfl(prev, 0, 64);
-		if (prev == (unsigned)(y->npad / 64 - 1)) {
-			__threadfence();
-			sepb_scan_cells(b, y);
-			if (lane == 0)
-				y->arrive_bin = 0u;
-		}
-	}
-}
-
-// otmp[off + start of its cell + slot] = local index, for every atom of a
-// flagged system.
-__global__ __launch_bounds__(SEPB_BLOCK) void k_bscatter(sepb_view b)
-{
-	const long stride = (long)gridDim.x * blockDim.x;
-	for (long g = (long)blockIdx.x * blockDim.x + threadIdx.x; g < b.slots; g += stride) {
-		const sepb_sys *y = b.sys + b.chunk_sys[g >> 6];
-		if (!y->neighb_flag)
-			continue;
-		const long t = g - y->off;
-		if (t < y->n)
-			b.otmp[y->off + b.cell_start[y->cell_off + b.cellid[g]] + b.cslot[g]] = (int)t;
-	}
+	const long long bits = __double_as_longlong(v);
+	const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)bits, l);
+	const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(bits >> 32), l);
+	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
 // Rank of `key` among the original indices of the m members seg[0..m) of a
@@ -311,21 +299,55 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bsort(sepb_view b)
 	}
 }
 
-// Per wave: the home atoms of the cell being built (64 at a time), their
-// list lengths, and the candidate segments (one per distinct neighbour
-// cell: first new index and prefix of lengths).
+// Per wave: the candidate segments of the cell being built (one per
+// distinct neighbour cell: first new index, prefix of lengths, and the
+// image shift that brings the cell next to the home cell).
 struct sepb_build_lds {
-	double hx[64], hy[64], hz[64];
-	int hcnt[64];
 	int segjs[27], segpre[28];
+	double segsh[27][3];
 };
+
+// The candidate loop of one run of home atoms against 64 candidates held
+// in the lanes.  SHIFT: the candidates carry their segment's image shift
+// and r^2 needs no wrap (cells >= 3 per axis, every atom inside the box);
+// otherwise the single-image wrap of every difference.
+template <bool SHIFT>
+__device__ __forceinline__ void sepb_build_chunk(
+	int hn, int h, double hxr, double hyr, double hzr, int &hc, bool valid, int j, double cx,
+	double cy, double cz, double L0, double L1, double L2, double rl2, int cap, int lg,
+	int sub, int lstride, int *lst)
+{
+	const int lane = threadIdx.x & 63;
+	const unsigned long long below = (1ull << lane) - 1ull;
+	for (int i = 0; i < hn; i++) {
+		double ddx = sepb_readlane(hxr, i) - cx;
+		double ddy = sepb_readlane(hyr, i) - cy;
+		double ddz = sepb_readlane(hzr, i) - cz;
+		if (!SHIFT) {
+			ddx = sepk_wrap(ddx, L0);
+			ddy = sepk_wrap(ddy, L1);
+			ddz = sepk_wrap(ddz, L2);
+		}
+		const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+		const int hi = h + i;
+		const bool hit = valid && r2 < rl2 && j != hi;
+		const unsigned long long bal = __ballot(hit);
+		if (bal) {
+			const int base = __builtin_amdgcn_readlane(hc, i);
+			const int e = base + __popcll(bal & below);
+			if (hit && e < cap)
+				lst[(e >> lg) * lstride + hi * sub + (e & (sub - 1))] = j;
+			if (lane == i)
+				hc = base + __popcll(bal);
+		}
+	}
+}
 
 __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 {
 	__shared__ sepb_build_lds Lall[SEPB_BLOCK / 64];
 	const int lane = threadIdx.x & 63;
 	sepb_build_lds &L = Lall[threadIdx.x >> 6];
-	const unsigned long long below = (1ull << lane) - 1ull;
 	const long nw = ((long)gridDim.x * blockDim.x) >> 6;
 	for (long gc = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gc < b.ncell_tot;
 	     gc += nw) {
@@ -348,7 +370,9 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 		const double rl2 = y->rlist2;
 		const int h0 = st[c], m = st[c + 1] - h0;
 		int *lst = b.nbr + y->list_off;
-		const long lstride = (long)npad * sub;
+		// a system's list region holds cap * npad < 2^31 entries
+		const int lstride = npad * sub;
+		const bool shift = nx >= 3 && ny >= 3 && nz >= 3 && !y->unclean;
 		// the distinct neighbour cells, in a fixed order
 		int ox[3], oy[3], oz[3];
 		const int nox = sepb_offsets(nx, ox), noy = sepb_offsets(ny, oy),
@@ -356,11 +380,13 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 		const int nseg = nox * noy * noz;
 		int len = 0;
 		if (lane < nseg) {
-			const int dx = ox[lane % nox], dy = oy[(lane / nox) % noy],
-				  dz = oz[lane / (nox * noy)];
-			const int c2 = sepb_mod(cxi + dx, nx) +
-				       nx * (sepb_mod(cyi + dy, ny) + ny * sepb_mod(czi + dz, nz));
+			const int ux = cxi + ox[lane % nox], uy = cyi + oy[(lane / nox) % noy],
+				  uz = czi + oz[lane / (nox * noy)];
+			const int c2 = sepb_mod(ux, nx) + nx * (sepb_mod(uy, ny) + ny * sepb_mod(uz, nz));
 			L.segjs[lane] = st[c2];
+			L.segsh[lane][0] = ux >= nx ? L0 : (ux < 0 ? -L0 : 0.0);
+			L.segsh[lane][1] = uy >= ny ? L1 : (uy < 0 ? -L1 : 0.0);
+			L.segsh[lane][2] = uz >= nz ? L2 : (uz < 0 ? -L2 : 0.0);
 			len = st[c2 + 1] - st[c2];
 		}
 		int incl = len;
@@ -377,52 +403,52 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 		__builtin_amdgcn_wave_barrier();
 		for (int hb = 0; hb < m; hb += 64) {
 			const int hn = min(m - hb, 64);
+			// lane i holds home atom i's position and list length; the
+			// candidate loop reads them with readlane (uniform)
+			double hxr = 0.0, hyr = 0.0, hzr = 0.0;
+			int hc = 0;
 			if (lane < hn) {
 				const sepk_pos q = pn[h0 + hb + lane];
-				L.hx[lane] = q.x;
-				L.hy[lane] = q.y;
-				L.hz[lane] = q.z;
-				L.hcnt[lane] = 0;
+				hxr = q.x;
+				hyr = q.y;
+				hzr = q.z;
 			}
-			__builtin_amdgcn_wave_barrier();
 			for (int q0 = 0; q0 < T; q0 += 64) {
 				const int q = q0 + lane;
 				const bool valid = q < T;
 				int j = 0;
 				double cx = 0.0, cy = 0.0, cz = 0.0;
 				if (valid) {
+					// the segment holding candidate q: the last one
+					// starting at or before q
 					int r = 0;
-					while (r + 1 < nseg && L.segpre[r + 1] <= q)
-						r++;
+					for (int step = 16; step > 0; step >>= 1)
+						if (r + step < nseg && L.segpre[r + step] <= q)
+							r += step;
 					j = L.segjs[r] + (q - L.segpre[r]);
 					const sepk_pos cq = pn[j];
 					cx = cq.x;
 					cy = cq.y;
 					cz = cq.z;
-				}
-				for (int i = 0; i < hn; i++) {
-					const int h = h0 + hb + i;
-					const double ddx = sepk_wrap(L.hx[i] - cx, L0);
-					const double ddy = sepk_wrap(L.hy[i] - cy, L1);
-					const double ddz = sepk_wrap(L.hz[i] - cz, L2);
-					const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-					const bool hit = valid && r2 < rl2 && j != h;
-					const unsigned long long bal = __ballot(hit);
-					if (bal) {
-						const int base = L.hcnt[i];
-						const int e = base + __popcll(bal & below);
-						if (hit && e < cap)
-							lst[(long)(e >> lg) * lstride + (long)h * sub + (e & (sub - 1))] = j;
-						__builtin_amdgcn_wave_barrier();
-						if (lane == 0)
-							L.hcnt[i] = base + __popcll(bal);
-						__builtin_amdgcn_wave_barrier();
+					if (shift) {
+						cx += L.segsh[r][0];
+						cy += L.segsh[r][1];
+						cz += L.segsh[r][2];
 					}
 				}
+				if (shift)
+					sepb_build_chunk<true>(hn, h0 + hb, hxr, hyr, hzr, hc, valid, j, cx, cy,
+							       cz, L0, L1, L2, rl2, cap, lg, sub, lstride, lst);
+				else
+					sepb_build_chunk<false>(hn, h0 + hb, hxr, hyr, hzr, hc, valid, j, cx,
+								cy, cz, L0, L1, L2, rl2, cap, lg, sub, lstride,
+								lst);
 			}
+			int full = 0;
 			if (lane < hn) {
 				const int h = h0 + hb + lane;
-				int cnt = L.hcnt[lane];
+				int cnt = hc;
+				full = cnt;
 				if (cnt > cap) {
 					atomicCAS(&y->error, 0, SEP_GPU_ENEIGHB);
 					cnt = cap;
@@ -431,24 +457,16 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 				const int qq = sub * SEPB_BATCH;
 				const int pad = min((cnt + qq - 1) / qq * qq, cap);
 				for (int e = cnt; e < pad; e++)
-					lst[(long)(e >> lg) * lstride + (long)h * sub + (e & (sub - 1))] = n;
+					lst[(e >> lg) * lstride + h * sub + (e & (sub - 1))] = n;
 				b.nbr_count[off + h] = cnt;
-				atomicMax(&y->max_full, cnt);
 			}
+			// one atomic per wave: every home atom of a system hitting
+			// one address would serialise in L2
+			for (int o = 32; o > 0; o >>= 1)
+				full = max(full, __shfl_xor(full, o, 64));
+			if (lane == 0 && full > 0)
+				atomicMax(&y->max_full, full);
 			__builtin_amdgcn_wave_barrier();
-		}
-		// the last cell of the system swaps the copies
-		__threadfence();
-		unsigned prev = 0;
-		if (lane == 0)
-			prev = atomicAdd(&y->arrive_build, 1u);
-		prev = __shfl(prev, 0, 64);
-		if (prev == (unsigned)(nx * ny * nz - 1) && lane == 0) {
-			__threadfence();
-			y->cur = 1 - y->cur;
-			y->p = 0;
-			y->neighb_flag = 0;
-			y->arrive_build = 0u;
 		}
 	}
 }
@@ -461,16 +479,16 @@ template <bool MIXED, bool INTEGRATE>
 __global__ __launch_bounds__(SEPB_BLOCK) void k_bstep(sepb_view b, sepb_lj c, const int4 *tiles,
 						      int ntiles)
 {
-	__shared__ double tot[SEPB_NV];
-	__shared__ double red[16][SEPB_NV];
-	__shared__ int last;
 	for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 		const int4 td = tiles[tile]; // system, first atom, row
 		sepb_sys *y = b.sys + td.x;
 		const long off = y->off;
 		const int n = y->n, npad = y->npad, sub = y->sub;
-		const sepb_state &S = b.st[y->cur];
-		const int p = y->p;
+		// a system rebuilt this step (flag still set) is in the spare
+		// copy, at position slot 0
+		const bool rebuilt = y->neighb_flag != 0u;
+		const sepb_state &S = b.st[rebuilt ? 1 - y->cur : y->cur];
+		const int p = rebuilt ? 0 : y->p;
 		const long slots = b.slots;
 		const double L0 = y->L[0], L1 = y->L[1], L2 = y->L[2];
 		const int t = td.y + (int)threadIdx.x / sub;
@@ -623,57 +641,67 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bstep(sepb_view b, sepb_lj c, co
 			}
 		}
 		sepk_block_partial<SEPB_NV>(acc, 1u << 15, b.partial, td.z, b.prow);
-		// the last tile of the system to finish combines its rows
-		__threadfence();
-		if (threadIdx.x == 0)
-			last = atomicAdd(&y->arrive_step, 1u) == (unsigned)(y->ntile - 1);
-		__syncthreads();
-		if (last) {
-			__threadfence();
-			{
-				// 16 threads per column over strided rows, then the 16
-				// part sums in order (a fixed order: deterministic)
-				const int col = threadIdx.x & 15, part = threadIdx.x >> 4;
-				const double *r = b.partial + (size_t)col * b.prow + y->row0;
-				double sacc = 0.0;
-				for (int k = part; k < y->ntile; k += 16)
-					sacc = col == 15 ? fmax(sacc, r[k]) : sacc + r[k];
-				red[part][col] = sacc;
-			}
-			__syncthreads();
-			if (threadIdx.x < SEPB_NV) {
-				const int col = threadIdx.x;
-				double sacc = red[0][col];
-				for (int k = 1; k < 16; k++)
-					sacc = col == 15 ? fmax(sacc, red[k][col]) : sacc + red[k][col];
-				tot[col] = sacc;
-			}
-			__syncthreads();
-			if (threadIdx.x == 0) {
-				double *ret = y->ret;
-				for (int k = 0; k < SEPDEV_RET_WORDS; k++)
-					ret[k] = 0.0;
-				// each pair was seen from both ends
-				ret[SEPDEV_RET_EPOT] = 0.5 * tot[0];
-				sepk_add_sym9(ret + SEPDEV_RET_POTP, tot + 1, 0.5);
-				if (INTEGRATE) {
-					ret[SEPDEV_RET_EKIN] = 0.5 * tot[7];
-					sepk_add_sym9(ret + SEPDEV_RET_KINP, tot + 8, 1.0);
-					y->alpha = al;
-					y->sumv2 = tot[14];
-					y->max_dist2 = tot[15];
-					if (sqrt(tot[15]) > 0.5 * y->skin) {
-						y->neighb_flag = 1u;
-						y->nupdate++;
-					}
-					y->tnow += y->dt;
-					y->p = 1 - p;
-				}
-				y->arrive_step = 0u;
-			}
-		}
-		__syncthreads();
+		__syncthreads(); // sepk_block_partial's LDS is reused by the next tile
 	}
+}
+
+// One workgroup per system, after k_bstep: the system's rows reduced in
+// tile order (16 threads per column over strided rows, then the 16 part
+// sums in order: a fixed order, so deterministic) and the finishing step.
+template <bool INTEGRATE>
+__global__ __launch_bounds__(SEPB_BLOCK) void k_bfinish(sepb_view b)
+{
+	__shared__ double red[16][SEPB_NV];
+	__shared__ double tot[SEPB_NV];
+	sepb_sys *y = b.sys + blockIdx.x;
+	{
+		const int col = threadIdx.x & 15, part = threadIdx.x >> 4;
+		const double *r = b.partial + (size_t)col * b.prow + y->row0;
+		const int nt = y->ntile;
+		double sacc = 0.0;
+		for (int k = part; k < nt; k += 16)
+			sacc = col == 15 ? fmax(sacc, r[k]) : sacc + r[k];
+		red[part][col] = sacc;
+	}
+	__syncthreads();
+	if (threadIdx.x < SEPB_NV) {
+		const int col = threadIdx.x;
+		double sacc = red[0][col];
+		for (int k = 1; k < 16; k++)
+			sacc = col == 15 ? fmax(sacc, red[k][col]) : sacc + red[k][col];
+		tot[col] = sacc;
+	}
+	__syncthreads();
+	if (threadIdx.x != 0)
+		return;
+	double *ret = y->ret;
+	for (int k = 0; k < SEPDEV_RET_WORDS; k++)
+		ret[k] = 0.0;
+	// each pair was seen from both ends
+	ret[SEPDEV_RET_EPOT] = 0.5 * tot[0];
+	sepk_add_sym9(ret + SEPDEV_RET_POTP, tot + 1, 0.5);
+	const bool rebuilt = y->neighb_flag != 0u;
+	int p = y->p;
+	if (rebuilt) {
+		// k_bsort moved the state to the spare copy, slot 0
+		y->cur = 1 - y->cur;
+		p = 0;
+		y->neighb_flag = 0u;
+	}
+	if (INTEGRATE) {
+		ret[SEPDEV_RET_EKIN] = 0.5 * tot[7];
+		sepk_add_sym9(ret + SEPDEV_RET_KINP, tot + 8, 1.0);
+		y->alpha = sepk_nh_alpha(y->alpha, y->sumv2, (double)y->n, y->Td, y->dt_tau2);
+		y->sumv2 = tot[14];
+		y->max_dist2 = tot[15];
+		if (sqrt(tot[15]) > 0.5 * y->skin) {
+			y->neighb_flag = 1u;
+			y->nupdate++;
+		}
+		y->tnow += y->dt;
+		p = 1 - p;
+	}
+	y->p = p;
 }
 
 // ---- host side ---------------------------------------------------------------------
@@ -735,11 +763,9 @@ extern "C" void sepdev_batch_destroy(sepdev_batch *b)
 	free_state(b->S[1]);
 	(void)hipFree(b->dS);
 	(void)hipFree(b->dsys);
-	(void)hipFree(b->chunk_sys);
 	(void)hipFree(b->cellid);
 	(void)hipFree(b->cslot);
 	(void)hipFree(b->otmp);
-	(void)hipFree(b->cell_count);
 	(void)hipFree(b->cell_start);
 	(void)hipFree(b->cell_sys);
 	(void)hipFree(b->nbr);
@@ -761,11 +787,9 @@ static sepb_view view_of(const sepdev_batch *b)
 	v.st = b->dS;
 	v.slots = b->slots;
 	v.nsys = b->nsys;
-	v.chunk_sys = b->chunk_sys;
 	v.cellid = b->cellid;
 	v.cslot = b->cslot;
 	v.otmp = b->otmp;
-	v.cell_count = b->cell_count;
 	v.cell_start = b->cell_start;
 	v.cell_sys = b->cell_sys;
 	v.ncell_tot = b->ncell_tot;
@@ -875,6 +899,12 @@ extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *i
 			y.L[k] = a.L[k];
 			y.iL[k] = 1.0 / a.L[k];
 		}
+		if ((long)y.ncell[0] * y.ncell[1] * y.ncell[2] > SEPB_MAX_CELLS) {
+			*err = sepk_fail(SEP_GPU_EARG, "batch: system %d needs more than %d cells", s,
+					 SEPB_MAX_CELLS);
+			sepdev_batch_destroy(b);
+			return nullptr;
+		}
 		y.cell_off = (int)cells;
 		cells += (long)y.ncell[0] * y.ncell[1] * y.ncell[2] + 1;
 		y.rlist2 = a.rlist2;
@@ -897,11 +927,9 @@ extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *i
 	rc |= alloc_state(b->S[1], slots);
 	rc |= balloc(&b->dS, 2);
 	rc |= balloc(&b->dsys, nsys);
-	rc |= balloc(&b->chunk_sys, slots / 64);
 	rc |= balloc(&b->cellid, slots);
 	rc |= balloc(&b->cslot, slots);
 	rc |= balloc(&b->otmp, slots);
-	rc |= balloc(&b->cell_count, cells);
 	rc |= balloc(&b->cell_start, cells);
 	rc |= balloc(&b->cell_sys, cells);
 	rc |= balloc(&b->nbr, (size_t)slots * cap);
@@ -916,6 +944,13 @@ extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *i
 	}
 	for (int s = 0; s < nsys; s++)
 		b->hsys[s].list_off = b->hsys[s].off * (long)cap;
+	for (int s = 0; s < nsys; s++)
+		if ((long)b->hsys[s].npad * cap >= (1L << 31)) {
+			*err = sepk_fail(SEP_GPU_EARG, "batch: system %d: list of %d x %d entries", s,
+					 b->hsys[s].npad, cap);
+			sepdev_batch_destroy(b);
+			return nullptr;
+		}
 	// LJ constants of sep_lj_shift (source/sepmisc.c:132-147)
 	b->lj.cf2 = cf * cf;
 	b->lj.A = 48.0;
@@ -931,12 +966,9 @@ extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *i
 	b->grid = 1024;
 
 	// host images of the per-slot and per-cell tables, then the state
-	int *chunk = (int *)malloc((slots / 64) * sizeof(int));
 	int *csys = (int *)malloc(cells * sizeof(int));
 	for (int s = 0; s < nsys; s++) {
 		const sepb_sys &y = b->hsys[s];
-		for (long k = y.off / 64; k < (y.off + y.npad) / 64; k++)
-			chunk[k] = s;
 		const int nc = y.ncell[0] * y.ncell[1] * y.ncell[2];
 		for (int k = 0; k < nc; k++)
 			csys[y.cell_off + k] = s;
@@ -996,14 +1028,9 @@ extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *i
 	if (e == hipSuccess)
 		e = hipMemcpy(b->dS, b->S, 2 * sizeof(sepb_state), hipMemcpyHostToDevice);
 	if (e == hipSuccess)
-		e = hipMemcpy(b->chunk_sys, chunk, (slots / 64) * sizeof(int), hipMemcpyHostToDevice);
-	if (e == hipSuccess)
 		e = hipMemcpy(b->cell_sys, csys, cells * sizeof(int), hipMemcpyHostToDevice);
 	if (e == hipSuccess)
-		e = hipMemset(b->cell_count, 0, cells * sizeof(int));
-	if (e == hipSuccess)
 		e = hipMemset(b->nbr_count, 0, slots * sizeof(int));
-	free(chunk);
 	free(csys);
 	free(hp);
 	free(hv);
@@ -1026,9 +1053,7 @@ extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *i
 static int launch_rebuild(sepdev_batch *b)
 {
 	const sepb_view v = view_of(b);
-	k_bbin<<<512, SEPB_BLOCK, 0, b->st>>>(v);
-	SEPK_LAUNCH_CHECK();
-	k_bscatter<<<512, SEPB_BLOCK, 0, b->st>>>(v);
+	k_bbin<<<b->nsys, SEPB_BIN_THREADS, 0, b->st>>>(v);
 	SEPK_LAUNCH_CHECK();
 	k_bsort<<<512, SEPB_BLOCK, 0, b->st>>>(v);
 	SEPK_LAUNCH_CHECK();
@@ -1037,24 +1062,34 @@ static int launch_rebuild(sepdev_batch *b)
 	return 0;
 }
 
-static int launch_step(sepdev_batch *b, int mixed, int integrate, const int4 *tiles, int ntiles)
+// The step (or a force evaluation) over `tiles`; `finish` adds the
+// per-system finishing launch, which needs every tile of the batch.
+static int launch_step(sepdev_batch *b, int mixed, int integrate, const int4 *tiles, int ntiles,
+		       int finish)
 {
 	const sepb_view v = view_of(b);
 	const int grid = ntiles < b->grid ? ntiles : b->grid;
-	if (grid <= 0)
-		return 0;
-	if (mixed) {
-		if (integrate)
-			k_bstep<true, true><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
-		else
-			k_bstep<true, false><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
-	} else {
-		if (integrate)
-			k_bstep<false, true><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
-		else
-			k_bstep<false, false><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
+	if (grid > 0) {
+		if (mixed) {
+			if (integrate)
+				k_bstep<true, true><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
+			else
+				k_bstep<true, false><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
+		} else {
+			if (integrate)
+				k_bstep<false, true><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
+			else
+				k_bstep<false, false><<<grid, SEPB_BLOCK, 0, b->st>>>(v, b->lj, tiles, ntiles);
+		}
+		SEPK_LAUNCH_CHECK();
 	}
-	SEPK_LAUNCH_CHECK();
+	if (finish) {
+		if (integrate)
+			k_bfinish<true><<<b->nsys, SEPB_BLOCK, 0, b->st>>>(v);
+		else
+			k_bfinish<false><<<b->nsys, SEPB_BLOCK, 0, b->st>>>(v);
+		SEPK_LAUNCH_CHECK();
+	}
 	return 0;
 }
 
@@ -1065,7 +1100,7 @@ extern "C" int sepdev_batch_run(sepdev_batch *b, long nsteps, int mixed, int gra
 	int rc;
 	if (!graph) {
 		for (long i = 0; i < nsteps; i++)
-			if ((rc = launch_rebuild(b)) || (rc = launch_step(b, mixed, 1, b->tiles, b->ntiles)))
+			if ((rc = launch_rebuild(b)) || (rc = launch_step(b, mixed, 1, b->tiles, b->ntiles, 1)))
 				return rc;
 		return 0;
 	}
@@ -1074,7 +1109,7 @@ extern "C" int sepdev_batch_run(sepdev_batch *b, long nsteps, int mixed, int gra
 		SEPK_CHECK(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
 		rc = launch_rebuild(b);
 		if (!rc)
-			rc = launch_step(b, mixed, 1, b->tiles, b->ntiles);
+			rc = launch_step(b, mixed, 1, b->tiles, b->ntiles, 1);
 		hipGraph_t g = nullptr;
 		const hipError_t e = hipStreamEndCapture(b->st, &g);
 		if (rc) {
@@ -1099,7 +1134,7 @@ extern "C" int sepdev_batch_eval(sepdev_batch *b, int mixed)
 {
 	int rc = launch_rebuild(b);
 	if (!rc)
-		rc = launch_step(b, mixed, 0, b->tiles, b->ntiles);
+		rc = launch_step(b, mixed, 0, b->tiles, b->ntiles, 1);
 	return rc;
 }
 
@@ -1220,7 +1255,9 @@ extern "C" int sepdev_batch_get_shape(sepdev_batch *b, int *sub, int *grid)
 extern "C" int sepdev_batch_time_eval(sepdev_batch *b, const int *mask, int mixed, int reps,
 				      double *ms)
 {
-	int rc = launch_rebuild(b);
+	// a complete evaluation first (rebuilds pending lists and finishes
+	// them), then the tiles of the chosen systems alone
+	int rc = sepdev_batch_eval(b, mixed);
 	if (rc || (rc = get_sys(b)))
 		return rc;
 	int4 *h = (int4 *)malloc(b->prow * sizeof(int4));
@@ -1240,11 +1277,11 @@ extern "C" int sepdev_batch_time_eval(sepdev_batch *b, const int *mask, int mixe
 	hipEvent_t e0, e1;
 	SEPK_CHECK(hipEventCreate(&e0));
 	SEPK_CHECK(hipEventCreate(&e1));
-	rc = launch_step(b, mixed, 0, b->sub_tiles, nt);
+	rc = launch_step(b, mixed, 0, b->sub_tiles, nt, 0);
 	if (!rc)
 		(void)hipEventRecord(e0, b->st);
 	for (int r = 0; r < reps && !rc; r++)
-		rc = launch_step(b, mixed, 0, b->sub_tiles, nt);
+		rc = launch_step(b, mixed, 0, b->sub_tiles, nt, 0);
 	if (!rc) {
 		(void)hipEventRecord(e1, b->st);
 		e = hipEventSynchronize(e1);
