@@ -13,9 +13,10 @@ flagged), sep_nosehoover, sep_leapfrog -- all on the device in resident
 mode, replayed from a captured HIP graph, energies and stress reduced on
 the device every step with no host copy.
 
-The timed region is K steps of the mixed form (FP32 pair arithmetic, FP64
-accumulation, FP64 positions and integrator) and K steps of the plain FP64
-form; ``value`` is the mixed form's atom-steps/s.  The mixed form's force
+The timed region is K steps of the mixed form (FP32 pair arithmetic on
+positions quantized to 2^-32 of the box, FP64 accumulation, FP64 positions
+and integrator) and K steps of the plain FP64 form; ``value`` is the mixed
+form's atom-steps/s.  The mixed form's force
 error against FP64 is measured on the final state.  The pair-force kernel's
 average launch duration (the roofline's ``achieved``) comes from HIP events
 on the library stream around 200 back-to-back launches of the step's
@@ -243,9 +244,10 @@ def main():
     m = res["mixed"]
     value = n * args.steps / m["secs"]
     fp64_value = n * args.steps / res["fp64"]["secs"]
-    # algorithmic bytes per atom per pair-kernel launch: int32 full-list
-    # entries, the list length, the atom's FP64 position, the FP64 force
-    bytes_per_atom = 4.0 * m["nbar"] + 4 + 24 + 24
+    # algorithmic bytes per atom per mixed pair-kernel launch: int32
+    # full-list entries, the list length, the atom's 16-byte quantized
+    # position record, the FP64 force written
+    bytes_per_atom = 4.0 * m["nbar"] + 4 + 16 + 24
     achieved = bytes_per_atom * n / (m["kms"] * 1e-3) / 1e9
     kname = "k_lj<false, true, true, true>"
     traffic = measured_traffic(kname)

@@ -47,6 +47,7 @@ static int alloc_state(sepk_state *s, long npad)
 	memset(s, 0, sizeof *s);
 	int rc = 0;
 	rc |= dalloc(&s->pos, npad);
+	rc |= dalloc(&s->qpos, npad);
 	rc |= dalloc(&s->v, 3 * npad);
 	rc |= dalloc(&s->f, 3 * npad);
 	rc |= dalloc(&s->a, 3 * npad);
@@ -62,7 +63,7 @@ static int alloc_state(sepk_state *s, long npad)
 
 static void free_state(sepk_state *s)
 {
-	void *ptrs[] = {s->pos, s->v, s->f, s->a, s->xn, s->cn, s->cr,
+	void *ptrs[] = {s->pos, s->qpos, s->v, s->f, s->a, s->xn, s->cn, s->cr,
 			s->m, s->type, s->mol, s->orig, s->partners};
 	for (void *p : ptrs)
 		if (p)
@@ -82,6 +83,7 @@ __global__ void k_init_identity(sepk_state s, int *iperm, long n, long npad)
 	// first of them, and a NaN position fails every cutoff test
 	const double nan = __builtin_nan("");
 	s.pos[t] = t < n ? sepk_pos{0.0, 0.0, 0.0, 0.0} : sepk_pos{nan, nan, nan, 0.0};
+	s.qpos[t] = make_int4(0, 0, 0, t < n ? 0 : 1);
 	s.m[t] = 1.0;
 	s.type[t] = t < n ? 'A' : 0;
 	s.mol[t] = -1;
@@ -106,6 +108,7 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 	d->n = npart;
 	// at least one slot past the last atom: the phantom that pads lists
 	d->npad = (npart + 1 + 63) / 64 * 64;
+	d->qpos_stale = 1;
 	int rc = 0;
 	if (hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking) != hipSuccess) {
 		*err = sepk_fail(SEP_GPU_EDEVICE, "hipStreamCreate failed");
@@ -335,6 +338,7 @@ extern "C" int sepdev_put_vec(sepdev *d, int which, const double *src3n)
 	const unsigned nb = sepk_blocks(d->n);
 	if (which == SEPDEV_X) {
 		k_scatter_pos<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0].pos, (const double *)d->dstage, d->iperm, d->n);
+		d->qpos_stale = 1; // requantized by the next mixed pair call
 	} else {
 		double *dst = vec_field(d, which);
 		if (!dst)

@@ -52,6 +52,7 @@ extern "C" int sepdev_zero_forces(sepdev *d)
 struct sepk_lj {
 	double cf2, A, nB, C, nD, sig2, ushift;
 	double L0, L1, L2, iL0, iL1, iL2;
+	double qd0, qd1, qd2; // L / 2^32: one step of the quantized positions
 	float Af, nBf, Cf, nDf, sig2f, ushiftf;
 	signed char t0, t1;
 };
@@ -187,6 +188,73 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 		ti = s.type[t];
 		act = ti == c.t0 || ti == c.t1;
 	}
+	if (MIXED) {
+		// positions quantized to the box (sepk_quant): the integer
+		// difference is the minimum image, one 16-byte gather per entry;
+		// the stress identity uses the quantized positions x~ = u L / 2^32
+		// and the image count k of the pair, nonzero when the unwrapped
+		// difference borrows differently from the wrapped one
+		int4 qi = make_int4(0, 0, 0, 1);
+		if (act) {
+			qi = s.qpos[t];
+			const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
+			const size_t stride = (size_t)npad * SEPK_SUB;
+			const int *lp = nbr + (size_t)t * SEPK_SUB + sub;
+			for (int b = 0; b < nbat; b++, lp += SEPK_BATCH * stride) {
+				int j[SEPK_BATCH];
+#pragma unroll
+				for (int u = 0; u < SEPK_BATCH; u++)
+					j[u] = lp[u * stride];
+				int4 qj[SEPK_BATCH];
+				signed char tj[SEPK_BATCH];
+#pragma unroll
+				for (int u = 0; u < SEPK_BATCH; u++) {
+					qj[u] = s.qpos[(unsigned)j[u]];
+					if (TYPED)
+						tj[u] = s.type[(unsigned)j[u]];
+				}
+#pragma unroll
+				for (int u = 0; u < SEPK_BATCH; u++) {
+					if (TYPED && !((ti == c.t0 && tj[u] == c.t1) ||
+						       (ti == c.t1 && tj[u] == c.t0)))
+						continue;
+					if (qj[u].w)
+						continue; // the phantom
+					const unsigned ux = (unsigned)qi.x, vx = (unsigned)qj[u].x;
+					const unsigned uy = (unsigned)qi.y, vy = (unsigned)qj[u].y;
+					const unsigned uz = (unsigned)qi.z, vz = (unsigned)qj[u].z;
+					const int ix = (int)(ux - vx), iy = (int)(uy - vy),
+						  iz = (int)(uz - vz);
+					const double dx = (double)ix * c.qd0;
+					const double dy = (double)iy * c.qd1;
+					const double dz = (double)iz * c.qd2;
+					const double r2 = dx * dx + dy * dy + dz * dz;
+					if (!(r2 < c.cf2))
+						continue;
+					const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
+					const float q3 = q * q * q;
+					const double ft = (double)(fmaf(c.Af, q3, c.nBf) * q3 * q);
+					const double e = (double)fmaf(fmaf(c.Cf, q3, c.nDf), q3, c.ushiftf);
+					const bool bx = (ux < vx) != (ix < 0), by = (uy < vy) != (iy < 0),
+						   bz = (uz < vz) != (iz < 0);
+					// k = +1 when the unwrapped difference is >= 2^31
+					// steps (wrapped negative), -1 when it is below
+					// -2^31 (wrapped non-negative)
+					const double kx = bx ? (ix < 0 ? 1.0 : -1.0) : 0.0;
+					const double ky = by ? (iy < 0 ? 1.0 : -1.0) : 0.0;
+					const double kz = bz ? (iz < 0 ? 1.0 : -1.0) : 0.0;
+					sepk_pair_accumulate(ft, e, dx, dy, dz, kx, ky, kz, c.L0, c.L1,
+							     c.L2, fx, fy, fz, acc);
+				}
+			}
+		}
+		const double xq = act ? (double)(unsigned)qi.x * c.qd0 : 0.0;
+		const double yq = act ? (double)(unsigned)qi.y * c.qd1 : 0.0;
+		const double zq = act ? (double)(unsigned)qi.z * c.qd2 : 0.0;
+		sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, xq, yq, zq, acc,
+					      s.f, partial, lb, rows);
+		return;
+	}
 	sepk_pos pi = {0.0, 0.0, 0.0, 0.0};
 	if (act) {
 		pi = s.pos[t];
@@ -223,18 +291,10 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 				const double r2 = dx * dx + dy * dy + dz * dz;
 				if (!(r2 < c.cf2))
 					continue;
-				double ft, e;
-				if (MIXED) {
-					const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
-					const float q3 = q * q * q;
-					ft = (double)(fmaf(c.Af, q3, c.nBf) * q3 * q);
-					e = (double)fmaf(fmaf(c.Cf, q3, c.nDf), q3, c.ushiftf);
-				} else {
-					const double q = c.sig2 * sepk_recip(r2);
-					const double q3 = q * q * q;
-					ft = fma(c.A, q3, c.nB) * q3 * q;
-					e = fma(fma(c.C, q3, c.nD), q3, c.ushift);
-				}
+				const double q = c.sig2 * sepk_recip(r2);
+				const double q3 = q * q * q;
+				const double ft = fma(c.A, q3, c.nB) * q3 * q;
+				const double e = fma(fma(c.C, q3, c.nD), q3, c.ushift);
 				sepk_pair_accumulate(ft, e, dx, dy, dz, kx, ky, kz, c.L0, c.L1, c.L2,
 						     fx, fy, fz, acc);
 			}
@@ -246,6 +306,28 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 
 typedef void (*sepk_lj_kernel)(sepk_state, const int *, const int *, long, long,
 			       sepk_lj, double *, long);
+
+// qpos from pos for every slot (phantoms get w = 1), after a position upload
+// or a change of box.
+__global__ void k_quantize(sepk_state s, long npad, double q0, double q1, double q2)
+{
+	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < npad)
+		s.qpos[t] = sepk_quant(s.pos[t], q0, q1, q2);
+}
+
+static int sepk_fresh_qpos(sepdev *d, const double *L)
+{
+	if (!d->qpos_stale && d->qL[0] == L[0] && d->qL[1] == L[1] && d->qL[2] == L[2])
+		return 0;
+	k_quantize<<<sepk_blocks(d->npad), SEPK_BLOCK, 0, d->st>>>(
+		d->s[0], d->npad, SEPK_QSCALE / L[0], SEPK_QSCALE / L[1], SEPK_QSCALE / L[2]);
+	SEPK_LAUNCH_CHECK();
+	for (int k = 0; k < 3; k++)
+		d->qL[k] = L[k];
+	d->qpos_stale = 0;
+	return 0;
+}
 
 template <bool T, bool M, bool O>
 static sepk_lj_kernel pick_lj3(bool fast)
@@ -295,6 +377,9 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 	c.iL0 = 1.0 / a->L[0];
 	c.iL1 = 1.0 / a->L[1];
 	c.iL2 = 1.0 / a->L[2];
+	c.qd0 = a->L[0] / SEPK_QSCALE;
+	c.qd1 = a->L[1] / SEPK_QSCALE;
+	c.qd2 = a->L[2] / SEPK_QSCALE;
 	c.Af = (float)c.A;
 	c.nBf = (float)c.nB;
 	c.Cf = (float)c.C;
@@ -304,7 +389,7 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 	c.t0 = a->t0;
 	c.t1 = a->t1;
 	int rc = sepk_flush_nh(d);
-	if (rc)
+	if (rc || (a->mixed && (rc = sepk_fresh_qpos(d, a->L))))
 		return rc;
 	const bool over = d->f_zero_pending != 0;
 	const bool fast = fast_image_ok(d, a->L);
@@ -565,7 +650,7 @@ extern "C" int sepdev_nosehoover(sepdev *d, double Td, double tau, double dt,
 __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 	sepk_state s, long n, long npad, double dt, double L0, double L1, double L2,
 	const sepdev_sys *sys, int nh, const double *alpha, double Td, double dt_tau2,
-	double *partial, long rows)
+	double *partial, long rows, double qs0, double qs1, double qs2)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -622,6 +707,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 		p.y = x[1];
 		p.z = x[2];
 		s.pos[t] = p;
+		s.qpos[t] = sepk_quant(p, qs0, qs1, qs2);
 		acc[1] = vh[0] * vh[0] * m;
 		acc[2] = vh[0] * vh[1] * m;
 		acc[3] = vh[0] * vh[2] * m;
@@ -649,8 +735,13 @@ extern "C" int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *
 	double *alpha = nh ? d->scal + d->nh_slot : d->scal;
 	k_leapfrog<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad, dt, L[0], L[1], L[2],
 						  d->sys, nh, alpha, d->nh_Td, d->nh_dt_tau2,
-						  sepk_region(d, SEPK_R_LF), d->region_rows);
+						  sepk_region(d, SEPK_R_LF), d->region_rows,
+						  SEPK_QSCALE / L[0], SEPK_QSCALE / L[1],
+						  SEPK_QSCALE / L[2]);
 	SEPK_LAUNCH_CHECK();
+	for (int k = 0; k < 3; k++)
+		d->qL[k] = L[k];
+	d->qpos_stale = 0;
 	sepk_task f;
 	memset(&f, 0, sizeof f);
 	f.mode = SEPK_FIN_LF;

@@ -37,6 +37,9 @@ struct __attribute__((aligned(32))) sepk_pos {
 // z block) so that per-atom streaming is coalesced.
 struct sepk_state {
 	sepk_pos *pos;
+	// the mixed-form pair kernels' copy of pos: each coordinate mod L as an
+	// unsigned 32-bit fraction of the box (sepk_quant), w = 1 on phantoms
+	int4 *qpos;
 	double *v, *f, *a, *xn;
 	int *cn, *cr;
 	double *m;
@@ -159,6 +162,11 @@ struct sepdev {
 	// partial rows and sums (allocated on first use)
 	double *adj_g, *adj_x, *adj_out, *adj_part;
 
+	// qpos holds pos quantized with box qL, unless qpos_stale (set by
+	// position uploads; the leapfrog and the rebuild keep qpos current)
+	int qpos_stale;
+	double qL[3];
+
 	// graph
 	hipGraph_t graph;
 	hipGraphExec_t exec;
@@ -225,6 +233,23 @@ __device__ __forceinline__ double sepk_wrap(double d, double L)
 __device__ __forceinline__ double sepk_wrap_fast(double d, double L, double invL)
 {
 	return fma(-L, rint(d * invL), d);
+}
+
+// A position as the mixed pair kernels read it: x mod L as the unsigned
+// 32-bit fraction rint(x 2^32 / L) mod 2^32 (resolution L / 2^32, 1.6e-8 at
+// L = 67), so that (int)(qi - qj) (L / 2^32) is the minimum image of
+// xi - xj to within that resolution, with no wrap arithmetic; one 16-byte
+// record per neighbour gather instead of the 32-byte FP64 one.  The phantom
+// (NaN) gets w = 1.
+#define SEPK_QSCALE 4294967296.0 // 2^32
+
+__device__ __forceinline__ int4 sepk_quant(const sepk_pos &p, double s0, double s1, double s2)
+{
+	if (!(p.x == p.x))
+		return make_int4(0, 0, 0, 1);
+	return make_int4((int)(unsigned)(long long)rint(p.x * s0),
+			 (int)(unsigned)(long long)rint(p.y * s1),
+			 (int)(unsigned)(long long)rint(p.z * s2), 0);
 }
 
 // Per-block reduction of NV values per thread into column-major partial

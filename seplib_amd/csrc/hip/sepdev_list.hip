@@ -231,6 +231,7 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 			const long t = s + rank;
 			const sepk_pos p = src.pos[o];
 			dst.pos[t] = p;
+			dst.qpos[t] = src.qpos[o];
 			const double xyz[3] = {p.x, p.y, p.z};
 			for (int k = 0; k < 3; k++) {
 				dst.v[k * npad + t] = src.v[k * npad + o];
@@ -273,6 +274,7 @@ __device__ __forceinline__ void sepk_copy_home(const sepk_state &src, const sepk
 					       long t, long npad, int has_partners)
 {
 	dst.pos[t] = src.pos[t];
+	dst.qpos[t] = src.qpos[t];
 	for (int k = 0; k < 3; k++) {
 		dst.v[k * npad + t] = src.v[k * npad + t];
 		dst.f[k * npad + t] = src.f[k * npad + t];
