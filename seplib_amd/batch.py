@@ -183,6 +183,8 @@ def main(argv=None):
     ap.add_argument("--single-steps", type=int, default=500,
                     help="steps per replica of the one-by-one comparison")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the one-by-one comparison")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -215,15 +217,18 @@ def main(argv=None):
     b.close()
     for s in systems:
         s.close()
-    t_single, as_single = one_by_one(api, reps, args.single_steps)
-    single = as_single / t_single
-    log("one by one: %.4g atom-steps/s" % single)
+    single = None
+    if not args.no_single:
+        t_single, as_single = one_by_one(api, reps, args.single_steps)
+        single = as_single / t_single
+        log("one by one: %.4g atom-steps/s" % single)
     print(json.dumps({
         "workload": "%d LJ NVT replicas, N log-uniform 500..32000 (total %d), rho U[0.70,0.95], "
                     "T U[0.8,2.0], rc 2.5, skin 0.3, dt 0.004, NH tau 0.2, FP32 pair arithmetic "
                     "+ FP64 accumulation" % (len(reps), natoms),
         "steps": args.steps, "atom_steps_per_s": value, "ms_per_step": 1e3 * secs / args.steps,
-        "one_by_one_atom_steps_per_s": single, "speedup_vs_one_by_one": value / single,
+        "one_by_one_atom_steps_per_s": single,
+        "speedup_vs_one_by_one": value / single if single else None,
         "grid": grid, "lanes": lanes,
         "max_epot_rel_vs_fp64": max(a["epot"] for a in agree),
         "max_force_err_vs_fp64": max(a["force"] for a in agree),
