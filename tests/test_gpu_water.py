@@ -1,0 +1,128 @@
+"""GPU parity on the full water system: 32000 flexible 3-site molecules
+(96000 atoms) at 3.16 atoms per unit volume; LJ O-O (eps = sigma = 1,
+rc 2.5) with same-molecule exclusion, charges O -0.8476 / H +0.4238 with
+shifted-force Coulomb at rc 2.9 on the pair call's list, harmonic O-H bonds
+(k 68000, l0 0.316) and the harmonic H-O-H angle (k 490, theta0 1.91);
+reference default skin 0.25, 9^3 cells.
+
+  * per term (LJ, bond, angle, Coulomb): FP64 energies and stress tensors
+    against the C oracle to 1e-12; forces max|dF_i| / RMS|F| to 1e-12 for
+    the total and for each term up to the angle (the oracle adds each
+    Coulomb pair to the atom's accumulated total force, |F| ~ 300 here, so
+    its per-term Coulomb difference carries that total's rounding; the
+    total force is compared instead);
+  * five resident NVT steps (Nose-Hoover T 3.86, dt 0.0005) of the whole
+    step against the oracle's synchronised loop: energies to 1e-10;
+  * the mixed form's total force and energy against FP64 to 1e-5.
+"""
+import numpy as np
+import pytest
+
+from seplib_amd import sep as S
+from seplib_amd import workloads as W
+from seplib_amd.sim import SepSystem
+from tests.golden import runs as R
+from tests.parity import FP64_TOL, force_error, tensor_error
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(lbond=0.316, kspring=68000.0, angle=1.91, kangle=490.0, cf=2.9)
+NNEIGHB = 600  # host half-list room per atom (about 210 are used)
+TEMP, TAU, DT = 3.86, 0.2, 0.0005
+
+
+def _calls():
+    p = CFG
+    return [
+        lambda s: s.force_pairs("OO", 2.5, "sep_lj_shift", S.SEP_EXCL_SAME_MOL),
+        lambda s: s.stretch_harmonic(0, p["lbond"], p["kspring"]),
+        lambda s: s.angle_harmonic(0, p["angle"], p["kangle"]),
+        lambda s: s.coulomb_sf(p["cf"], S.SEP_EXCL_SAME_MOL),
+    ]
+
+
+@pytest.fixture(scope="module")
+def water():
+    w = W.water(32000, 3.16, TEMP, seed=5, l0=CFG["lbond"], theta0=CFG["angle"])
+    # off the ideal geometry, so that bonds and angles carry force
+    n = len(w["m"])
+    w["x"] = np.mod(w["x"] + (W.uniform(41, 3 * n).reshape(n, 3) - 0.5) * 0.01, w["L"])
+    return w
+
+
+def _system(api, w):
+    return SepSystem.from_workload(api, w, cf=CFG["cf"], dt=DT, nneighb=NNEIGHB)
+
+
+def _terms(api, w, precision=None):
+    s = _system(api, w)
+    try:
+        if precision is not None:
+            s.gpu_precision(precision)
+        s.reset_retval()
+        s.reset_force()
+        forces, rets = [], []
+        prev = np.zeros((s.n, 3))
+        for c in _calls():
+            c(s)
+            f = np.array(s.view["f"])
+            forces.append(f - prev)
+            prev = f
+            rets.append(R.ret_vec(s))
+        return np.array(forces), np.array(rets), s.sys.nsubbox[0]
+    finally:
+        s.close()
+
+
+def test_water_96k_terms_match_oracle(product, oracle, water):
+    f, r, ncell = _terms(product, water)
+    fo, ro, _ = _terms(oracle, water)
+    assert ncell == 9
+    tot, toto = f.sum(axis=0), fo.sum(axis=0)
+    assert force_error(tot, toto) <= FP64_TOL, force_error(tot, toto)
+    for k, name in enumerate(("lj_OO", "bond", "angle", "coulomb")):
+        if name != "coulomb":
+            assert force_error(f[k], fo[k]) <= FP64_TOL, (name, force_error(f[k], fo[k]))
+        # epot, ekin, ecoul after each term
+        for i in (0, 2):
+            assert abs(r[k][i] - ro[k][i]) <= FP64_TOL * abs(ro[k][i]), (name, i)
+        # pot_P (the bonded terms add to it only where the reference does)
+        if np.abs(ro[k][3:12]).max() > 0:
+            assert tensor_error(r[k][3:12], ro[k][3:12]) <= FP64_TOL, name
+
+
+def _steps(api, w, resident, nsteps=5):
+    s = _system(api, w)
+    try:
+        if resident:
+            s.gpu_resident(True)
+        for _ in range(nsteps):
+            s.reset_retval()
+            s.reset_force()
+            for c in _calls():
+                c(s)
+            s.nosehoover(TEMP, TAU)
+            s.leapfrog()
+        if resident:
+            s.gpu_sync()
+        return s.ret.epot, s.ret.ekin, s.ret.ecoul, s.alpha.value, np.array(s.view["x"])
+    finally:
+        s.close()
+
+
+def test_water_96k_resident_steps_match_oracle(product, oracle, water):
+    e, k, ec, al, x = _steps(product, water, resident=True)
+    eo, ko, eco, alo, xo = _steps(oracle, water, resident=False)
+    assert abs(e - eo) <= 1e-10 * abs(eo)
+    assert abs(k - ko) <= 1e-10 * abs(ko)
+    assert abs(ec - eco) <= 1e-10 * abs(eco)
+    assert abs(al - alo) <= 1e-10 * max(1.0, abs(alo))
+    assert np.abs(x - xo).max() <= 1e-10 * water["L"][0]
+
+
+def test_water_96k_mixed_against_fp64(product, water):
+    f64, r64, _ = _terms(product, water, S.SEP_GPU_FP64)
+    f32, r32, _ = _terms(product, water, S.SEP_GPU_MIXED)
+    tot64, tot32 = f64.sum(axis=0), f32.sum(axis=0)
+    assert force_error(tot32, tot64) <= 1e-5, force_error(tot32, tot64)
+    assert abs(r32[-1][0] - r64[-1][0]) <= 1e-5 * abs(r64[-1][0])
