@@ -17,10 +17,12 @@ The timed region is K steps of the mixed form (FP32 pair arithmetic on
 positions quantized to 2^-32 of the box, FP64 accumulation, FP64 positions
 and integrator) and K steps of the plain FP64 form; ``value`` is the mixed
 form's atom-steps/s.  The mixed form's force
-error against FP64 is measured on the final state.  The pair-force kernel's
-average launch duration (the roofline's ``achieved``) comes from HIP events
-on the library stream around 200 back-to-back launches of the step's
-pair-force kernel on the final state of the timed run.
+error against FP64 is measured on the final state.  The step's dominant
+kernel -- pair forces, the Nose-Hoover term and the leapfrog fused in one
+launch -- gives the roofline: its average launch duration (``achieved``)
+comes from HIP events on the library stream around 200 back-to-back
+launches on the final state of the timed run, with the integrator's outputs
+sent to scratch so that every launch sees the same state.
 ``traffic`` is the HBM bytes per launch of that kernel from the rocprofv3
 FETCH_SIZE / WRITE_SIZE counter summary committed under profiles/, scaled by
 the counters' reading for a plain device copy of known size.
@@ -88,7 +90,7 @@ def run_gpu(api, w, precision, steps, warmup, time_kernel=True):
     if time_kernel:
         import ctypes as C
         # HIP events on the library stream around KERNEL_REPS back-to-back
-        # launches of the step's pair-force kernel on the final state
+        # launches of the step's fused kernel on the final state
         ms = C.c_double(0)
         s.check(lib.sep_gpu_time_pair_kernel(s.atoms, KERNEL_REPS, C.byref(ms)))
         kern_ms = ms.value
@@ -244,17 +246,23 @@ def main():
     m = res["mixed"]
     value = n * args.steps / m["secs"]
     fp64_value = n * args.steps / res["fp64"]["secs"]
-    # algorithmic bytes per atom per mixed pair-kernel launch: int32
-    # full-list entries, the list length, the atom's 16-byte quantized
-    # position record, the FP64 force written
-    bytes_per_atom = 4.0 * m["nbar"] + 4 + 16 + 24
+    # The step's dominant kernel is the fused pair + Nose-Hoover + leapfrog
+    # kernel.  Algorithmic bytes per atom per launch: int32 full-list
+    # entries and the list length (4 nbar + 4); the pair part's own 16-byte
+    # quantized record; the integrator's reads of x (32), v (24), m (8),
+    # xn (24), cross_neighb and crossings (12 + 12); its writes of f, a, v
+    # (3 x 24), the new position and its quantized record (32 + 16),
+    # cross_neighb and crossings (12 + 12).
+    bytes_per_atom = 4.0 * m["nbar"] + 4 + 16 + (32 + 24 + 8 + 24 + 12 + 12) + \
+        (3 * 24 + 32 + 16 + 12 + 12)
     achieved = bytes_per_atom * n / (m["kms"] * 1e-3) / 1e9
-    kname = "k_lj<false, true, true, true>"
+    kname = "k_lj_step<true, true>"
     traffic = measured_traffic(kname)
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=None if traffic is None else round(traffic),
-                    kernel=kname + " (mixed pair forces)", kernel_us=round(m["kms"] * 1e3, 3),
+                    kernel=kname + " (mixed pair forces, Nose-Hoover and leapfrog)",
+                    kernel_us=round(m["kms"] * 1e3, 3),
                     algorithmic_bytes=round(bytes_per_atom * n),
                     bytes_per_atom=round(bytes_per_atom, 2))
     drift = None

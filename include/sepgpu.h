@@ -132,7 +132,10 @@ int sep_gpu_timing_get(seppart *atoms, const char *kernel, double *total_ms,
 /* Mean duration of the pair-force kernel: the last configuration that a
  * pair call launched, re-launched reps times back to back on the library
  * stream between one pair of HIP events.  Afterwards f holds the pair forces
- * of the current positions (as after sep_reset_force + that call). */
+ * of the current positions (as after sep_reset_force + that call).  When
+ * the last launch was a fused step (pair forces, Nose-Hoover and leapfrog
+ * of a sep_gpu_run step), that kernel is timed instead, with its integrator
+ * outputs sent to scratch: the state is left as it was. */
 int sep_gpu_time_pair_kernel(seppart *atoms, int reps, double *ms_per_launch);
 
 /* Mean full-list length of the current device list (entries per atom). */

@@ -97,6 +97,17 @@ __global__ void k_init_identity(sepk_state s, int *iperm, long n, long npad)
 	}
 }
 
+// The fused step's second position buffer: phantoms as in k_init_identity.
+__global__ void k_init_alt(sepk_pos *pos, int4 *qpos, long n, long npad)
+{
+	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= npad)
+		return;
+	const double nan = __builtin_nan("");
+	pos[t] = t < n ? sepk_pos{0.0, 0.0, 0.0, 0.0} : sepk_pos{nan, nan, nan, 0.0};
+	qpos[t] = make_int4(0, 0, 0, t < n ? 0 : 1);
+}
+
 extern "C" sepdev *sepdev_create(long npart, int *err)
 {
 	*err = 0;
@@ -117,6 +128,8 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 	}
 	rc |= alloc_state(&d->s[0], d->npad);
 	rc |= alloc_state(&d->s[1], d->npad);
+	rc |= dalloc(&d->pos_alt, d->npad);
+	rc |= dalloc(&d->qpos_alt, d->npad);
 	rc |= dalloc(&d->iperm, d->npad);
 	rc |= dalloc(&d->cellid_old, d->npad);
 	rc |= dalloc(&d->order_tmp, d->npad);
@@ -163,6 +176,8 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 	for (int b = 0; b < 2; b++) {
 		k_init_identity<<<sepk_blocks(d->npad), SEPK_BLOCK, 0, d->st>>>(d->s[b], d->iperm, d->n, d->npad);
 	}
+	k_init_alt<<<sepk_blocks(d->npad), SEPK_BLOCK, 0, d->st>>>(d->pos_alt, d->qpos_alt, d->n,
+								    d->npad);
 	if (hipStreamSynchronize(d->st) != hipSuccess) {
 		*err = sepk_fail(SEP_GPU_EDEVICE, "device init failed");
 		sepdev_destroy(d);
@@ -194,6 +209,9 @@ extern "C" void sepdev_destroy(sepdev *d)
 		(void)hipHostFree(d->hstage);
 	free(d->ret_zero);
 	free(d->last_lj);
+	(void)hipFree(d->pos_alt);
+	(void)hipFree(d->qpos_alt);
+	(void)hipFree(d->tscratch);
 	void *adj[] = {d->adj_g, d->adj_x, d->adj_out, d->adj_part};
 	for (void *p : adj)
 		if (p)

@@ -27,6 +27,9 @@ __global__ void k_zero_forces(double *f, long count)
 
 int sepk_materialize_forces(sepdev *d)
 {
+	int rc = sepk_flush_pair(d);
+	if (rc)
+		return rc;
 	if (!d->f_zero_pending)
 		return 0;
 	const long cnt = 3 * d->npad;
@@ -49,13 +52,6 @@ extern "C" int sepdev_zero_forces(sepdev *d)
 // ft = eps48 q^3 (q^3 - aw/2) q and u = eps4 q^3 (q^3 - aw) + ushift with
 // q = sig2 / r^2, evaluated as ft = (A q^3 + nB) q^3 q, u = (C q^3 + nD) q^3 + ushift
 // (nB = -eps48 aw/2, nD = -eps4 aw)
-struct sepk_lj {
-	double cf2, A, nB, C, nD, sig2, ushift;
-	double L0, L1, L2, iL0, iL1, iL2;
-	double qd0, qd1, qd2; // L / 2^32: one step of the quantized positions
-	float Af, nBf, Cf, nDf, sig2f, ushiftf;
-	signed char t0, t1;
-};
 
 // Shared epilogue of the list kernels: combine the SEPK_SUB lanes of each
 // atom (fixed xor order), write or add its force from lane 0, and leave the
@@ -172,22 +168,15 @@ __device__ __forceinline__ double sepk_recip(double r2)
 	return fma(y, e, y);
 }
 
-template <bool TYPED, bool MIXED, bool OVERWRITE, bool FAST>
-__global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
-	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt,
-	long n, long npad, sepk_lj c, double *partial, long rows)
+// The pair loop of atom t, lane `sub`: the lane's share of the atom's force
+// and of the block's energy and crossing stress, and the position the
+// stress identity pairs the summed force with (px, py, pz).
+template <bool TYPED, bool MIXED, bool FAST>
+__device__ __forceinline__ void sepk_lj_pairs(
+	const sepk_state &s, const int *__restrict__ nbr, const int *__restrict__ cnt, long npad,
+	const sepk_lj &c, long t, int sub, bool act, signed char ti, double &fx, double &fy,
+	double &fz, double (&acc)[7], double &px, double &py, double &pz)
 {
-	const long lb = sepk_xcd_block();
-	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
-	const int sub = threadIdx.x % SEPK_SUB;
-	double fx = 0.0, fy = 0.0, fz = 0.0;
-	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-	bool act = t < n;
-	signed char ti = 0;
-	if (act && TYPED) {
-		ti = s.type[t];
-		act = ti == c.t0 || ti == c.t1;
-	}
 	if (MIXED) {
 		// positions quantized to the box (sepk_quant): the integer
 		// difference is the minimum image, one 16-byte gather per entry;
@@ -248,11 +237,9 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 				}
 			}
 		}
-		const double xq = act ? (double)(unsigned)qi.x * c.qd0 : 0.0;
-		const double yq = act ? (double)(unsigned)qi.y * c.qd1 : 0.0;
-		const double zq = act ? (double)(unsigned)qi.z * c.qd2 : 0.0;
-		sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, xq, yq, zq, acc,
-					      s.f, partial, lb, rows);
+		px = act ? (double)(unsigned)qi.x * c.qd0 : 0.0;
+		py = act ? (double)(unsigned)qi.y * c.qd1 : 0.0;
+		pz = act ? (double)(unsigned)qi.z * c.qd2 : 0.0;
 		return;
 	}
 	sepk_pos pi = {0.0, 0.0, 0.0, 0.0};
@@ -300,8 +287,33 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 			}
 		}
 	}
-	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, pi.x, pi.y, pi.z, acc,
-				      s.f, partial, lb, rows);
+	px = pi.x;
+	py = pi.y;
+	pz = pi.z;
+}
+
+
+template <bool TYPED, bool MIXED, bool OVERWRITE, bool FAST>
+__global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
+	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt,
+	long n, long npad, sepk_lj c, double *partial, long rows)
+{
+	const long lb = sepk_xcd_block();
+	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
+	const int sub = threadIdx.x % SEPK_SUB;
+	double fx = 0.0, fy = 0.0, fz = 0.0;
+	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+	bool act = t < n;
+	signed char ti = 0;
+	if (act && TYPED) {
+		ti = s.type[t];
+		act = ti == c.t0 || ti == c.t1;
+	}
+	double px = 0.0, py = 0.0, pz = 0.0;
+	sepk_lj_pairs<TYPED, MIXED, FAST>(s, nbr, cnt, npad, c, t, sub, act, ti, fx, fy, fz, acc,
+					   px, py, pz);
+	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, px, py, pz, acc, s.f,
+				      partial, lb, rows);
 }
 
 typedef void (*sepk_lj_kernel)(sepk_state, const int *, const int *, long, long,
@@ -357,6 +369,50 @@ static bool fast_image_ok(const sepdev *d, const double *L)
 	return true;
 }
 
+// One pair-force launch and its finishing task.
+static int launch_lj(sepdev *d, const sepk_lj &c, int typed, int mixed, bool fast, int ret_slot,
+		     int epot_assign)
+{
+	// first: claiming the region may launch a recorded pair call, which
+	// consumes the pending f = 0
+	int rc;
+	if ((rc = sepk_claim_region(d, SEPK_R_PAIR)))
+		return rc;
+	const bool over = d->f_zero_pending != 0;
+	sepk_lj_kernel k;
+	if (typed)
+		k = mixed ? pick_lj<true, true>(over, fast) : pick_lj<true, false>(over, fast);
+	else
+		k = mixed ? pick_lj<false, true>(over, fast) : pick_lj<false, false>(over, fast);
+	if ((rc = sepk_timer_begin(d, &d->t_pair)))
+		return rc;
+	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
+	double *part = sepk_region(d, SEPK_R_PAIR);
+	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c, part,
+					  d->region_rows);
+	SEPK_LAUNCH_CHECK();
+	d->f_zero_pending = 0;
+	if ((rc = sepk_timer_end(d, &d->t_pair)))
+		return rc;
+	if (!d->last_lj)
+		d->last_lj = malloc(sizeof(sepk_lj));
+	memcpy(d->last_lj, &c, sizeof c);
+	d->last_lj_kernel = (const void *)(typed ? (mixed ? pick_lj<true, true>(true, fast)
+							  : pick_lj<true, false>(true, fast))
+					       : (mixed ? pick_lj<false, true>(true, fast)
+							: pick_lj<false, false>(true, fast)));
+	d->last_fused = 0;
+	sepk_task f;
+	memset(&f, 0, sizeof f);
+	f.mode = SEPK_FIN_PAIR;
+	f.region = SEPK_R_PAIR;
+	f.nblk = (int)nb;
+	f.nv = 7;
+	f.epot_assign = epot_assign;
+	sepk_task_ret(d, f, ret_slot);
+	return sepk_queue_task(d, f);
+}
+
 extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 {
 	if (!d->list_built)
@@ -393,85 +449,36 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 		return rc;
 	const bool over = d->f_zero_pending != 0;
 	const bool fast = fast_image_ok(d, a->L);
-	sepk_lj_kernel k;
-	if (a->typed)
-		k = a->mixed ? pick_lj<true, true>(over, fast) : pick_lj<true, false>(over, fast);
-	else
-		k = a->mixed ? pick_lj<false, true>(over, fast) : pick_lj<false, false>(over, fast);
-	if ((rc = sepk_claim_region(d, SEPK_R_PAIR)))
-		return rc;
-	if ((rc = sepk_timer_begin(d, &d->t_pair)))
-		return rc;
-	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
-	double *part = sepk_region(d, SEPK_R_PAIR);
-	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c, part,
-					  d->region_rows);
-	SEPK_LAUNCH_CHECK();
-	d->f_zero_pending = 0;
-	if ((rc = sepk_timer_end(d, &d->t_pair)))
-		return rc;
-	if (!d->last_lj)
-		d->last_lj = malloc(sizeof(sepk_lj));
-	memcpy(d->last_lj, &c, sizeof c);
-	d->last_lj_kernel = (const void *)(a->typed ? (a->mixed ? pick_lj<true, true>(true, fast)
-							       : pick_lj<true, false>(true, fast))
-						    : (a->mixed ? pick_lj<false, true>(true, fast)
-							       : pick_lj<false, false>(true, fast)));
-	sepk_task f;
-	memset(&f, 0, sizeof f);
-	f.mode = SEPK_FIN_PAIR;
-	f.region = SEPK_R_PAIR;
-	f.nblk = (int)nb;
-	f.nv = 7;
-	f.epot_assign = a->epot_assign;
-	sepk_task_ret(d, f, ret_slot);
-	return sepk_queue_task(d, f);
+	if (d->fuse_armed && !a->typed && over && !d->fused_pending) {
+		// recorded: the step's leapfrog launches it fused with itself
+		d->fused_pending = 1;
+		d->fused_lj = c;
+		d->fused_mixed = a->mixed;
+		d->fused_fast = fast;
+		d->fused_ret = ret_slot;
+		d->fused_epot_assign = a->epot_assign;
+		return 0;
+	}
+	return launch_lj(d, c, a->typed, a->mixed, fast, ret_slot, a->epot_assign);
 }
 
-// Launch duration of the pair-force kernel: the last configuration launched
-// by sepdev_lj, in its overwriting form, `reps` times back to back on the
-// library stream between one pair of HIP events.  Leaves f holding the pair
-// forces of the current positions (what sep_reset_force + that call give)
-// and queues no reduction.
-extern "C" int sepdev_time_pair(sepdev *d, int reps, double *ms_per_launch)
+// Launches a recorded pair call on its own (something other than the
+// step's leapfrog needs its forces or its rows).
+int sepk_flush_pair(sepdev *d)
 {
-	if (!d->last_lj || !d->list_built)
-		return sepk_fail(SEP_GPU_ESTATE, "no pair-force launch to time");
-	if (reps < 1 || d->capturing)
-		return sepk_fail(SEP_GPU_EARG, "time_pair: reps %d%s", reps,
-				 d->capturing ? " during capture" : "");
-	int rc = sepk_flush_nh(d);
-	if (rc || (rc = sepk_flush_tasks(d)))
-		return rc;
-	sepk_lj c;
-	memcpy(&c, d->last_lj, sizeof c);
-	const sepk_lj_kernel k = (sepk_lj_kernel)d->last_lj_kernel;
-	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
-	double *part = sepk_region(d, SEPK_R_PAIR);
-	hipEvent_t e0, e1;
-	SEPK_CHECK(hipEventCreate(&e0));
-	SEPK_CHECK(hipEventCreate(&e1));
-	hipError_t err = hipEventRecord(e0, d->st);
-	for (int r = 0; r < reps && err == hipSuccess; r++) {
-		k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c,
-						  part, d->region_rows);
-		err = hipGetLastError();
-	}
-	if (err == hipSuccess)
-		err = hipEventRecord(e1, d->st);
-	if (err == hipSuccess)
-		err = hipEventSynchronize(e1);
-	float ms = 0.0f;
-	if (err == hipSuccess)
-		err = hipEventElapsedTime(&ms, e0, e1);
-	(void)hipEventDestroy(e0);
-	(void)hipEventDestroy(e1);
-	if (err != hipSuccess)
-		return sepk_fail(SEP_GPU_EDEVICE, "time_pair: %s", hipGetErrorString(err));
-	d->f_zero_pending = 0;
-	*ms_per_launch = ms / reps;
-	return 0;
+	if (!d->fused_pending)
+		return 0;
+	d->fused_pending = 0;
+	return launch_lj(d, d->fused_lj, 0, d->fused_mixed, d->fused_fast, d->fused_ret,
+			 d->fused_epot_assign);
 }
+
+extern "C" int sepdev_fuse_step(sepdev *d, int on)
+{
+	d->fuse_armed = on != 0;
+	return on ? 0 : sepk_flush_pair(d);
+}
+
 
 // ---- shifted-force Coulomb (source/sepcoulomb.c:96-160) ---------------------
 
@@ -647,101 +654,233 @@ extern "C" int sepdev_nosehoover(sepdev *d, double Td, double tau, double dt,
 // x += v dt, wrap once per axis counting crossings, squared displacement
 // from xn.  Row: {ekin sum, kin_P (6), max d2, sum m v_new^2}.
 
+// Output arrays of the per-atom integrator step: the state's own arrays,
+// except the positions of a fused step (the spare buffer, while other atoms
+// still gather the old ones) and a timing run's scratch.
+struct sepk_lfout {
+	double *v, *f, *a, *xn;
+	int *cn, *cr;
+	sepk_pos *pos;
+	int4 *qpos;
+};
+
+// sep_nosehoover's force term and sep_leapfrog for atom t
+// (source/sepintgr.c:149-168, :42-88) from its force F before the thermostat
+// term and its position p.  acc: m vh^2 summed, the six m vh (x) vh
+// components, d^2 and m v^2.
+__device__ __forceinline__ void sepk_lf_atom(const sepk_state &s, const sepk_lfout &o, long t,
+					     long npad, const double (&F)[3], sepk_pos p, double dt,
+					     double L0, double L1, double L2, int nh, double al,
+					     int pending, double qs0, double qs1, double qs2,
+					     double (&acc)[9])
+{
+	const double L[3] = {L0, L1, L2};
+	double x[3] = {p.x, p.y, p.z};
+	const double m = s.m[t];
+	double vh[3];
+	double d2 = 0.0;
+	for (int k = 0; k < 3; k++) {
+		const long i = k * npad + t;
+		double xn = s.xn[i];
+		int cn = s.cn[i];
+		if (pending) {
+			xn = x[k];
+			cn = 0;
+			o.xn[i] = xn;
+		}
+		double v = s.v[i];
+		double f = F[k];
+		if (nh)
+			f -= al * m * v;
+		const double a = f / m;
+		v += a * dt;
+		x[k] += v * dt;
+		vh[k] = v - 0.5 * a * dt;
+		acc[0] += vh[k] * vh[k] * m;
+		acc[8] += v * v * m;
+		int cr = s.cr[i];
+		if (x[k] > L[k]) {
+			x[k] -= L[k];
+			cn++;
+			cr++;
+		} else if (x[k] < 0.0) {
+			x[k] += L[k];
+			cn--;
+			cr--;
+		}
+		const double dr = (x[k] + cn * L[k]) - xn;
+		d2 += dr * dr;
+		o.f[i] = f;
+		o.a[i] = a;
+		o.v[i] = v;
+		o.cn[i] = cn;
+		o.cr[i] = cr;
+	}
+	p.x = x[0];
+	p.y = x[1];
+	p.z = x[2];
+	o.pos[t] = p;
+	o.qpos[t] = sepk_quant(p, qs0, qs1, qs2);
+	acc[1] = vh[0] * vh[0] * m;
+	acc[2] = vh[0] * vh[1] * m;
+	acc[3] = vh[0] * vh[2] * m;
+	acc[4] = vh[1] * vh[1] * m;
+	acc[5] = vh[1] * vh[2] * m;
+	acc[6] = vh[2] * vh[2] * m;
+	acc[7] = d2;
+}
+
+// sepk_lf_atom spread over lanes 0..2 of the atom's SEPK_SUB lanes, one
+// axis each; lane 0 gathers the other axes' values by shuffles and forms
+// the sums in sepk_lf_atom's order (the same bits).  Every lane of the wave
+// calls it.
+__device__ __forceinline__ void sepk_lf_atom_lanes(const sepk_state &s, const sepk_lfout &o,
+						   long t, long npad, int sub, bool act, double F0,
+						   double F1, double F2, double dt, double L0,
+						   double L1, double L2, int nh, double al,
+						   int pending, double qs0, double qs1, double qs2,
+						   double (&acc)[9])
+{
+	double x = 0.0, v = 0.0, vh = 0.0, dr = 0.0, m = 0.0, q = 0.0;
+	if (act && sub < 3) {
+		const int k = sub;
+		const sepk_pos p = s.pos[t];
+		q = p.q;
+		x = k == 0 ? p.x : (k == 1 ? p.y : p.z);
+		const double L = k == 0 ? L0 : (k == 1 ? L1 : L2);
+		const double Fk = k == 0 ? F0 : (k == 1 ? F1 : F2);
+		m = s.m[t];
+		const long i = k * npad + t;
+		double xn = s.xn[i];
+		int cn = s.cn[i];
+		if (pending) {
+			xn = x;
+			cn = 0;
+			o.xn[i] = xn;
+		}
+		v = s.v[i];
+		double f = Fk;
+		if (nh)
+			f -= al * m * v;
+		const double a = f / m;
+		v += a * dt;
+		x += v * dt;
+		vh = v - 0.5 * a * dt;
+		int cr = s.cr[i];
+		if (x > L) {
+			x -= L;
+			cn++;
+			cr++;
+		} else if (x < 0.0) {
+			x += L;
+			cn--;
+			cr--;
+		}
+		dr = (x + cn * L) - xn;
+		o.f[i] = f;
+		o.a[i] = a;
+		o.v[i] = v;
+		o.cn[i] = cn;
+		o.cr[i] = cr;
+	}
+	const int base = (int)(threadIdx.x & (SEPK_WAVE - 1)) - sub;
+	const double x1 = __shfl(x, base + 1, SEPK_WAVE), x2 = __shfl(x, base + 2, SEPK_WAVE);
+	const double v1 = __shfl(v, base + 1, SEPK_WAVE), v2 = __shfl(v, base + 2, SEPK_WAVE);
+	const double h1 = __shfl(vh, base + 1, SEPK_WAVE), h2 = __shfl(vh, base + 2, SEPK_WAVE);
+	const double r1 = __shfl(dr, base + 1, SEPK_WAVE), r2 = __shfl(dr, base + 2, SEPK_WAVE);
+	if (act && sub == 0) {
+		acc[0] = (vh * vh * m + h1 * h1 * m) + h2 * h2 * m;
+		acc[8] = (v * v * m + v1 * v1 * m) + v2 * v2 * m;
+		acc[1] = vh * vh * m;
+		acc[2] = vh * h1 * m;
+		acc[3] = vh * h2 * m;
+		acc[4] = h1 * h1 * m;
+		acc[5] = h1 * h2 * m;
+		acc[6] = h2 * h2 * m;
+		acc[7] = (dr * dr + r1 * r1) + r2 * r2;
+		const sepk_pos p = {x, x1, x2, q};
+		o.pos[t] = p;
+		o.qpos[t] = sepk_quant(p, qs0, qs1, qs2);
+	}
+}
+
+// The leapfrog on its own.  Its atoms map onto blocks exactly as the pair
+// kernels' (SEPK_SUB lanes per atom, lane 0 working, XCD-remapped block
+// index), so that its rows -- and hence every reduced sum -- are the same
+// bits as those of the fused step kernel.
 __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 	sepk_state s, long n, long npad, double dt, double L0, double L1, double L2,
 	const sepdev_sys *sys, int nh, const double *alpha, double Td, double dt_tau2,
 	double *partial, long rows, double qs0, double qs1, double qs2)
 {
-	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long lb = sepk_xcd_block();
+	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
+	const int sub = threadIdx.x % SEPK_SUB;
 	double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	const int pending = sys->xn_pending;
 	double al = 0.0;
 	if (nh)
 		al = sepk_nh_alpha(*alpha, sys->sumv2, (double)n, Td, dt_tau2);
-	if (t < n) {
-		const double L[3] = {L0, L1, L2};
-		sepk_pos p = s.pos[t];
-		double x[3] = {p.x, p.y, p.z};
-		const double m = s.m[t];
-		double vh[3];
-		double d2 = 0.0;
-		for (int k = 0; k < 3; k++) {
-			const long i = k * npad + t;
-			double xn = s.xn[i];
-			int cn = s.cn[i];
-			if (pending) {
-				xn = x[k];
-				cn = 0;
-				s.xn[i] = xn;
-			}
-			double v = s.v[i];
-			double f = s.f[i];
-			if (nh) {
-				f -= al * m * v;
-				s.f[i] = f;
-			}
-			const double a = f / m;
-			v += a * dt;
-			x[k] += v * dt;
-			vh[k] = v - 0.5 * a * dt;
-			acc[0] += vh[k] * vh[k] * m;
-			acc[8] += v * v * m;
-			int cr = s.cr[i];
-			if (x[k] > L[k]) {
-				x[k] -= L[k];
-				cn++;
-				cr++;
-			} else if (x[k] < 0.0) {
-				x[k] += L[k];
-				cn--;
-				cr--;
-			}
-			const double dr = (x[k] + cn * L[k]) - xn;
-			d2 += dr * dr;
-			s.a[i] = a;
-			s.v[i] = v;
-			s.cn[i] = cn;
-			s.cr[i] = cr;
-		}
-		p.x = x[0];
-		p.y = x[1];
-		p.z = x[2];
-		s.pos[t] = p;
-		s.qpos[t] = sepk_quant(p, qs0, qs1, qs2);
-		acc[1] = vh[0] * vh[0] * m;
-		acc[2] = vh[0] * vh[1] * m;
-		acc[3] = vh[0] * vh[2] * m;
-		acc[4] = vh[1] * vh[1] * m;
-		acc[5] = vh[1] * vh[2] * m;
-		acc[6] = vh[2] * vh[2] * m;
-		acc[7] = d2;
+	if (t < n && sub == 0) {
+		const double F[3] = {s.f[t], s.f[npad + t], s.f[2 * npad + t]};
+		const sepk_lfout o = {s.v, s.f, s.a, s.xn, s.cn, s.cr, s.pos, s.qpos};
+		sepk_lf_atom(s, o, t, npad, F, s.pos[t], dt, L0, L1, L2, nh, al, pending, qs0, qs1,
+			     qs2, acc);
 	}
-	sepk_block_partial<9>(acc, 1u << 7, partial, blockIdx.x, rows);
+	sepk_block_partial<9>(acc, 1u << 7, partial, lb, rows);
 }
 
-extern "C" int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *L,
-			       int ret_slot)
+// Pair forces, Nose-Hoover and leapfrog in one launch (a step whose only
+// calls are one overwriting pair call, sep_nosehoover and sep_leapfrog):
+// the pair loop and its rows as in k_lj, then the atom's integrator step
+// from its summed force, with the new positions to o.pos / o.qpos.
+template <bool MIXED, bool FAST>
+__global__ __launch_bounds__(SEPK_BLOCK) void k_lj_step(
+	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt, long n,
+	long npad, sepk_lj c, const sepdev_sys *sys, int nh, const double *alpha, double Td,
+	double dt_tau2, double dt, double qs0, double qs1, double qs2, sepk_lfout o,
+	double *ppart, double *lpart, long rows)
 {
-	if (ret_slot < 0 || ret_slot >= d->nret)
-		return sepk_fail(SEP_GPU_EARG, "ret slot %d", ret_slot);
-	int rc = sepk_materialize_forces(d);
-	if (rc)
-		return rc;
-	if ((rc = sepk_claim_region(d, SEPK_R_LF)))
-		return rc;
-	const int nh = d->nh_pending;
-	d->nh_pending = 0;
-	const unsigned nb = sepk_blocks(d->n);
-	double *alpha = nh ? d->scal + d->nh_slot : d->scal;
-	k_leapfrog<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad, dt, L[0], L[1], L[2],
-						  d->sys, nh, alpha, d->nh_Td, d->nh_dt_tau2,
-						  sepk_region(d, SEPK_R_LF), d->region_rows,
-						  SEPK_QSCALE / L[0], SEPK_QSCALE / L[1],
-						  SEPK_QSCALE / L[2]);
-	SEPK_LAUNCH_CHECK();
-	for (int k = 0; k < 3; k++)
-		d->qL[k] = L[k];
-	d->qpos_stale = 0;
+	const long lb = sepk_xcd_block();
+	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
+	const int sub = threadIdx.x % SEPK_SUB;
+	double fx = 0.0, fy = 0.0, fz = 0.0;
+	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+	const bool act = t < n;
+	double px = 0.0, py = 0.0, pz = 0.0;
+	sepk_lj_pairs<false, MIXED, FAST>(s, nbr, cnt, npad, c, t, sub, act, 0, fx, fy, fz, acc,
+					   px, py, pz);
+#pragma unroll
+	for (int m = 1; m < SEPK_SUB; m <<= 1) {
+		fx += __shfl_xor(fx, m, SEPK_WAVE);
+		fy += __shfl_xor(fy, m, SEPK_WAVE);
+		fz += __shfl_xor(fz, m, SEPK_WAVE);
+	}
+	double lf[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+	const int pending = sys->xn_pending;
+	double al = 0.0;
+	if (nh)
+		al = sepk_nh_alpha(*alpha, sys->sumv2, (double)n, Td, dt_tau2);
+	if (act && sub == 0) {
+		// the stress identity's per-atom term, as sepk_pair_epilogue adds it
+		const double gx = 2.0 * fx, gy = 2.0 * fy, gz = 2.0 * fz;
+		acc[1] = fma(gx, px, acc[1]);
+		acc[2] = fma(gx, py, acc[2]);
+		acc[3] = fma(gx, pz, acc[3]);
+		acc[4] = fma(gy, py, acc[4]);
+		acc[5] = fma(gy, pz, acc[5]);
+		acc[6] = fma(gz, pz, acc[6]);
+	}
+	sepk_lf_atom_lanes(s, o, t, npad, sub, act, fx, fy, fz, dt, c.L0, c.L1, c.L2, nh, al,
+			   pending, qs0, qs1, qs2, lf);
+	sepk_block_partial<7>(acc, 0u, ppart, lb, rows);
+	sepk_block_partial<9>(lf, 1u << 7, lpart, lb, rows);
+}
+
+// The leapfrog's finishing task (after the step's pair task, if fused).
+static int queue_lf_task(sepdev *d, int nh, double *alpha, unsigned nb, double dt, double skin,
+			 int ret_slot)
+{
 	sepk_task f;
 	memset(&f, 0, sizeof f);
 	f.mode = SEPK_FIN_LF;
@@ -761,10 +900,99 @@ extern "C" int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *
 	sepk_task_ret(d, f, ret_slot);
 	// one finisher launch for the whole step: any queued build and pair
 	// results, then the leapfrog's
+	int rc;
 	if ((rc = sepk_queue_task(d, f)) || (rc = sepk_flush_tasks(d)))
 		return rc;
 	d->sumv2_valid = 1;
 	return 0;
+}
+
+static void launch_step_kernel(sepdev *d, const sepk_lj &c, int mixed, bool fast, int nh,
+			       const double *alpha, double dt, const double *L,
+			       const sepk_lfout &o, unsigned nb)
+{
+	auto k = mixed ? (fast ? k_lj_step<true, true> : k_lj_step<true, false>)
+		       : (fast ? k_lj_step<false, true> : k_lj_step<false, false>);
+	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c, d->sys,
+					  nh, alpha, d->nh_Td, d->nh_dt_tau2, dt,
+					  SEPK_QSCALE / L[0], SEPK_QSCALE / L[1], SEPK_QSCALE / L[2], o,
+					  sepk_region(d, SEPK_R_PAIR), sepk_region(d, SEPK_R_LF),
+					  d->region_rows);
+}
+
+// The recorded pair call, the pending Nose-Hoover call and this leapfrog
+// in one launch.
+static int sepdev_fused_step(sepdev *d, double dt, double skin, const double *L, int ret_slot)
+{
+	d->fused_pending = 0;
+	int rc;
+	if ((rc = sepk_claim_region(d, SEPK_R_PAIR)) || (rc = sepk_claim_region(d, SEPK_R_LF)))
+		return rc;
+	const int nh = d->nh_pending;
+	d->nh_pending = 0;
+	double *alpha = nh ? d->scal + d->nh_slot : d->scal;
+	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
+	const sepk_state &s = d->s[0];
+	const sepk_lfout o = {s.v, s.f, s.a, s.xn, s.cn, s.cr, d->pos_alt, d->qpos_alt};
+	launch_step_kernel(d, d->fused_lj, d->fused_mixed, d->fused_fast != 0, nh, alpha, dt, L, o,
+			   nb);
+	SEPK_LAUNCH_CHECK();
+	d->f_zero_pending = 0;
+	// the new positions are current from here on
+	sepk_pos *p = d->s[0].pos;
+	d->s[0].pos = d->pos_alt;
+	d->pos_alt = p;
+	int4 *q = d->s[0].qpos;
+	d->s[0].qpos = d->qpos_alt;
+	d->qpos_alt = q;
+	for (int k = 0; k < 3; k++) {
+		d->qL[k] = L[k];
+		d->fused_L[k] = L[k];
+	}
+	d->qpos_stale = 0;
+	d->last_fused = 1;
+	d->fused_nh = nh;
+	d->fused_alpha = alpha;
+	d->fused_dt = dt;
+	sepk_task f;
+	memset(&f, 0, sizeof f);
+	f.mode = SEPK_FIN_PAIR;
+	f.region = SEPK_R_PAIR;
+	f.nblk = (int)nb;
+	f.nv = 7;
+	f.epot_assign = d->fused_epot_assign;
+	sepk_task_ret(d, f, d->fused_ret);
+	if ((rc = sepk_queue_task(d, f)))
+		return rc;
+	return queue_lf_task(d, nh, alpha, nb, dt, skin, ret_slot);
+}
+
+extern "C" int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *L,
+			       int ret_slot)
+{
+	if (ret_slot < 0 || ret_slot >= d->nret)
+		return sepk_fail(SEP_GPU_EARG, "ret slot %d", ret_slot);
+	if (d->fused_pending)
+		return sepdev_fused_step(d, dt, skin, L, ret_slot);
+	int rc = sepk_materialize_forces(d);
+	if (rc)
+		return rc;
+	if ((rc = sepk_claim_region(d, SEPK_R_LF)))
+		return rc;
+	const int nh = d->nh_pending;
+	d->nh_pending = 0;
+	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
+	double *alpha = nh ? d->scal + d->nh_slot : d->scal;
+	k_leapfrog<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad, dt, L[0], L[1], L[2],
+						  d->sys, nh, alpha, d->nh_Td, d->nh_dt_tau2,
+						  sepk_region(d, SEPK_R_LF), d->region_rows,
+						  SEPK_QSCALE / L[0], SEPK_QSCALE / L[1],
+						  SEPK_QSCALE / L[2]);
+	SEPK_LAUNCH_CHECK();
+	for (int k = 0; k < 3; k++)
+		d->qL[k] = L[k];
+	d->qpos_stale = 0;
+	return queue_lf_task(d, nh, alpha, nb, dt, skin, ret_slot);
 }
 
 // ---- list extraction ------------------------------------------------------------
@@ -832,5 +1060,67 @@ extern "C" int sepdev_list_stats(sepdev *d, double *mean_full, int *max_full)
 	free(h);
 	*mean_full = s / (double)d->n;
 	*max_full = mx;
+	return 0;
+}
+
+extern "C" int sepdev_time_pair(sepdev *d, int reps, double *ms_per_launch)
+{
+	if ((!d->last_lj && !d->last_fused) || !d->list_built)
+		return sepk_fail(SEP_GPU_ESTATE, "no pair-force launch to time");
+	if (reps < 1 || d->capturing)
+		return sepk_fail(SEP_GPU_EARG, "time_pair: reps %d%s", reps,
+				 d->capturing ? " during capture" : "");
+	int rc = sepk_flush_nh(d);
+	if (rc || (rc = sepk_flush_tasks(d)))
+		return rc;
+	sepk_lj c;
+	memcpy(&c, d->last_fused ? (const void *)&d->fused_lj : d->last_lj, sizeof c);
+	const sepk_lj_kernel k = (sepk_lj_kernel)d->last_lj_kernel;
+	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
+	double *part = sepk_region(d, SEPK_R_PAIR);
+	// a fused step kernel is timed with its integrator outputs sent to
+	// scratch, so that back-to-back launches see the same state
+	sepk_lfout dry;
+	memset(&dry, 0, sizeof dry);
+	if (d->last_fused) {
+		const size_t np = (size_t)d->npad;
+		if (!d->tscratch)
+			SEPK_CHECK(hipMalloc(&d->tscratch, 21 * np * sizeof(double)));
+		double *b = d->tscratch;
+		dry.v = b;
+		dry.f = b + 3 * np;
+		dry.a = b + 6 * np;
+		dry.xn = b + 9 * np;
+		dry.cn = (int *)(b + 12 * np);
+		dry.cr = (int *)(b + 12 * np) + 3 * np;
+		dry.pos = (sepk_pos *)(b + 15 * np);
+		dry.qpos = (int4 *)(b + 19 * np);
+	}
+	hipEvent_t e0, e1;
+	SEPK_CHECK(hipEventCreate(&e0));
+	SEPK_CHECK(hipEventCreate(&e1));
+	hipError_t err = hipEventRecord(e0, d->st);
+	for (int r = 0; r < reps && err == hipSuccess; r++) {
+		if (d->last_fused)
+			launch_step_kernel(d, c, d->fused_mixed, d->fused_fast != 0, d->fused_nh,
+					   d->fused_alpha, d->fused_dt, d->fused_L, dry, nb);
+		else
+			k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n,
+							  d->npad, c, part, d->region_rows);
+		err = hipGetLastError();
+	}
+	if (err == hipSuccess)
+		err = hipEventRecord(e1, d->st);
+	if (err == hipSuccess)
+		err = hipEventSynchronize(e1);
+	float ms = 0.0f;
+	if (err == hipSuccess)
+		err = hipEventElapsedTime(&ms, e0, e1);
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	if (err != hipSuccess)
+		return sepk_fail(SEP_GPU_EDEVICE, "time_pair: %s", hipGetErrorString(err));
+	d->f_zero_pending = 0;
+	*ms_per_launch = ms / reps;
 	return 0;
 }

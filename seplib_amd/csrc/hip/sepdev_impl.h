@@ -49,6 +49,15 @@ struct sepk_state {
 	int *partners; // 40 blocks of npad, allocated with the topology
 };
 
+// Constants of one LJ pair-force launch (sep_force_pairs / sep_force_lj).
+struct sepk_lj {
+	double cf2, A, nB, C, nD, sig2, ushift;
+	double L0, L1, L2, iL0, iL1, iL2;
+	double qd0, qd1, qd2; // L / 2^32: one step of the quantized positions
+	float Af, nBf, Cf, nDf, sig2f, ushiftf;
+	signed char t0, t1;
+};
+
 // What the finisher does with the combined rows.  Partial rows live in
 // separate regions so that a build, a pair call and a leapfrog of one step
 // can all be finished by a single launch after the leapfrog.
@@ -166,6 +175,24 @@ struct sepdev {
 	// position uploads; the leapfrog and the rebuild keep qpos current)
 	int qpos_stale;
 	double qL[3];
+
+	// Fused step (pair forces, Nose-Hoover and leapfrog in one kernel),
+	// armed by the host around a step whose only calls are one pair call,
+	// sep_nosehoover and sep_leapfrog: the pair call is recorded instead of
+	// launched and the leapfrog launches both.  The fused kernel writes the
+	// new positions into pos_alt / qpos_alt (other atoms still gather the
+	// old ones) and the host swaps them with s[0].pos / s[0].qpos, so that
+	// s[0] always names the current positions.
+	sepk_pos *pos_alt;
+	int4 *qpos_alt;
+	int fuse_armed, fused_pending;
+	sepk_lj fused_lj;
+	int fused_mixed, fused_fast, fused_ret, fused_epot_assign;
+	// the last fused launch, for sepdev_time_pair (dry: outputs to tscratch)
+	int last_fused, fused_nh;
+	double *fused_alpha;
+	double fused_dt, fused_L[3];
+	double *tscratch;
 
 	// graph
 	hipGraph_t graph;
@@ -326,6 +353,8 @@ __device__ __forceinline__ void sepk_add_sym9(double *dst9, const double *w6,
 
 // deferred-work helpers (sepdev_force.hip, sepdev_reduce.hip)
 int sepk_materialize_forces(sepdev *d);
+// launches a pair call recorded for a fused step, if any (sepdev_force.hip)
+int sepk_flush_pair(sepdev *d);
 int sepk_flush_nh(sepdev *d);
 int sepk_put_vec_to(sepdev *d, double *dst, const double *src3n);
 int sepk_get_vec_from(sepdev *d, double *dst3n, const double *src);

@@ -212,6 +212,10 @@ double *sepk_region(sepdev *d, int region)
 
 int sepk_flush_tasks(sepdev *d)
 {
+	// a recorded pair call queues its task when it is launched
+	int rc = sepk_flush_pair(d);
+	if (rc)
+		return rc;
 	if (d->npending == 0)
 		return 0;
 	sepk_fin f;
@@ -232,6 +236,9 @@ int sepk_flush_tasks(sepdev *d)
 
 int sepk_claim_region(sepdev *d, int region)
 {
+	int rc = sepk_flush_pair(d);
+	if (rc)
+		return rc;
 	for (int i = 0; i < d->npending; i++)
 		if (d->pending[i].region == region)
 			return sepk_flush_tasks(d);

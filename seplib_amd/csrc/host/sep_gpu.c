@@ -69,8 +69,42 @@ int sep_gpu_upload(seppart *atoms, sepsys *sys)
 
 /* ---- fused run -------------------------------------------------------- */
 
+/* A step of one pair call, an optional sep_nosehoover and sep_leapfrog: on
+ * a resident system the device runs it as one fused kernel. */
+static int fusable(const sep_gpu_op *ops, int nops)
+{
+	if (nops < 2 || nops > 3)
+		return 0;
+	if (ops[0].op != SEP_GPU_OP_PAIRS && ops[0].op != SEP_GPU_OP_LJ)
+		return 0;
+	if (ops[nops - 1].op != SEP_GPU_OP_LEAPFROG)
+		return 0;
+	return nops == 2 || ops[1].op == SEP_GPU_OP_NOSEHOOVER;
+}
+
+static int run_ops(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
+		   int nops);
+
 static int run_one_step(seppart *atoms, sepsys *sys, sepret *ret,
 			const sep_gpu_op *ops, int nops)
+{
+	seph_ctx *c = seph_ctx_for(atoms, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	const int fuse = c->resident && fusable(ops, nops);
+	if (fuse)
+		sepdev_fuse_step(c->dev, 1);
+	int rc = run_ops(atoms, sys, ret, ops, nops);
+	if (fuse) {
+		const int rc2 = seph_dev_fail(sepdev_fuse_step(c->dev, 0));
+		if (!rc)
+			rc = rc2;
+	}
+	return rc;
+}
+
+static int run_ops(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
+		   int nops)
 {
 	sep_reset_retval(ret);
 	sep_reset_force(atoms, sys);
@@ -132,14 +166,26 @@ int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
 				return rc;
 		return 0;
 	}
-	if ((rc = seph_dev_fail(sepdev_graph_begin(c->dev))))
-		return rc;
-	rc = run_one_step(atoms, sys, ret, ops, nops);
-	int rc2 = seph_dev_fail(sepdev_graph_end(c->dev));
-	if (rc || rc2)
-		return rc ? rc : rc2;
-	/* the captured step was not executed; replay it nsteps - 1 times */
-	return seph_dev_fail(sepdev_graph_launch(c->dev, nsteps - 1));
+	/* Two steps per graph: a fused step writes the new positions into the
+	 * other of two buffers, so consecutive steps alternate between them
+	 * and a replayed pair of steps returns to the captured assignment. */
+	const long rest = nsteps - 1;
+	if (rest >= 2) {
+		if ((rc = seph_dev_fail(sepdev_graph_begin(c->dev))))
+			return rc;
+		rc = run_one_step(atoms, sys, ret, ops, nops);
+		if (!rc)
+			rc = run_one_step(atoms, sys, ret, ops, nops);
+		int rc2 = seph_dev_fail(sepdev_graph_end(c->dev));
+		if (rc || rc2)
+			return rc ? rc : rc2;
+		/* the captured steps were not executed */
+		if ((rc = seph_dev_fail(sepdev_graph_launch(c->dev, rest / 2))))
+			return rc;
+	}
+	if (rest % 2)
+		rc = run_one_step(atoms, sys, ret, ops, nops);
+	return rc;
 }
 
 /* ---- diagnostics ------------------------------------------------------ */

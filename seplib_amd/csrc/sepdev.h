@@ -168,6 +168,11 @@ int sepdev_nosehoover(sepdev *d, double Td, double tau, double dt,
 		      int alpha_slot);
 int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *L,
 		    int ret_slot);
+/* on = 1: until the matching off call, a non-typed pair call that starts the
+ * step's forces is recorded and launched fused with the next leapfrog (any
+ * other use of the forces launches it first); off launches a recorded call
+ * that is still waiting */
+int sepdev_fuse_step(sepdev *d, int on);
 
 /* ---- misc */
 int sepdev_synchronize(sepdev *d);
