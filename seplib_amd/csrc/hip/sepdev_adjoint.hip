@@ -48,12 +48,6 @@ __device__ __forceinline__ double sepk_img(double d, double L, double iL)
 	return sepk_wrap(d, L);
 }
 
-// Pointers to the SEPK_PAD-padded list of lane `sub` of atom t.
-__device__ __forceinline__ const int *sepk_lane_list(const int *nbr, long t, int sub)
-{
-	return nbr + (size_t)t * SEPK_SUB + sub;
-}
-
 template <bool OVERWRITE>
 __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_table(
 	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt, long n,
@@ -75,12 +69,10 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_table(
 		const sepk_pos pi = s.pos[t];
 		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 		const size_t stride = (size_t)npad * SEPK_SUB;
-		const int *lp = sepk_lane_list(nbr, t, sub);
-		for (int b = 0; b < nbat; b++, lp += SEPK_BATCH * stride) {
-			int j[SEPK_BATCH];
-#pragma unroll
-			for (int u = 0; u < SEPK_BATCH; u++)
-				j[u] = lp[u * stride];
+		const int4 *lp = sepk_lane_list(nbr, t, sub);
+		for (int b = 0; b < nbat; b++, lp += stride) {
+			const int4 jb = *lp;
+			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
 			sepk_pos pj[SEPK_BATCH];
 			int tj[SEPK_BATCH];
 #pragma unroll
@@ -164,12 +156,10 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_adjoint(
 		const double gix = g[t], giy = g[npad + t], giz = g[2 * npad + t];
 		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 		const size_t stride = (size_t)npad * SEPK_SUB;
-		const int *lp = sepk_lane_list(nbr, t, sub);
-		for (int b = 0; b < nbat; b++, lp += SEPK_BATCH * stride) {
-			int j[SEPK_BATCH];
-#pragma unroll
-			for (int u = 0; u < SEPK_BATCH; u++)
-				j[u] = lp[u * stride];
+		const int4 *lp = sepk_lane_list(nbr, t, sub);
+		for (int b = 0; b < nbat; b++, lp += stride) {
+			const int4 jb = *lp;
+			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {
 				const unsigned jj = (unsigned)j[u];

@@ -122,6 +122,20 @@ __device__ __forceinline__ double sepk_image(double d, double L, double iL, doub
 
 // Adds one pair's force ft r to the atom and its energy; the stress gets
 // only the crossing term -f (x) s (see sepk_pair_epilogue).
+__device__ __forceinline__ void sepk_pair_cross(
+	double ft, double dx, double dy, double dz, double kx, double ky, double kz, double L0,
+	double L1, double L2, double (&acc)[7])
+{
+	const double sx = kx * L0, sy = ky * L1, sz = kz * L2;
+	const double gx = -ft * dx, gy = -ft * dy, gz = -ft * dz;
+	acc[1] = fma(gx, sx, acc[1]);
+	acc[2] = fma(gx, sy, acc[2]);
+	acc[3] = fma(gx, sz, acc[3]);
+	acc[4] = fma(gy, sy, acc[4]);
+	acc[5] = fma(gy, sz, acc[5]);
+	acc[6] = fma(gz, sz, acc[6]);
+}
+
 __device__ __forceinline__ void sepk_pair_accumulate(
 	double ft, double e, double dx, double dy, double dz, double kx, double ky, double kz,
 	double L0, double L1, double L2, double &fx, double &fy, double &fz, double (&acc)[7])
@@ -130,16 +144,8 @@ __device__ __forceinline__ void sepk_pair_accumulate(
 	fy = fma(ft, dy, fy);
 	fz = fma(ft, dz, fz);
 	acc[0] += e;
-	if (kx != 0.0 || ky != 0.0 || kz != 0.0) {
-		const double sx = kx * L0, sy = ky * L1, sz = kz * L2;
-		const double gx = -ft * dx, gy = -ft * dy, gz = -ft * dz;
-		acc[1] = fma(gx, sx, acc[1]);
-		acc[2] = fma(gx, sy, acc[2]);
-		acc[3] = fma(gx, sz, acc[3]);
-		acc[4] = fma(gy, sy, acc[4]);
-		acc[5] = fma(gy, sz, acc[5]);
-		acc[6] = fma(gz, sz, acc[6]);
-	}
+	if (kx != 0.0 || ky != 0.0 || kz != 0.0)
+		sepk_pair_cross(ft, dx, dy, dz, kx, ky, kz, L0, L1, L2, acc);
 }
 
 // Entries of one lane are taken SEPK_BATCH at a time: every index and
@@ -149,9 +155,9 @@ __device__ __forceinline__ void sepk_pair_accumulate(
 // which fails every test, so the loads need neither clamps nor branches; the
 // accumulation order is the list order.
 //
-// Addressing: lane `sub` of atom t reads its e-th entry at
-// lp + e * stride with lp = nbr + 4 t + sub and stride = 4 npad; neighbour
-// records are fetched with 32-bit byte offsets from the uniform base.
+// Addressing: lane `sub` of atom t reads the SEPK_BATCH indices of its
+// batch b as one int4 (sepk_entry, sepk_lane_list); neighbour records are
+// fetched with 32-bit byte offsets from the uniform base.
 __device__ __forceinline__ const sepk_pos &sepk_rec(const char *base, int j)
 {
 	return *(const sepk_pos *)(base + (unsigned)j * (unsigned)sizeof(sepk_pos));
@@ -188,12 +194,10 @@ __device__ __forceinline__ void sepk_lj_pairs(
 			qi = s.qpos[t];
 			const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 			const size_t stride = (size_t)npad * SEPK_SUB;
-			const int *lp = nbr + (size_t)t * SEPK_SUB + sub;
-			for (int b = 0; b < nbat; b++, lp += SEPK_BATCH * stride) {
-				int j[SEPK_BATCH];
-#pragma unroll
-				for (int u = 0; u < SEPK_BATCH; u++)
-					j[u] = lp[u * stride];
+			const int4 *lp = sepk_lane_list(nbr, t, sub);
+			for (int b = 0; b < nbat; b++, lp += stride) {
+				const int4 jb = *lp;
+				const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
 				int4 qj[SEPK_BATCH];
 				signed char tj[SEPK_BATCH];
 #pragma unroll
@@ -224,16 +228,22 @@ __device__ __forceinline__ void sepk_lj_pairs(
 					const float q3 = q * q * q;
 					const double ft = (double)(fmaf(c.Af, q3, c.nBf) * q3 * q);
 					const double e = (double)fmaf(fmaf(c.Cf, q3, c.nDf), q3, c.ushiftf);
+					fx = fma(ft, dx, fx);
+					fy = fma(ft, dy, fy);
+					fz = fma(ft, dz, fz);
+					acc[0] += e;
 					const bool bx = (ux < vx) != (ix < 0), by = (uy < vy) != (iy < 0),
 						   bz = (uz < vz) != (iz < 0);
-					// k = +1 when the unwrapped difference is >= 2^31
-					// steps (wrapped negative), -1 when it is below
-					// -2^31 (wrapped non-negative)
-					const double kx = bx ? (ix < 0 ? 1.0 : -1.0) : 0.0;
-					const double ky = by ? (iy < 0 ? 1.0 : -1.0) : 0.0;
-					const double kz = bz ? (iz < 0 ? 1.0 : -1.0) : 0.0;
-					sepk_pair_accumulate(ft, e, dx, dy, dz, kx, ky, kz, c.L0, c.L1,
-							     c.L2, fx, fy, fz, acc);
+					if (bx | by | bz) {
+						// k = +1 when the unwrapped difference is >= 2^31
+						// steps (wrapped negative), -1 when it is below
+						// -2^31 (wrapped non-negative)
+						const double kx = bx ? (ix < 0 ? 1.0 : -1.0) : 0.0;
+						const double ky = by ? (iy < 0 ? 1.0 : -1.0) : 0.0;
+						const double kz = bz ? (iz < 0 ? 1.0 : -1.0) : 0.0;
+						sepk_pair_cross(ft, dx, dy, dz, kx, ky, kz, c.L0, c.L1, c.L2,
+								acc);
+					}
 				}
 			}
 		}
@@ -247,13 +257,11 @@ __device__ __forceinline__ void sepk_lj_pairs(
 		pi = s.pos[t];
 		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 		const size_t stride = (size_t)npad * SEPK_SUB;
-		const int *lp = nbr + (size_t)t * SEPK_SUB + sub;
+		const int4 *lp = sepk_lane_list(nbr, t, sub);
 		const char *pb = (const char *)s.pos;
-		for (int b = 0; b < nbat; b++, lp += SEPK_BATCH * stride) {
-			int j[SEPK_BATCH];
-#pragma unroll
-			for (int u = 0; u < SEPK_BATCH; u++)
-				j[u] = lp[u * stride];
+		for (int b = 0; b < nbat; b++, lp += stride) {
+			const int4 jb = *lp;
+			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
 			double px[SEPK_BATCH], py[SEPK_BATCH], pz[SEPK_BATCH];
 			signed char tj[SEPK_BATCH];
 #pragma unroll
@@ -506,13 +514,11 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 	if (act) {
 		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 		const size_t stride = (size_t)npad * SEPK_SUB;
-		const int *lp = nbr + (size_t)t * SEPK_SUB + sub;
+		const int4 *lp = sepk_lane_list(nbr, t, sub);
 		const char *pb = (const char *)s.pos;
-		for (int b = 0; b < nbat; b++, lp += SEPK_BATCH * stride) {
-			int j[SEPK_BATCH];
-#pragma unroll
-			for (int u = 0; u < SEPK_BATCH; u++)
-				j[u] = lp[u * stride];
+		for (int b = 0; b < nbat; b++, lp += stride) {
+			const int4 jb = *lp;
+			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
 			sepk_pos pj[SEPK_BATCH];
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++)
@@ -1006,7 +1012,7 @@ __global__ void k_half_list(const int *orig, const int *nbr, const int *cnt, lon
 	const int o = orig[t];
 	int c = 0;
 	for (int k = 0; k < cnt[t]; k++) {
-		const int oj = orig[nbr[sepk_slot(k / SEPK_SUB, t, npad, k % SEPK_SUB)]];
+		const int oj = orig[nbr[sepk_entry(k, t, npad)]];
 		if (oj > o) {
 			if (c < stride)
 				half[(long)o * stride + c] = oj;

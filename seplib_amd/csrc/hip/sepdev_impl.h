@@ -10,15 +10,15 @@
 
 #define SEPK_BLOCK 256
 #define SEPK_WAVE 64
-// Lanes per atom in the list kernels.  Entry k of atom t's list is stored
-// at slot ((k/SEPK_SUB)*npad + t)*SEPK_SUB + k%SEPK_SUB, and lane k%SEPK_SUB
-// of the atom reads it: a wave covers 64/SEPK_SUB consecutive atoms, each of
-// its list reads is 256 contiguous bytes, and since every list is in cell
-// order the lanes of one atom gather neighbours from the same cache lines.
+// Lanes per atom in the list kernels.  Entry k of atom t's list is read by
+// lane k%SEPK_SUB of the atom (slot: sepk_entry): a wave covers
+// 64/SEPK_SUB consecutive atoms, each of its list reads is 1 KB of
+// contiguous memory, and since every list is in cell order the lanes of one
+// atom gather neighbours from the same cache lines.
 #define SEPK_SUB SEPDEV_LANES
-// entries a lane of a pair kernel takes at a time; every atom's list is
-// padded with the phantom slot n to a multiple of SEPK_PAD entries, so a
-// lane's entries come in full batches
+// entries a lane of a pair kernel takes at a time (one int4 of indices);
+// every atom's list is padded with the phantom slot n to a multiple of
+// SEPK_PAD entries, so a lane's entries come in full batches
 #define SEPK_BATCH 4
 #define SEPK_PAD (SEPK_SUB * SEPK_BATCH)
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
@@ -237,9 +237,21 @@ __device__ __forceinline__ long sepk_xcd_block()
 	return (long)(blockIdx.x % SEPK_NXCD) * per + blockIdx.x / SEPK_NXCD;
 }
 
-__device__ __forceinline__ long sepk_slot(long e, long t, long npad, int sub)
+// Slot of entry e of atom t's list.  Entries come in batches of SEPK_PAD;
+// within batch b, lane s of the atom (s = e % SEPK_SUB) takes entries
+// s, s + SEPK_SUB, ... and finds them side by side, so it reads its
+// SEPK_BATCH indices of a batch with one 16-byte load, and a wave's load
+// is 1 KB of consecutive memory.
+__device__ __forceinline__ size_t sepk_entry(long e, long t, long npad)
 {
-	return (e * npad + t) * SEPK_SUB + sub;
+	return (((size_t)(e / SEPK_PAD) * npad + t) * SEPK_SUB + e % SEPK_SUB) * SEPK_BATCH +
+	       (e / SEPK_SUB) % SEPK_BATCH;
+}
+
+// Batch 0 of lane `sub` of atom t; batch b is at [b * npad * SEPK_SUB].
+__device__ __forceinline__ const int4 *sepk_lane_list(const int *nbr, long t, int sub)
+{
+	return (const int4 *)nbr + (size_t)t * SEPK_SUB + sub;
 }
 
 // Single-image minimum image with strict comparisons against +-L/2, the

@@ -197,6 +197,75 @@ __global__ void k_scatter(const int *cellid, const int *slot, const int *start, 
 		order[start[cellid[t]] + slot[t]] = (int)t;
 }
 
+// Copies the partner table of atom o in src to slot t of dst, eight
+// entries per round trip.
+__device__ __forceinline__ void sepk_copy_partners(const sepk_state &src, const sepk_state &dst,
+						   long o, long t, long npad)
+{
+	for (int k0 = 0; k0 < 40; k0 += 8) {
+		int p[8];
+#pragma unroll
+		for (int k = 0; k < 8; k++)
+			p[k] = src.partners[(k0 + k) * npad + o];
+#pragma unroll
+		for (int k = 0; k < 8; k++)
+			dst.partners[(k0 + k) * npad + t] = p[k];
+	}
+}
+
+// Moves atom o of src to slot t of dst.  Each half of the state is loaded
+// whole before any of it is stored, so the move pays two memory round
+// trips instead of one per field (source and destination may alias as far
+// as the compiler knows).  pending: xn = x and cross_neighb = 0 are owed.
+__device__ __forceinline__ void sepk_move_atom(const sepk_state &src, long o,
+					       const sepk_state &dst, long t, long npad,
+					       bool pending)
+{
+	const sepk_pos p = src.pos[o];
+	{
+		const int4 q = src.qpos[o];
+		const double v0 = src.v[o], v1 = src.v[npad + o], v2 = src.v[2 * npad + o];
+		const double f0 = src.f[o], f1 = src.f[npad + o], f2 = src.f[2 * npad + o];
+		const double a0 = src.a[o], a1 = src.a[npad + o], a2 = src.a[2 * npad + o];
+		dst.pos[t] = p;
+		dst.qpos[t] = q;
+		dst.v[t] = v0;
+		dst.v[npad + t] = v1;
+		dst.v[2 * npad + t] = v2;
+		dst.f[t] = f0;
+		dst.f[npad + t] = f1;
+		dst.f[2 * npad + t] = f2;
+		dst.a[t] = a0;
+		dst.a[npad + t] = a1;
+		dst.a[2 * npad + t] = a2;
+	}
+	double x0 = src.xn[o], x1 = src.xn[npad + o], x2 = src.xn[2 * npad + o];
+	int n0 = src.cn[o], n1 = src.cn[npad + o], n2 = src.cn[2 * npad + o];
+	const int r0 = src.cr[o], r1 = src.cr[npad + o], r2 = src.cr[2 * npad + o];
+	const double m = src.m[o];
+	const int mol = src.mol[o], orig = src.orig[o];
+	const signed char ty = src.type[o];
+	if (pending) {
+		x0 = p.x;
+		x1 = p.y;
+		x2 = p.z;
+		n0 = n1 = n2 = 0;
+	}
+	dst.xn[t] = x0;
+	dst.xn[npad + t] = x1;
+	dst.xn[2 * npad + t] = x2;
+	dst.cn[t] = n0;
+	dst.cn[npad + t] = n1;
+	dst.cn[2 * npad + t] = n2;
+	dst.cr[t] = r0;
+	dst.cr[npad + t] = r1;
+	dst.cr[2 * npad + t] = r2;
+	dst.m[t] = m;
+	dst.mol[t] = mol;
+	dst.orig[t] = orig;
+	dst.type[t] = ty;
+}
+
 // One wave per cell (grid-stride over cells): rank each member by its
 // original index -- the keys of the cell are held one per lane and compared
 // through wave shuffles -- and move its state to start + rank in the spare
@@ -229,26 +298,10 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 			if (i >= e)
 				continue;
 			const long t = s + rank;
-			const sepk_pos p = src.pos[o];
-			dst.pos[t] = p;
-			dst.qpos[t] = src.qpos[o];
-			const double xyz[3] = {p.x, p.y, p.z};
-			for (int k = 0; k < 3; k++) {
-				dst.v[k * npad + t] = src.v[k * npad + o];
-				dst.f[k * npad + t] = src.f[k * npad + o];
-				dst.a[k * npad + t] = src.a[k * npad + o];
-				dst.xn[k * npad + t] = pending ? xyz[k] : src.xn[k * npad + o];
-				dst.cn[k * npad + t] = pending ? 0 : src.cn[k * npad + o];
-				dst.cr[k * npad + t] = src.cr[k * npad + o];
-			}
-			dst.m[t] = src.m[o];
-			dst.type[t] = src.type[o];
-			dst.mol[t] = src.mol[o];
-			dst.orig[t] = key;
+			sepk_move_atom(src, o, dst, t, npad, pending);
 			iperm[key] = (int)t;
 			if (has_partners)
-				for (int k = 0; k < 40; k++)
-					dst.partners[k * npad + t] = src.partners[k * npad + o];
+				sepk_copy_partners(src, dst, o, t, npad);
 		}
 	}
 }
@@ -273,23 +326,9 @@ __device__ __forceinline__ int sepk_share(const int *P, long npad, long ti, long
 __device__ __forceinline__ void sepk_copy_home(const sepk_state &src, const sepk_state &dst,
 					       long t, long npad, int has_partners)
 {
-	dst.pos[t] = src.pos[t];
-	dst.qpos[t] = src.qpos[t];
-	for (int k = 0; k < 3; k++) {
-		dst.v[k * npad + t] = src.v[k * npad + t];
-		dst.f[k * npad + t] = src.f[k * npad + t];
-		dst.a[k * npad + t] = src.a[k * npad + t];
-		dst.xn[k * npad + t] = src.xn[k * npad + t];
-		dst.cn[k * npad + t] = src.cn[k * npad + t];
-		dst.cr[k * npad + t] = src.cr[k * npad + t];
-	}
-	dst.m[t] = src.m[t];
-	dst.type[t] = src.type[t];
-	dst.mol[t] = src.mol[t];
-	dst.orig[t] = src.orig[t];
+	sepk_move_atom(src, t, dst, t, npad, false);
 	if (has_partners)
-		for (int k = 0; k < 40; k++)
-			dst.partners[k * npad + t] = src.partners[k * npad + t];
+		sepk_copy_partners(src, dst, t, t, npad);
 }
 
 // One workgroup per home cell at a time (cells split into one contiguous
@@ -299,10 +338,13 @@ __device__ __forceinline__ void sepk_copy_home(const sepk_state &src, const sepk
 // and the cell's members ("home" atoms, SEPK_BHOME at a time) beside them.
 // Each wave owns every fourth home atom: it takes 64 staged candidates into
 // registers and tests them against each of its atoms, whose position is an
-// LDS broadcast; a wave ballot gives every accepted candidate its place in
-// that atom's list.  Lists therefore come out in stencil-offset order and,
-// within a cell, in device order.  Entry k of atom t goes to slot
-// sepk_slot(k / SEPK_SUB, t, npad, k % SEPK_SUB).
+// LDS broadcast; the wave ballot of each test (one bit per candidate) is
+// kept in LDS.  An emit pass then writes the lists: each thread takes one
+// 64-candidate word of one home atom, finds the word's first entry number
+// from the atom's running length and the popcounts of its earlier words,
+// and walks the set bits.  Lists therefore come out in stencil-offset
+// order and, within a cell, in device order.  Entry k of atom t goes to
+// slot sepk_entry(k, t, npad).
 //
 // SHIFT: with at least 5 cells on every axis and every atom inside its own
 // cell, |x_i - x_j| < 2l <= 0.4 L for a target cell reached without
@@ -312,9 +354,11 @@ __device__ __forceinline__ void sepk_copy_home(const sepk_state &src, const sepk
 // its d - L / d + L, and d + 0 is d.  Otherwise the wrap is evaluated per
 // candidate.
 #define SEPK_BT 256      // threads of a build workgroup (4 waves)
-#define SEPK_BCHUNK 768  // candidates staged at a time
+#define SEPK_BCHUNK 640  // candidates staged at a time
+#define SEPK_BWORDS (SEPK_BCHUNK / 64)
 #define SEPK_BHOME 64    // home atoms staged at a time
 #define SEPK_BGRID 1280  // build workgroups at most (a multiple of 8)
+static_assert(SEPK_BCHUNK % 64 == 0, "candidates are tested 64 at a time");
 
 struct sepk_build_lds {
 	double cx[SEPK_BCHUNK], cy[SEPK_BCHUNK], cz[SEPK_BCHUNK];
@@ -325,18 +369,21 @@ struct sepk_build_lds {
 	int segjs[SEPDEV_MAX_OFF], segpre[SEPDEV_MAX_OFF + 1];
 	int segrule[SEPDEV_MAX_OFF]; // rule when the target is the home cell, else -1
 	double segsh[SEPDEV_MAX_OFF][3];
+	// accepted candidates of word w of the chunk for home atom i
+	unsigned long long vote[SEPK_BHOME][SEPK_BWORDS];
+	int hcnt[2][SEPK_BHOME]; // list lengths before / after the chunk
 	int hs, h, anyshift;
 };
 
 // Tests the 64 staged candidates [q0, q0 + 64) of a chunk of kn against
-// the wave's home atoms i = wid, wid + 4, ... < hn.  The wave's running
-// list lengths live in mreg, lane kk holding that of its kk-th atom.
+// the wave's home atoms i = wid, wid + 4, ... < hn and stores each ballot
+// in L.vote[i][q0 / 64].
 // SH: some stencil target of the cell lies across the box (shift adds).
 // EXCL: 0 = SEP_ALL, 2 = SEP_EXCL_BONDED, 3 = SEP_EXCL_SAME_MOL.
 template <bool SHIFT, bool SH, int EXCL>
-__device__ __forceinline__ void sepk_build_chunk(
-	const sepk_build_lds &L, const sepk_state &s, const sepdev_grid &g, int q0, int kn,
-	long tbase, int hn, long npad, int cap, int *nbr, int &mreg)
+__device__ __forceinline__ void sepk_build_test(
+	sepk_build_lds &L, const sepk_state &s, const sepdev_grid &g, int q0, int kn,
+	long tbase, int hn, long npad)
 {
 	const int lane = threadIdx.x & 63;
 	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
@@ -353,17 +400,19 @@ __device__ __forceinline__ void sepk_build_chunk(
 		sy = L.segsh[o][1];
 		sz = L.segsh[o][2];
 	}
-	const size_t stride = (size_t)npad * SEPK_SUB;
+	// lane kk collects the ballot of the wave's kk-th home atom
+	unsigned vlo = 0u, vhi = 0u;
 	int kk = 0;
 	for (int i = wid; i < hn; i += SEPK_BT / 64, kk++) {
 		const long t = tbase + i;
+		const int t32 = (int)t;
 		bool take = valid;
 		if (SHIFT) {
 			// the 27-cell stencil reaches the home cell only through
 			// offset 0, whose rule is NE
-			take = take && !(rule >= 0 && j == t);
+			take = take && !(rule >= 0 && j == t32);
 		} else if (rule == SEPDEV_RULE_NE) {
-			take = take && j != t;
+			take = take && j != t32;
 		} else if (rule >= 0 && take) {
 			const int oj = s.orig[j];
 			const int oi = L.horig[i];
@@ -389,7 +438,7 @@ __device__ __forceinline__ void sepk_build_chunk(
 			dz = sepk_wrap(L.hz[i] - pz, g.L[2]);
 		}
 		const double r2 = dx * dx + dy * dy + dz * dz;
-		take = take && r2 < g.rlist2;
+		take = take & (r2 < g.rlist2);
 		if (EXCL == 2 /* SEP_EXCL_BONDED */ && take) {
 			const int oi = L.horig[i];
 			const int oj = s.orig[j];
@@ -403,19 +452,51 @@ __device__ __forceinline__ void sepk_build_chunk(
 			take = g.bonded_all == 2 ? sh != 1 : sh == 0;
 		}
 		const unsigned long long vote = __ballot(take);
-		if (vote) {
-			const int m = __builtin_amdgcn_readlane(mreg, kk);
-			if (take) {
-				const unsigned below = __builtin_amdgcn_mbcnt_hi(
-					(unsigned)(vote >> 32),
-					__builtin_amdgcn_mbcnt_lo((unsigned)vote, 0u));
-				const unsigned e = (unsigned)m + below;
-				if (e < (unsigned)cap)
-					nbr[(size_t)t * SEPK_SUB + (size_t)(e >> 2) * stride + (e & 3u)] = j;
-			}
-			if (lane == kk)
-				mreg = m + __popcll(vote);
+		if (lane == kk) {
+			vlo = (unsigned)vote;
+			vhi = (unsigned)(vote >> 32);
 		}
+	}
+	const int i = wid + (SEPK_BT / 64) * lane;
+	if (lane < kk && i < hn)
+		L.vote[i][q0 / 64] = ((unsigned long long)vhi << 32) | vlo;
+}
+
+// Slot of entry e of atom t (sepk_entry); lists of fewer than 2^20 padded
+// atoms take 32-bit arithmetic with a full-rate 24-bit multiply.
+__device__ __forceinline__ int *sepk_slot_ptr(int *nbr, unsigned e, long t, long npad)
+{
+	if (npad < (1l << 20)) {
+		const unsigned npad16 = (unsigned)npad * SEPK_PAD;
+		const unsigned off = __umul24(e / SEPK_PAD, npad16) + (unsigned)t * SEPK_PAD +
+				     (e % SEPK_SUB) * SEPK_BATCH + (e / SEPK_SUB) % SEPK_BATCH;
+		return nbr + off;
+	}
+	return nbr + sepk_entry(e, t, npad);
+}
+
+// Writes the accepted candidates of a chunk of kn to the lists of the hn
+// home atoms from tbase: thread x takes the words (home atom, word) x,
+// x + SEPK_BT, ...  The lengths before the chunk are in L.hcnt[par]; the
+// last word of each atom leaves the length after it in L.hcnt[par ^ 1].
+// Entries at or beyond cap are counted but not stored.
+__device__ __forceinline__ void sepk_build_emit(sepk_build_lds &L, int kn, int hn, int par,
+						long tbase, long npad, int cap, int *nbr)
+{
+	const int nw = (kn + 63) / 64;
+	for (int it = threadIdx.x; it < hn * nw; it += SEPK_BT) {
+		const int i = it / nw, w = it - i * nw;
+		int e = L.hcnt[par][i];
+		for (int u = 0; u < w; u++)
+			e += __popcll(L.vote[i][u]);
+		unsigned long long m = L.vote[i][w];
+		if (w == nw - 1)
+			L.hcnt[par ^ 1][i] = e + __popcll(m);
+		const long t = tbase + i;
+		const int *cj = L.cj + w * 64;
+		for (; m; m &= m - 1, e++)
+			if (e < cap)
+				*sepk_slot_ptr(nbr, (unsigned)e, t, npad) = cj[__builtin_ctzll(m)];
 	}
 }
 
@@ -474,27 +555,51 @@ __device__ __forceinline__ void sepk_build_cell(
 	const bool anyshift = L.anyshift;
 	for (int hb = 0; hb < h; hb += SEPK_BHOME) {
 		const int hn = min(h - hb, SEPK_BHOME);
-		int mreg = 0;
-		for (int k0 = 0; k0 < T; k0 += SEPK_BCHUNK) {
+		if (threadIdx.x < SEPK_BHOME)
+			L.hcnt[0][threadIdx.x] = 0;
+		int par = 0;
+		for (int k0 = 0; k0 < T; k0 += SEPK_BCHUNK, par ^= 1) {
 			const int kn = min(T - k0, SEPK_BCHUNK);
 			if (k0 > 0)
 				__syncthreads(); // the previous chunk is no longer read
-			// stage: wave w copies the segments w, w + 4, ... that overlap
-			// [k0, k0 + kn)
-			for (int o = wid; o < g.noff; o += SEPK_BT / 64) {
-				const int p0 = L.segpre[o], p1 = L.segpre[o + 1];
-				const int a = max(p0, k0), b = min(p1, k0 + kn);
-				for (int k = a + lane; k < b; k += 64) {
-					const int j = L.segjs[o] + (k - p0);
-					const sepk_pos p = s.pos[j];
-					const int q = k - k0;
-					L.cx[q] = p.x;
-					L.cy[q] = p.y;
-					L.cz[q] = p.z;
-					L.cj[q] = j;
-					L.co[q] = (unsigned char)o;
+			// stage [k0, k0 + kn): thread x takes candidates x, x + SEPK_BT,
+			// ...; each finds its segment by a binary search of the prefix
+			// sums and all of its loads are issued before any LDS store
+			constexpr int PER = (SEPK_BCHUNK + SEPK_BT - 1) / SEPK_BT;
+			int jq[PER], oq[PER];
+#pragma unroll
+			for (int r = 0; r < PER; r++) {
+				const int q = threadIdx.x + r * SEPK_BT;
+				const int k = k0 + q;
+				int o = 0; // the last segment starting at or before k
+#pragma unroll
+				for (int step = 32; step > 0; step >>= 1)
+					if (o + step < g.noff && L.segpre[o + step] <= k)
+						o += step;
+				oq[r] = q < kn && q < SEPK_BCHUNK ? o : -1;
+				jq[r] = L.segjs[o] + (k - L.segpre[o]);
+			}
+			sepk_pos pq[PER];
+			int mq[PER];
+#pragma unroll
+			for (int r = 0; r < PER; r++) {
+				if (oq[r] >= 0) {
+					pq[r] = s.pos[jq[r]];
 					if (EXCL == 3)
-						L.cm[q] = s.mol[j];
+						mq[r] = s.mol[jq[r]];
+				}
+			}
+#pragma unroll
+			for (int r = 0; r < PER; r++) {
+				if (oq[r] >= 0) {
+					const int q = threadIdx.x + r * SEPK_BT;
+					L.cx[q] = pq[r].x;
+					L.cy[q] = pq[r].y;
+					L.cz[q] = pq[r].z;
+					L.cj[q] = jq[r];
+					L.co[q] = (unsigned char)oq[r];
+					if (EXCL == 3)
+						L.cm[q] = mq[r];
 				}
 			}
 			if (k0 == 0)
@@ -510,26 +615,26 @@ __device__ __forceinline__ void sepk_build_cell(
 			__syncthreads();
 			for (int q0 = 0; q0 < kn; q0 += 64) {
 				if (SHIFT && anyshift)
-					sepk_build_chunk<SHIFT, true, EXCL>(L, s, g, q0, kn, hs + hb, hn, npad,
-									   cap, nbr, mreg);
+					sepk_build_test<SHIFT, true, EXCL>(L, s, g, q0, kn, hs + hb, hn, npad);
 				else
-					sepk_build_chunk<SHIFT, false, EXCL>(L, s, g, q0, kn, hs + hb, hn,
-									    npad, cap, nbr, mreg);
+					sepk_build_test<SHIFT, false, EXCL>(L, s, g, q0, kn, hs + hb, hn, npad);
 			}
+			__syncthreads();
+			sepk_build_emit(L, kn, hn, par, hs + hb, npad, cap, nbr);
 		}
-		// lane kk of wave w holds the length of home atom w + 4 kk; the
-		// list is padded with the phantom slot to a multiple of SEPK_PAD
-		const int i = wid + (SEPK_BT / 64) * lane;
-		if (lane < SEPK_BHOME / (SEPK_BT / 64) && i < hn) {
+		__syncthreads();
+		// the list is padded with the phantom slot to a multiple of SEPK_PAD
+		if (threadIdx.x < hn) {
+			const int i = threadIdx.x;
+			const int mreg = L.hcnt[par][i];
 			const long t = hs + hb + i;
 			cnt[t] = mreg < cap ? mreg : cap;
 			if (mreg > cap)
 				sepk_set_error(sys, SEP_GPU_ENEIGHB);
 			mymax = max(mymax, mreg);
 			const int pad = min((mreg + SEPK_PAD - 1) / SEPK_PAD * SEPK_PAD, cap);
-			const size_t stride = (size_t)npad * SEPK_SUB;
 			for (int e = mreg; e < pad; e++)
-				nbr[(size_t)t * SEPK_SUB + (size_t)(e >> 2) * stride + (e & 3)] = phantom;
+				nbr[sepk_entry(e, t, npad)] = phantom;
 		}
 		if (hb + SEPK_BHOME < h)
 			__syncthreads(); // the next batch restages the candidates
@@ -541,7 +646,7 @@ __device__ __forceinline__ void sepk_build_cell(
 // permuted state of the workgroup's cells goes home at the end, in one
 // batch, so its memory round trip is paid once.
 template <int EXCL>
-__global__ __launch_bounds__(SEPK_BT) void k_build(
+__global__ __launch_bounds__(SEPK_BT) __attribute__((amdgpu_waves_per_eu(EXCL == 2 ? 4 : 5))) void k_build(
 	sepk_state s, sepk_state home, const int *start, sepdev_grid g, long ncell, long n,
 	long npad, int cap, int *nbr, int *cnt, sepdev_sys *sys, int conditional,
 	int has_partners)
@@ -563,9 +668,11 @@ __global__ __launch_bounds__(SEPK_BT) void k_build(
 			sepk_build_cell<false, EXCL>(L, s, c, start, g, npad, cap, (int)n, nbr, cnt,
 						     sys, mymax);
 	}
-	for (long c = lo + k; c < hi; c += per) {
+	// one wave per cell, SEPK_BT / 64 cells at a time
+	const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+	for (long c = lo + k + wv * per; c < hi; c += (SEPK_BT / 64) * per) {
 		const int a = start[c], b = start[c + 1];
-		for (int t = a + threadIdx.x; t < b; t += SEPK_BT)
+		for (int t = a + ln; t < b; t += 64)
 			sepk_copy_home(s, home, t, npad, has_partners);
 	}
 	for (int o = 32; o > 0; o >>= 1)
