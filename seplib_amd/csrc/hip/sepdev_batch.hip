@@ -129,6 +129,29 @@ __device__ __forceinline__ int sepb_mod(int a, int n)
 
 #define SEPB_BIN_THREADS 1024
 #define SEPB_MAX_CELLS 8192 // cells per system (LDS counters)
+#define SEPB_BIN_UNROLL 8
+
+// Cell of position q in system y (floor and periodic wrap); an atom outside
+// [0, L) on some axis (x == L after a wrap, or a starting position) sets
+// `outside`, which rules out the per-segment image shifts of the build.
+__device__ __forceinline__ int sepb_cell(sepb_sys *y, const sepk_pos &q, int &outside)
+{
+	const double xs[3] = {q.x, q.y, q.z};
+	int ci[3];
+	for (int k = 0; k < 3; k++) {
+		const double w = xs[k] / y->lcell[k];
+		if (!(fabs(w) < 1.0e9)) {
+			atomicCAS(&y->error, 0, SEP_GPU_ECELL);
+			ci[k] = 0;
+		} else {
+			const int f = (int)floor(w);
+			if (f < 0 || f >= y->ncell[k])
+				outside = 1;
+			ci[k] = sepb_mod(f, y->ncell[k]);
+		}
+	}
+	return ci[0] + y->ncell[0] * (ci[1] + y->ncell[1] * ci[2]);
+}
 
 // One workgroup per flagged system: the cell of every atom, its slot among
 // the cell's members (LDS atomics), the exclusive scan of the counts into
@@ -151,28 +174,26 @@ __global__ __launch_bounds__(SEPB_BIN_THREADS) void k_bbin(sepb_view b)
 		outside = 0;
 	__syncthreads();
 	const sepk_pos *pos = b.st[y->cur].pos[y->p] + off;
-	for (int t = threadIdx.x; t < n; t += SEPB_BIN_THREADS) {
-		const sepk_pos q = pos[t];
-		const double xs[3] = {q.x, q.y, q.z};
-		int ci[3];
-		for (int k = 0; k < 3; k++) {
-			const double u = xs[k] / y->lcell[k];
-			if (!(fabs(u) < 1.0e9)) {
-				atomicCAS(&y->error, 0, SEP_GPU_ECELL);
-				ci[k] = 0;
-			} else {
-				const int f = (int)floor(u);
-				// an atom outside [0, L) on some axis (x == L after a
-				// wrap, or a starting position) rules out the
-				// per-segment image shifts of the build
-				if (f < 0 || f >= y->ncell[k])
-					outside = 1;
-				ci[k] = sepb_mod(f, y->ncell[k]);
+	// SEPB_BIN_UNROLL positions per thread are loaded before any is
+	// binned, so a large system pays a few memory round trips, not one
+	// per SEPB_BIN_THREADS atoms
+	for (int t0 = threadIdx.x; t0 < n; t0 += SEPB_BIN_THREADS * SEPB_BIN_UNROLL) {
+		sepk_pos qq[SEPB_BIN_UNROLL];
+#pragma unroll
+		for (int u = 0; u < SEPB_BIN_UNROLL; u++) {
+			const int t = t0 + u * SEPB_BIN_THREADS;
+			if (t < n)
+				qq[u] = pos[t];
+		}
+#pragma unroll
+		for (int u = 0; u < SEPB_BIN_UNROLL; u++) {
+			const int t = t0 + u * SEPB_BIN_THREADS;
+			if (t < n) {
+				const int c = sepb_cell(y, qq[u], outside);
+				b.cellid[off + t] = c;
+				b.cslot[off + t] = atomicAdd(&cnt[c], 1);
 			}
 		}
-		const int c = ci[0] + y->ncell[0] * (ci[1] + y->ncell[1] * ci[2]);
-		b.cellid[off + t] = c;
-		b.cslot[off + t] = atomicAdd(&cnt[c], 1);
 	}
 	__syncthreads();
 	// exclusive scan of cnt in place, SEPB_BIN_THREADS cells at a time
@@ -207,8 +228,21 @@ __global__ __launch_bounds__(SEPB_BIN_THREADS) void k_bbin(sepb_view b)
 		st[nc] = carry;
 		y->unclean = outside;
 	}
-	for (int t = threadIdx.x; t < n; t += SEPB_BIN_THREADS)
-		b.otmp[off + cnt[b.cellid[off + t]] + b.cslot[off + t]] = t;
+	for (int t0 = threadIdx.x; t0 < n; t0 += SEPB_BIN_THREADS * SEPB_BIN_UNROLL) {
+		int cc[SEPB_BIN_UNROLL], ss[SEPB_BIN_UNROLL];
+#pragma unroll
+		for (int u = 0; u < SEPB_BIN_UNROLL; u++) {
+			const int t = t0 + u * SEPB_BIN_THREADS;
+			cc[u] = t < n ? b.cellid[off + t] : 0;
+			ss[u] = t < n ? b.cslot[off + t] : 0;
+		}
+#pragma unroll
+		for (int u = 0; u < SEPB_BIN_UNROLL; u++) {
+			const int t = t0 + u * SEPB_BIN_THREADS;
+			if (t < n)
+				b.otmp[off + cnt[cc[u]] + ss[u]] = t;
+		}
+	}
 }
 
 // A lane's double as a uniform value.
@@ -250,21 +284,92 @@ __device__ __forceinline__ int sepb_offsets(int nc, int *o)
 	return nc;
 }
 
+// The cells of the flagged systems, numbered in system order, are the work
+// of k_bsort and k_bbuild: fpre[s] is the number of such cells before
+// system s (fpre[nsys]: all of them), scanned by each workgroup into LDS.
+// Batches of more than SEPB_FMAP systems walk every cell instead.
+#define SEPB_FMAP 1024
+__device__ __forceinline__ void sepb_flag_prefix(const sepb_view &b, int *fpre)
+{
+	__shared__ int wtot[SEPB_BLOCK / 64];
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	int carry = 0;
+	for (int base = 0; base < b.nsys; base += SEPB_BLOCK) {
+		const int s = base + threadIdx.x;
+		int v = 0;
+		if (s < b.nsys) {
+			const sepb_sys *y = b.sys + s;
+			v = y->neighb_flag ? y->ncell[0] * y->ncell[1] * y->ncell[2] : 0;
+		}
+		int incl = v;
+		for (int o = 1; o < 64; o <<= 1) {
+			const int u = __shfl_up(incl, o, 64);
+			if (lane >= o)
+				incl += u;
+		}
+		if (lane == 63)
+			wtot[wid] = incl;
+		__syncthreads();
+		int before = carry, tot = 0;
+		for (int w = 0; w < SEPB_BLOCK / 64; w++) {
+			if (w < wid)
+				before += wtot[w];
+			tot += wtot[w];
+		}
+		if (s < b.nsys)
+			fpre[s] = before + incl - v;
+		carry += tot;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		fpre[b.nsys] = carry;
+	__syncthreads();
+}
+
+// Work item w of a rebuild kernel -> (system, cell); false if the cell's
+// system is not flagged (only when every cell is walked).
+__device__ __forceinline__ bool sepb_work_cell(const sepb_view &b, const int *fpre, bool mapped,
+					       long w, int &s, int &c)
+{
+	if (mapped) {
+		// the last system whose flagged cells start at or before w
+		int lo = 0, hi = b.nsys;
+		while (hi - lo > 1) {
+			const int mid = (lo + hi) >> 1;
+			if (fpre[mid] <= w)
+				lo = mid;
+			else
+				hi = mid;
+		}
+		s = lo;
+		c = (int)(w - fpre[lo]);
+		return true;
+	}
+	s = b.cell_sys[w];
+	if (s < 0 || !b.sys[s].neighb_flag)
+		return false; // the scan's extra entry of a system, or no rebuild
+	c = (int)(w - b.sys[s].cell_off);
+	return true;
+}
+
 // One wave per cell (grid-stride): the new order of the cell's members and
 // their state moved to it.
 __global__ __launch_bounds__(SEPB_BLOCK) void k_bsort(sepb_view b)
 {
+	__shared__ int fpre[SEPB_FMAP + 1];
+	const bool mapped = b.nsys <= SEPB_FMAP;
+	long total = b.ncell_tot;
+	if (mapped) {
+		sepb_flag_prefix(b, fpre);
+		total = fpre[b.nsys];
+	}
 	const int lane = threadIdx.x & 63;
 	const long nw = ((long)gridDim.x * blockDim.x) >> 6;
-	for (long gc = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gc < b.ncell_tot;
-	     gc += nw) {
-		const int s = b.cell_sys[gc];
-		if (s < 0)
-			continue; // the scan's extra entry of a system
-		const sepb_sys *y = b.sys + s;
-		if (!y->neighb_flag)
+	for (long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < total; w += nw) {
+		int s, c;
+		if (!sepb_work_cell(b, fpre, mapped, w, s, c))
 			continue;
-		const int c = (int)(gc - y->cell_off);
+		const sepb_sys *y = b.sys + s;
 		const long off = y->off, slots = b.slots;
 		const int *st = b.cell_start + y->cell_off;
 		const int h0 = st[c], m = st[c + 1] - h0;
@@ -280,21 +385,36 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bsort(sepb_view b)
 				continue;
 			// xn = x and cross_neighb = 0 go with a rebuild
 			// (source/sepintgr.c:72-84)
+			// every field is loaded before any is stored (one memory
+			// round trip: the copies may alias as far as the compiler
+			// knows)
 			const long go = off + o, gn = off + h0 + r;
 			const sepk_pos q = So.pos[y->p][go];
-			Sn.pos[0][gn] = q;
+			double vv[3], ff[3], aa[3];
+			int cr[3];
+#pragma unroll
 			for (int k = 0; k < 3; k++) {
-				Sn.v[k * slots + gn] = So.v[k * slots + go];
-				Sn.f[k * slots + gn] = So.f[k * slots + go];
-				Sn.a[k * slots + gn] = So.a[k * slots + go];
-				Sn.cr[k * slots + gn] = So.cr[k * slots + go];
+				vv[k] = So.v[k * slots + go];
+				ff[k] = So.f[k * slots + go];
+				aa[k] = So.a[k * slots + go];
+				cr[k] = So.cr[k * slots + go];
+			}
+			const double mm = So.m[go];
+			const int og = So.orig[go];
+			Sn.pos[0][gn] = q;
+#pragma unroll
+			for (int k = 0; k < 3; k++) {
+				Sn.v[k * slots + gn] = vv[k];
+				Sn.f[k * slots + gn] = ff[k];
+				Sn.a[k * slots + gn] = aa[k];
+				Sn.cr[k * slots + gn] = cr[k];
 				Sn.cn[k * slots + gn] = 0;
 			}
 			Sn.xn[gn] = q.x;
 			Sn.xn[slots + gn] = q.y;
 			Sn.xn[2 * slots + gn] = q.z;
-			Sn.m[gn] = So.m[go];
-			Sn.orig[gn] = So.orig[go];
+			Sn.m[gn] = mm;
+			Sn.orig[gn] = og;
 		}
 	}
 }
@@ -343,21 +463,39 @@ __device__ __forceinline__ void sepb_build_chunk(
 	}
 }
 
+// Candidate q (< T) of the cell being built: its new index j, segment r
+// and unshifted position.
+__device__ __forceinline__ void sepb_fetch(const sepb_build_lds &L, const sepk_pos *pn, int nseg,
+					   int T, int q, int &j, int &r, sepk_pos &p)
+{
+	if (q >= T)
+		return;
+	r = 0; // the last segment starting at or before q
+	for (int step = 16; step > 0; step >>= 1)
+		if (r + step < nseg && L.segpre[r + step] <= q)
+			r += step;
+	j = L.segjs[r] + (q - L.segpre[r]);
+	p = pn[j];
+}
+
 __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 {
 	__shared__ sepb_build_lds Lall[SEPB_BLOCK / 64];
 	const int lane = threadIdx.x & 63;
 	sepb_build_lds &L = Lall[threadIdx.x >> 6];
+	__shared__ int fpre[SEPB_FMAP + 1];
+	const bool mapped = b.nsys <= SEPB_FMAP;
+	long total = b.ncell_tot;
+	if (mapped) {
+		sepb_flag_prefix(b, fpre);
+		total = fpre[b.nsys];
+	}
 	const long nw = ((long)gridDim.x * blockDim.x) >> 6;
-	for (long gc = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; gc < b.ncell_tot;
-	     gc += nw) {
-		const int s = b.cell_sys[gc];
-		if (s < 0)
+	for (long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < total; w += nw) {
+		int s, c;
+		if (!sepb_work_cell(b, fpre, mapped, w, s, c))
 			continue;
 		sepb_sys *y = b.sys + s;
-		if (!y->neighb_flag)
-			continue;
-		const int c = (int)(gc - y->cell_off);
 		const int nx = y->ncell[0], ny = y->ncell[1], nz = y->ncell[2];
 		const int cxi = c % nx, cyi = (c / nx) % ny, czi = c / (nx * ny);
 		const long off = y->off;
@@ -413,28 +551,22 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 				hyr = q.y;
 				hzr = q.z;
 			}
+			// the lane's candidate of the next chunk is fetched while the
+			// current chunk is tested (its segment by a binary search of
+			// the prefix sums; the image shift is added when it is used)
+			int jn = 0, rn = 0;
+			sepk_pos pnx = {0.0, 0.0, 0.0, 0.0};
+			sepb_fetch(L, pn, nseg, T, lane, jn, rn, pnx);
 			for (int q0 = 0; q0 < T; q0 += 64) {
-				const int q = q0 + lane;
-				const bool valid = q < T;
-				int j = 0;
-				double cx = 0.0, cy = 0.0, cz = 0.0;
-				if (valid) {
-					// the segment holding candidate q: the last one
-					// starting at or before q
-					int r = 0;
-					for (int step = 16; step > 0; step >>= 1)
-						if (r + step < nseg && L.segpre[r + step] <= q)
-							r += step;
-					j = L.segjs[r] + (q - L.segpre[r]);
-					const sepk_pos cq = pn[j];
-					cx = cq.x;
-					cy = cq.y;
-					cz = cq.z;
-					if (shift) {
-						cx += L.segsh[r][0];
-						cy += L.segsh[r][1];
-						cz += L.segsh[r][2];
-					}
+				const bool valid = q0 + lane < T;
+				const int j = jn, r = rn;
+				double cx = pnx.x, cy = pnx.y, cz = pnx.z;
+				if (q0 + 64 < T)
+					sepb_fetch(L, pn, nseg, T, q0 + 64 + lane, jn, rn, pnx);
+				if (shift && valid) {
+					cx += L.segsh[r][0];
+					cy += L.segsh[r][1];
+					cz += L.segsh[r][2];
 				}
 				if (shift)
 					sepb_build_chunk<true>(hn, h0 + hb, hxr, hyr, hzr, hc, valid, j, cx, cy,
