@@ -128,3 +128,35 @@ def test_position_beyond_grid_is_reported(product):
         lib.sep_gpu_clear_error()
     finally:
         ga.close()
+
+
+def test_lj_1m_wide_list_slots(product, oracle):
+    """N = 1048576 (64^3 fcc cells): npad >= 2^20, so the list build writes
+    its entries through the 64-bit slot arithmetic instead of the 32-bit one
+    (24-bit multiply) that every smaller system takes.  The pair set is
+    bit-exact against the oracle, FP64 forces, energy and virial within
+    1e-12, and the mixed form within 1e-5 of them."""
+    w = W.lj_fcc(64, 0.8442, 1.0, seed=3)
+    n = len(w["m"])
+    d = (W.uniform(15, 3 * n).reshape(n, 3) - 0.5) * 0.2
+    w["x"] = np.mod(w["x"] + d, w["L"])
+    assert n == 1 << 20
+    # host half-list room: a half-shell list gives some atoms up to the
+    # full-list length (about 80 here)
+    kw = dict(cf=2.5, dt=0.005, skin=0.3, nsubbox=(38, 38, 38), nneighb=128)
+    ga = SepSystem.from_workload(product, w, **kw)
+    orc = SepSystem.from_workload(oracle, w, **kw)
+    try:
+        _pair(ga, orc, types="AA", cf=2.5)
+        npairs = _check_pairs(ga, orc, oracle)
+        assert 36 * n < npairs < 42 * n
+        _check_forces(ga, orc)
+        f64 = np.array(ga.view["f"])
+        ga.gpu_precision(S.SEP_GPU_MIXED)
+        ga.reset_retval()
+        ga.reset_force()
+        ga.force_pairs("AA", 2.5)
+        assert force_error(np.array(ga.view["f"]), f64) <= 1e-5
+    finally:
+        ga.close()
+        orc.close()
