@@ -245,14 +245,6 @@ __global__ __launch_bounds__(SEPB_BIN_THREADS) void k_bbin(sepb_view b)
 	}
 }
 
-// A lane's double as a uniform value.
-__device__ __forceinline__ double sepb_readlane(double v, int l)
-{
-	const long long bits = __double_as_longlong(v);
-	const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)bits, l);
-	const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(bits >> 32), l);
-	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
 
 // Rank of `key` among the original indices of the m members seg[0..m) of a
 // cell (all distinct), by the whole wave: the members' new order within the
@@ -419,70 +411,104 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bsort(sepb_view b)
 	}
 }
 
-// Per wave: the candidate segments of the cell being built (one per
-// distinct neighbour cell: first new index, prefix of lengths, and the
-// image shift that brings the cell next to the home cell).
+// One workgroup per cell of a flagged system (the flagged cells' work
+// list).  Wave 0 lays out the candidate segments (one per distinct
+// neighbour cell: first new index, prefix of lengths, and the image shift
+// that brings the cell next to the home cell); every thread then stages
+// candidates -- position with the shift added, new index -- in LDS, up to
+// SEPB_BCHUNK at a time, and the home atoms beside them.  Test pass: wave w
+// owns home atoms w, w + 4, ... and tests 64 register-held candidates per
+// round against each, keeping the ballot in LDS.  Emit pass: one thread per
+// (home atom, 64-candidate word) finds the word's first entry number from
+// the atom's running length and the earlier words' popcounts and writes
+// the set bits in order, so every list is in candidate order.
+#define SEPB_BCHUNK 640
+#define SEPB_BWORDS (SEPB_BCHUNK / 64)
+#define SEPB_BHOME 64
+#define SEPB_BGRID 1280 // build workgroups (5 per CU)
+static_assert(SEPB_BCHUNK % 64 == 0, "candidates are tested 64 at a time");
+
 struct sepb_build_lds {
+	double cx[SEPB_BCHUNK], cy[SEPB_BCHUNK], cz[SEPB_BCHUNK];
+	int cj[SEPB_BCHUNK];
+	double hx[SEPB_BHOME], hy[SEPB_BHOME], hz[SEPB_BHOME];
+	unsigned long long vote[SEPB_BHOME][SEPB_BWORDS];
+	int hcnt[2][SEPB_BHOME]; // list lengths before / after the chunk
 	int segjs[27], segpre[28];
 	double segsh[27][3];
+	int wmax[SEPB_BLOCK / 64];
 };
 
-// The candidate loop of one run of home atoms against 64 candidates held
-// in the lanes.  SHIFT: the candidates carry their segment's image shift
-// and r^2 needs no wrap (cells >= 3 per axis, every atom inside the box);
-// otherwise the single-image wrap of every difference.
+// Tests the staged candidates [q0, q0 + 64) of a chunk of kn against the
+// wave's home atoms (hbase + i for i = wid, wid + 4, ... < hn) and keeps
+// each ballot in L.vote[i][q0 / 64].  SHIFT: the staged positions carry
+// their segment's image shift and r^2 needs no wrap; otherwise the
+// single-image wrap of every difference.
 template <bool SHIFT>
-__device__ __forceinline__ void sepb_build_chunk(
-	int hn, int h, double hxr, double hyr, double hzr, int &hc, bool valid, int j, double cx,
-	double cy, double cz, double L0, double L1, double L2, double rl2, int cap, int lg,
-	int sub, int lstride, int *lst)
+__device__ __forceinline__ void sepb_build_test(sepb_build_lds &L, int q0, int kn, int hbase,
+						int hn, double L0, double L1, double L2,
+						double rl2)
 {
 	const int lane = threadIdx.x & 63;
-	const unsigned long long below = (1ull << lane) - 1ull;
-	for (int i = 0; i < hn; i++) {
-		double ddx = sepb_readlane(hxr, i) - cx;
-		double ddy = sepb_readlane(hyr, i) - cy;
-		double ddz = sepb_readlane(hzr, i) - cz;
+	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
+	const int q = min(q0 + lane, kn - 1);
+	const bool valid = q0 + lane < kn;
+	const double cx = L.cx[q], cy = L.cy[q], cz = L.cz[q];
+	const int j = L.cj[q];
+	unsigned vlo = 0u, vhi = 0u; // lane kk keeps the ballot of the kk-th atom
+	int kk = 0;
+	for (int i = wid; i < hn; i += SEPB_BLOCK / 64, kk++) {
+		double ddx = L.hx[i] - cx;
+		double ddy = L.hy[i] - cy;
+		double ddz = L.hz[i] - cz;
 		if (!SHIFT) {
 			ddx = sepk_wrap(ddx, L0);
 			ddy = sepk_wrap(ddy, L1);
 			ddz = sepk_wrap(ddz, L2);
 		}
 		const double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-		const int hi = h + i;
-		const bool hit = valid && r2 < rl2 && j != hi;
-		const unsigned long long bal = __ballot(hit);
-		if (bal) {
-			const int base = __builtin_amdgcn_readlane(hc, i);
-			const int e = base + __popcll(bal & below);
-			if (hit && e < cap)
-				lst[(e >> lg) * lstride + hi * sub + (e & (sub - 1))] = j;
-			if (lane == i)
-				hc = base + __popcll(bal);
+		const bool hit = valid & (r2 < rl2) & (j != hbase + i);
+		const unsigned long long vote = __ballot(hit);
+		if (lane == kk) {
+			vlo = (unsigned)vote;
+			vhi = (unsigned)(vote >> 32);
 		}
 	}
+	const int i = wid + (SEPB_BLOCK / 64) * lane;
+	if (lane < kk && i < hn)
+		L.vote[i][q0 / 64] = ((unsigned long long)vhi << 32) | vlo;
 }
 
-// Candidate q (< T) of the cell being built: its new index j, segment r
-// and unshifted position.
-__device__ __forceinline__ void sepb_fetch(const sepb_build_lds &L, const sepk_pos *pn, int nseg,
-					   int T, int q, int &j, int &r, sepk_pos &p)
+// Writes the accepted candidates of a chunk of kn to the lists of home
+// atoms hbase .. hbase + hn - 1 (list slot of entry e of atom h:
+// (e >> lg) * lstride + h * sub + (e & (sub - 1))).  Lengths before the
+// chunk are in L.hcnt[par]; the last word of each atom leaves the length
+// after it in L.hcnt[par ^ 1].  Entries at or beyond cap are counted only.
+__device__ __forceinline__ void sepb_build_emit(sepb_build_lds &L, int kn, int hn, int par,
+						int hbase, int sub, int lg, int lstride, int cap,
+						int *lst)
 {
-	if (q >= T)
-		return;
-	r = 0; // the last segment starting at or before q
-	for (int step = 16; step > 0; step >>= 1)
-		if (r + step < nseg && L.segpre[r + step] <= q)
-			r += step;
-	j = L.segjs[r] + (q - L.segpre[r]);
-	p = pn[j];
+	const int nw = (kn + 63) / 64;
+	for (int it = threadIdx.x; it < hn * nw; it += SEPB_BLOCK) {
+		const int i = it / nw, w = it - i * nw;
+		int e = L.hcnt[par][i];
+		for (int u = 0; u < w; u++)
+			e += __popcll(L.vote[i][u]);
+		unsigned long long m = L.vote[i][w];
+		if (w == nw - 1)
+			L.hcnt[par ^ 1][i] = e + __popcll(m);
+		const int h = hbase + i;
+		const int *cj = L.cj + w * 64;
+		for (; m; m &= m - 1, e++)
+			if (e < cap)
+				lst[(e >> lg) * lstride + h * sub + (e & (sub - 1))] =
+					cj[__builtin_ctzll(m)];
+	}
 }
 
 __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 {
-	__shared__ sepb_build_lds Lall[SEPB_BLOCK / 64];
-	const int lane = threadIdx.x & 63;
-	sepb_build_lds &L = Lall[threadIdx.x >> 6];
+	__shared__ sepb_build_lds L;
 	__shared__ int fpre[SEPB_FMAP + 1];
 	const bool mapped = b.nsys <= SEPB_FMAP;
 	long total = b.ncell_tot;
@@ -490,14 +516,14 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 		sepb_flag_prefix(b, fpre);
 		total = fpre[b.nsys];
 	}
-	const long nw = ((long)gridDim.x * blockDim.x) >> 6;
-	for (long w = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6; w < total; w += nw) {
+	const int lane = threadIdx.x & 63;
+	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
+	for (long w = blockIdx.x; w < total; w += gridDim.x) {
 		int s, c;
 		if (!sepb_work_cell(b, fpre, mapped, w, s, c))
-			continue;
+			continue; // the same for the whole workgroup
 		sepb_sys *y = b.sys + s;
 		const int nx = y->ncell[0], ny = y->ncell[1], nz = y->ncell[2];
-		const int cxi = c % nx, cyi = (c / nx) % ny, czi = c / (nx * ny);
 		const long off = y->off;
 		const int n = y->n, npad = y->npad, sub = y->sub, cap = b.cap;
 		const int lg = __ffs(sub) - 1;
@@ -516,71 +542,102 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 		const int nox = sepb_offsets(nx, ox), noy = sepb_offsets(ny, oy),
 			  noz = sepb_offsets(nz, oz);
 		const int nseg = nox * noy * noz;
-		int len = 0;
-		if (lane < nseg) {
-			const int ux = cxi + ox[lane % nox], uy = cyi + oy[(lane / nox) % noy],
-				  uz = czi + oz[lane / (nox * noy)];
-			const int c2 = sepb_mod(ux, nx) + nx * (sepb_mod(uy, ny) + ny * sepb_mod(uz, nz));
-			L.segjs[lane] = st[c2];
-			L.segsh[lane][0] = ux >= nx ? L0 : (ux < 0 ? -L0 : 0.0);
-			L.segsh[lane][1] = uy >= ny ? L1 : (uy < 0 ? -L1 : 0.0);
-			L.segsh[lane][2] = uz >= nz ? L2 : (uz < 0 ? -L2 : 0.0);
-			len = st[c2 + 1] - st[c2];
-		}
-		int incl = len;
-		for (int o = 1; o < 64; o <<= 1) {
-			const int u = __shfl_up(incl, o, 64);
-			if (lane >= o)
-				incl += u;
-		}
-		if (lane < nseg)
-			L.segpre[lane + 1] = incl;
-		if (lane == 0)
-			L.segpre[0] = 0;
-		const int T = __shfl(incl, nseg - 1, 64);
-		__builtin_amdgcn_wave_barrier();
-		for (int hb = 0; hb < m; hb += 64) {
-			const int hn = min(m - hb, 64);
-			// lane i holds home atom i's position and list length; the
-			// candidate loop reads them with readlane (uniform)
-			double hxr = 0.0, hyr = 0.0, hzr = 0.0;
-			int hc = 0;
-			if (lane < hn) {
-				const sepk_pos q = pn[h0 + hb + lane];
-				hxr = q.x;
-				hyr = q.y;
-				hzr = q.z;
+		__syncthreads(); // the previous cell's readers are done with the LDS
+		if (wid == 0) {
+			int len = 0;
+			if (lane < nseg) {
+				const int cxi = c % nx, cyi = (c / nx) % ny, czi = c / (nx * ny);
+				const int ux = cxi + ox[lane % nox], uy = cyi + oy[(lane / nox) % noy],
+					  uz = czi + oz[lane / (nox * noy)];
+				const int c2 =
+					sepb_mod(ux, nx) + nx * (sepb_mod(uy, ny) + ny * sepb_mod(uz, nz));
+				L.segjs[lane] = st[c2];
+				L.segsh[lane][0] = ux >= nx ? L0 : (ux < 0 ? -L0 : 0.0);
+				L.segsh[lane][1] = uy >= ny ? L1 : (uy < 0 ? -L1 : 0.0);
+				L.segsh[lane][2] = uz >= nz ? L2 : (uz < 0 ? -L2 : 0.0);
+				len = st[c2 + 1] - st[c2];
 			}
-			// the lane's candidate of the next chunk is fetched while the
-			// current chunk is tested (its segment by a binary search of
-			// the prefix sums; the image shift is added when it is used)
-			int jn = 0, rn = 0;
-			sepk_pos pnx = {0.0, 0.0, 0.0, 0.0};
-			sepb_fetch(L, pn, nseg, T, lane, jn, rn, pnx);
-			for (int q0 = 0; q0 < T; q0 += 64) {
-				const bool valid = q0 + lane < T;
-				const int j = jn, r = rn;
-				double cx = pnx.x, cy = pnx.y, cz = pnx.z;
-				if (q0 + 64 < T)
-					sepb_fetch(L, pn, nseg, T, q0 + 64 + lane, jn, rn, pnx);
-				if (shift && valid) {
-					cx += L.segsh[r][0];
-					cy += L.segsh[r][1];
-					cz += L.segsh[r][2];
+			int incl = len;
+			for (int o = 1; o < 64; o <<= 1) {
+				const int u = __shfl_up(incl, o, 64);
+				if (lane >= o)
+					incl += u;
+			}
+			if (lane < nseg)
+				L.segpre[lane + 1] = incl;
+			if (lane == 0)
+				L.segpre[0] = 0;
+		}
+		__syncthreads();
+		const int T = L.segpre[nseg];
+		int full = 0;
+		for (int hb = 0; hb < m; hb += SEPB_BHOME) {
+			const int hn = min(m - hb, SEPB_BHOME);
+			if (threadIdx.x < SEPB_BHOME)
+				L.hcnt[0][threadIdx.x] = 0;
+			int par = 0;
+			for (int k0 = 0; k0 < T; k0 += SEPB_BCHUNK, par ^= 1) {
+				const int kn = min(T - k0, SEPB_BCHUNK);
+				if (k0 > 0)
+					__syncthreads(); // the previous chunk is no longer read
+				// stage [k0, k0 + kn): thread x takes x, x + SEPB_BLOCK, ...;
+				// its loads are issued before any LDS store
+				constexpr int PER = (SEPB_BCHUNK + SEPB_BLOCK - 1) / SEPB_BLOCK;
+				int jq[PER], rq[PER];
+				sepk_pos pq[PER];
+#pragma unroll
+				for (int r = 0; r < PER; r++) {
+					const int q = threadIdx.x + r * SEPB_BLOCK;
+					const int k = k0 + q;
+					int o = 0; // the last segment starting at or before k
+#pragma unroll
+					for (int step = 16; step > 0; step >>= 1)
+						if (o + step < nseg && L.segpre[o + step] <= k)
+							o += step;
+					rq[r] = q < kn ? o : -1;
+					jq[r] = L.segjs[o] + (k - L.segpre[o]);
 				}
-				if (shift)
-					sepb_build_chunk<true>(hn, h0 + hb, hxr, hyr, hzr, hc, valid, j, cx, cy,
-							       cz, L0, L1, L2, rl2, cap, lg, sub, lstride, lst);
-				else
-					sepb_build_chunk<false>(hn, h0 + hb, hxr, hyr, hzr, hc, valid, j, cx,
-								cy, cz, L0, L1, L2, rl2, cap, lg, sub, lstride,
-								lst);
+#pragma unroll
+				for (int r = 0; r < PER; r++)
+					if (rq[r] >= 0)
+						pq[r] = pn[jq[r]];
+#pragma unroll
+				for (int r = 0; r < PER; r++) {
+					if (rq[r] < 0)
+						continue;
+					const int q = threadIdx.x + r * SEPB_BLOCK;
+					double px = pq[r].x, py = pq[r].y, pz = pq[r].z;
+					if (shift) {
+						px += L.segsh[rq[r]][0];
+						py += L.segsh[rq[r]][1];
+						pz += L.segsh[rq[r]][2];
+					}
+					L.cx[q] = px;
+					L.cy[q] = py;
+					L.cz[q] = pz;
+					L.cj[q] = jq[r];
+				}
+				if (k0 == 0 && threadIdx.x < hn) {
+					const sepk_pos q = pn[h0 + hb + threadIdx.x];
+					L.hx[threadIdx.x] = q.x;
+					L.hy[threadIdx.x] = q.y;
+					L.hz[threadIdx.x] = q.z;
+				}
+				__syncthreads();
+				for (int q0 = 0; q0 < kn; q0 += 64) {
+					if (shift)
+						sepb_build_test<true>(L, q0, kn, h0 + hb, hn, L0, L1, L2, rl2);
+					else
+						sepb_build_test<false>(L, q0, kn, h0 + hb, hn, L0, L1, L2, rl2);
+				}
+				__syncthreads();
+				sepb_build_emit(L, kn, hn, par, h0 + hb, sub, lg, lstride, cap, lst);
 			}
-			int full = 0;
-			if (lane < hn) {
-				const int h = h0 + hb + lane;
-				int cnt = hc;
-				full = cnt;
+			__syncthreads();
+			if (threadIdx.x < hn) {
+				const int h = h0 + hb + threadIdx.x;
+				int cnt = L.hcnt[par][threadIdx.x];
+				full = max(full, cnt);
 				if (cnt > cap) {
 					atomicCAS(&y->error, 0, SEP_GPU_ENEIGHB);
 					cnt = cap;
@@ -592,13 +649,22 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
 					lst[(e >> lg) * lstride + h * sub + (e & (sub - 1))] = n;
 				b.nbr_count[off + h] = cnt;
 			}
-			// one atomic per wave: every home atom of a system hitting
-			// one address would serialise in L2
-			for (int o = 32; o > 0; o >>= 1)
-				full = max(full, __shfl_xor(full, o, 64));
-			if (lane == 0 && full > 0)
-				atomicMax(&y->max_full, full);
-			__builtin_amdgcn_wave_barrier();
+			if (hb + SEPB_BHOME < m)
+				__syncthreads(); // the next batch restages the candidates
+		}
+		// one atomic per cell: every home atom of a system hitting one
+		// address would serialise in L2
+		for (int o = 32; o > 0; o >>= 1)
+			full = max(full, __shfl_xor(full, o, 64));
+		if (lane == 0)
+			L.wmax[wid] = full;
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			int mx = 0;
+			for (int v = 0; v < SEPB_BLOCK / 64; v++)
+				mx = max(mx, L.wmax[v]);
+			if (mx > 0)
+				atomicMax(&y->max_full, mx);
 		}
 	}
 }
@@ -1189,7 +1255,7 @@ static int launch_rebuild(sepdev_batch *b)
 	SEPK_LAUNCH_CHECK();
 	k_bsort<<<512, SEPB_BLOCK, 0, b->st>>>(v);
 	SEPK_LAUNCH_CHECK();
-	k_bbuild<<<1024, SEPB_BLOCK, 0, b->st>>>(v);
+	k_bbuild<<<SEPB_BGRID, SEPB_BLOCK, 0, b->st>>>(v);
 	SEPK_LAUNCH_CHECK();
 	return 0;
 }
