@@ -131,15 +131,20 @@ __device__ __forceinline__ int sepb_mod(int a, int n)
 #define SEPB_MAX_CELLS 8192 // cells per system (LDS counters)
 #define SEPB_BIN_UNROLL 8
 
-// Cell of position q in system y (floor and periodic wrap); an atom outside
-// [0, L) on some axis (x == L after a wrap, or a starting position) sets
-// `outside`, which rules out the per-segment image shifts of the build.
-__device__ __forceinline__ int sepb_cell(sepb_sys *y, const sepk_pos &q, int &outside)
+// Cell of position q in system y (floor of x / l, by the reciprocal edge il,
+// and periodic wrap); an atom outside [0, L) on some axis (x == L after a
+// wrap, or a starting position) sets `outside`, which rules out the
+// per-segment image shifts of the build.  Rounding of x il can put an atom
+// within an ulp of a cell face into the next cell; its partners across the
+// face are then at most one cell away, and only a pair of two such atoms
+// at nearly the list cutoff (beyond the force cutoff) could be missed.
+__device__ __forceinline__ int sepb_cell(sepb_sys *y, const double (&il)[3], const sepk_pos &q,
+					 int &outside)
 {
 	const double xs[3] = {q.x, q.y, q.z};
 	int ci[3];
 	for (int k = 0; k < 3; k++) {
-		const double w = xs[k] / y->lcell[k];
+		const double w = xs[k] * il[k];
 		if (!(fabs(w) < 1.0e9)) {
 			atomicCAS(&y->error, 0, SEP_GPU_ECELL);
 			ci[k] = 0;
@@ -174,6 +179,7 @@ __global__ __launch_bounds__(SEPB_BIN_THREADS) void k_bbin(sepb_view b)
 		outside = 0;
 	__syncthreads();
 	const sepk_pos *pos = b.st[y->cur].pos[y->p] + off;
+	const double il[3] = {1.0 / y->lcell[0], 1.0 / y->lcell[1], 1.0 / y->lcell[2]};
 	// SEPB_BIN_UNROLL positions per thread are loaded before any is
 	// binned, so a large system pays a few memory round trips, not one
 	// per SEPB_BIN_THREADS atoms
@@ -189,7 +195,7 @@ __global__ __launch_bounds__(SEPB_BIN_THREADS) void k_bbin(sepb_view b)
 		for (int u = 0; u < SEPB_BIN_UNROLL; u++) {
 			const int t = t0 + u * SEPB_BIN_THREADS;
 			if (t < n) {
-				const int c = sepb_cell(y, qq[u], outside);
+				const int c = sepb_cell(y, il, qq[u], outside);
 				b.cellid[off + t] = c;
 				b.cslot[off + t] = atomicAdd(&cnt[c], 1);
 			}
