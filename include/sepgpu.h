@@ -114,6 +114,25 @@ int sep_gpu_lj_adjoint(seppart *atoms, sepsys *sys, int ntypes, const char *type
 		       const double *params, const double *dLdF, double *dLdx,
 		       double *dLdtheta);
 
+/* ---- per-step record --------------------------------------------------- */
+/* No counterpart in the reference, whose samplers read sepret on the host
+ * after every step (the stress autocorrelation of source/sepsampler.c:
+ * 280-351 reads pot_P and kin_P each step).  With a record of `capacity`
+ * steps, every leapfrog of these atoms appends its sepret's epot, ecoul,
+ * ekin, pot_P, kin_P and pot_P_bond (the values the reference's
+ * sep_leapfrog call would leave in that sepret) and sys->tnow to a device
+ * ring, inside captured graphs too, so a resident run needs no host sync
+ * per step.  capacity 0 stops recording.  Allocates: call it outside
+ * sep_gpu_run. */
+int sep_gpu_record_steps(seppart *atoms, long capacity);
+/* Copies the recorded steps still held, oldest first, into rets[0..*n)
+ * (only the fields listed above; others are zero) and tnow[0..*n) (either
+ * may be NULL), and empties the record.  max (the room in each array) must
+ * be at least the capacity; *lost receives the number of steps overwritten
+ * since the last read. */
+int sep_gpu_read_steps(seppart *atoms, sepret *rets, double *tnow, long max, long *n,
+		       long *lost);
+
 /* ---- diagnostics ----------------------------------------------------- */
 /* Unordered neighbour pairs (i < j, original atom indices) of the current
  * device list, written as pairs[2k], pairs[2k+1]; *npairs receives the

@@ -197,7 +197,7 @@ extern "C" void sepdev_destroy(sepdev *d)
 	free_state(&d->s[1]);
 	void *ptrs[] = {d->iperm, d->cell_count, d->cell_start,
 			d->cellid_old, d->order_tmp, d->cell_slot,
-			d->nbr, d->nbr_count, d->partials, d->fin_l2, d->fin_count, d->ret,
+			d->nbr, d->nbr_count, d->partials, d->fin_l2, d->fin_count, d->ring, d->ring_n, d->ret,
 			d->scal, d->sys, d->dstage, d->blist, d->alist, d->dlist,
 			d->bcontrib, d->acontrib, d->dcontrib, d->bcsr_off,
 			d->bcsr_idx, d->acsr_off, d->acsr_idx, d->dcsr_off,

@@ -904,6 +904,11 @@ static int queue_lf_task(sepdev *d, int nh, double *alpha, unsigned nb, double d
 	f.skin = skin;
 	f.dt = dt;
 	sepk_task_ret(d, f, ret_slot);
+	if (d->ring) {
+		f.ring = d->ring;
+		f.ring_cap = d->ring_cap;
+		f.ring_n = d->ring_n;
+	}
 	// one finisher launch for the whole step: any queued build and pair
 	// results, then the leapfrog's
 	int rc;

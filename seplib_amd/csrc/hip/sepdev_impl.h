@@ -83,6 +83,11 @@ struct sepk_task {
 	int reset_max;    // LF: max_dist2 was reset since the last leapfrog
 	double skin, dt;
 	int conditional;  // BUILD: only when the rebuild flag is set
+	// LF: the per-step record (null: none); entry (*ring_n % ring_cap)
+	// receives the return slot's words and tnow, then *ring_n counts it
+	double *ring;
+	long ring_cap;
+	unsigned long long *ring_n;
 };
 
 struct sepk_fin {
@@ -130,6 +135,9 @@ struct sepdev {
 	int npending;
 	double *fin_l2;
 	unsigned *fin_count;
+	double *ring;                // per-step record (SEPDEV_RING_WORDS each)
+	long ring_cap;
+	unsigned long long *ring_n;  // steps recorded since the last read
 
 	// return and scalar slots
 	double *ret;

@@ -197,6 +197,21 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, 
 		if (rmap[u] == u)
 			f.t[u].ret[word] = sret[u][word];
 	}
+	// the per-step record: a leapfrog's return slot and the new tnow
+	for (int u = 0; u < f.ntask; u++) {
+		const sepk_task &k = f.t[u];
+		if (k.mode != SEPK_FIN_LF || !k.ring || rmap[u] < 0)
+			continue;
+		const unsigned long long at = *k.ring_n;
+		double *rec = k.ring + (size_t)(at % (unsigned long long)k.ring_cap) * SEPDEV_RING_WORDS;
+		if (threadIdx.x < SEPDEV_RET_WORDS)
+			rec[threadIdx.x] = sret[rmap[u]][threadIdx.x];
+		else if (threadIdx.x == SEPDEV_RET_WORDS)
+			rec[SEPDEV_RET_WORDS] = ssys.tnow;
+		__syncthreads(); // every thread has read the count
+		if (threadIdx.x == 0)
+			*k.ring_n = at + 1;
+	}
 	if (threadIdx.x < f.ntask && amap[threadIdx.x] == (int)threadIdx.x)
 		*f.t[threadIdx.x].alpha = salpha[threadIdx.x];
 	if (threadIdx.x >= 64 && threadIdx.x < 64 + nsys)
@@ -309,5 +324,68 @@ int sepk_flush_nh(sepdev *d)
 	k_nh_apply2<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad,
 								   d->scal + d->nh_slot);
 	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+// ---- per-step record -----------------------------------------------------------
+
+extern "C" int sepdev_ring_alloc(sepdev *d, long capacity)
+{
+	if (d->capturing)
+		return sepk_fail(SEP_GPU_ESTATE, "step record changed during graph capture");
+	if (capacity < 0)
+		return sepk_fail(SEP_GPU_EARG, "step record capacity %ld", capacity);
+	// finish queued work that may still write the old record, and drop a
+	// captured graph that holds its address
+	int rc = sepk_flush_tasks(d);
+	if (rc)
+		return rc;
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	sepdev_graph_drop(d);
+	(void)hipFree(d->ring);
+	(void)hipFree(d->ring_n);
+	d->ring = nullptr;
+	d->ring_n = nullptr;
+	d->ring_cap = 0;
+	if (capacity == 0)
+		return 0;
+	SEPK_CHECK(hipMalloc(&d->ring, (size_t)capacity * SEPDEV_RING_WORDS * sizeof(double)));
+	SEPK_CHECK(hipMalloc(&d->ring_n, sizeof(unsigned long long)));
+	SEPK_CHECK(hipMemsetAsync(d->ring_n, 0, sizeof(unsigned long long), d->st));
+	d->ring_cap = capacity;
+	return 0;
+}
+
+extern "C" int sepdev_ring_read(sepdev *d, double *words, long max, long *n, long *lost)
+{
+	*n = 0;
+	*lost = 0;
+	if (!d->ring)
+		return sepk_fail(SEP_GPU_ESTATE, "no step record (sep_gpu_record_steps)");
+	if (max < d->ring_cap)
+		return sepk_fail(SEP_GPU_EARG, "step record read of %ld < capacity %ld", max,
+				 d->ring_cap);
+	int rc = sepk_flush_tasks(d);
+	if (rc)
+		return rc;
+	unsigned long long cnt = 0;
+	SEPK_CHECK(hipMemcpyAsync(&cnt, d->ring_n, sizeof cnt, hipMemcpyDeviceToHost, d->st));
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	const unsigned long long cap = (unsigned long long)d->ring_cap;
+	const unsigned long long held = cnt < cap ? cnt : cap;
+	const size_t rec = SEPDEV_RING_WORDS * sizeof(double);
+	// oldest first: entries (cnt - held) .. cnt - 1, modulo the capacity
+	const unsigned long long first = (cnt - held) % cap;
+	const unsigned long long tail = held < cap - first ? held : cap - first;
+	if (tail)
+		SEPK_CHECK(hipMemcpyAsync(words, d->ring + first * SEPDEV_RING_WORDS, tail * rec,
+					  hipMemcpyDeviceToHost, d->st));
+	if (held > tail)
+		SEPK_CHECK(hipMemcpyAsync(words + tail * SEPDEV_RING_WORDS, d->ring,
+					  (held - tail) * rec, hipMemcpyDeviceToHost, d->st));
+	SEPK_CHECK(hipMemsetAsync(d->ring_n, 0, sizeof(unsigned long long), d->st));
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	*n = (long)held;
+	*lost = (long)(cnt - held);
 	return 0;
 }

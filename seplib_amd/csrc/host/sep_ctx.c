@@ -352,7 +352,7 @@ static void ret_pack(const sepret *r, double *w)
 		}
 }
 
-static void ret_unpack(sepret *r, const double *w)
+void seph_ret_unpack(sepret *r, const double *w)
 {
 	r->epot = w[SEPDEV_RET_EPOT];
 	r->ecoul = w[SEPDEV_RET_ECOUL];
@@ -393,7 +393,7 @@ int seph_get_all(seph_ctx *c, sepsys *sys)
 		double w[SEPDEV_RET_WORDS];
 		if ((rc = seph_dev_fail(sepdev_ret_get(c->dev, c->rets[b].slot, w))))
 			return rc;
-		ret_unpack((sepret *)c->rets[b].host, w);
+		seph_ret_unpack((sepret *)c->rets[b].host, w);
 	}
 	for (int b = 0; b < c->nalphas; b++)
 		if ((rc = seph_dev_fail(sepdev_scalar_get(c->dev, c->alphas[b].slot,

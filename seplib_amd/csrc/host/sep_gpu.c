@@ -263,6 +263,45 @@ int sep_gpu_list_stats(seppart *atoms, double *mean_full, int *max_full)
 	return seph_dev_fail(sepdev_list_stats(c->dev, mean_full, max_full));
 }
 
+/* ---- per-step record ---------------------------------------------------- */
+
+int sep_gpu_record_steps(seppart *atoms, long capacity)
+{
+	seph_ctx *c = seph_find(atoms);
+	if (!c)
+		return seph_fail(SEP_GPU_ESTATE, "no device context for these atoms");
+	return seph_dev_fail(sepdev_ring_alloc(c->dev, capacity));
+}
+
+int sep_gpu_read_steps(seppart *atoms, sepret *rets, double *tnow, long max, long *n,
+		       long *lost)
+{
+	if (!n || !lost)
+		return seph_fail(SEP_GPU_EARG, "read_steps: null count");
+	*n = 0;
+	*lost = 0;
+	seph_ctx *c = seph_find(atoms);
+	if (!c)
+		return seph_fail(SEP_GPU_ESTATE, "no device context for these atoms");
+	if (max <= 0)
+		return seph_fail(SEP_GPU_EARG, "read_steps: max %ld", max);
+	double *w = malloc((size_t)max * SEPDEV_RING_WORDS * sizeof(double));
+	if (!w)
+		return seph_fail(SEP_GPU_EHOST, "read_steps: out of host memory");
+	int rc = seph_dev_fail(sepdev_ring_read(c->dev, w, max, n, lost));
+	for (long k = 0; !rc && k < *n; k++) {
+		const double *rec = w + (size_t)k * SEPDEV_RING_WORDS;
+		if (rets) {
+			memset(&rets[k], 0, sizeof(sepret));
+			seph_ret_unpack(&rets[k], rec);
+		}
+		if (tnow)
+			tnow[k] = rec[SEPDEV_RET_WORDS];
+	}
+	free(w);
+	return rc;
+}
+
 int sep_gpu_synchronize(seppart *atoms)
 {
 	seph_ctx *c = seph_find(atoms);

@@ -82,6 +82,9 @@ int seph_get_f(seph_ctx *c);
 int seph_get_state(seph_ctx *c); /* x, v, a, cross counters */
 int seph_put_all(seph_ctx *c, const sepsys *sys);
 int seph_get_all(seph_ctx *c, sepsys *sys);
+/* The sepret fields of a device return slot (SEPDEV_RET_WORDS) into r;
+ * the fields the hot path does not write are left as they are. */
+void seph_ret_unpack(sepret *r, const double *w);
 int seph_check_device_error(seph_ctx *c);
 
 /* return/alpha slots */

@@ -96,6 +96,15 @@ int sepdev_ret_zero(sepdev *d, int slot);
 int sepdev_ret_get(sepdev *d, int slot, double *words);
 int sepdev_ret_put(sepdev *d, int slot, const double *words);
 /* scalar slots (Nose-Hoover alpha) */
+/* ---- per-step record: at every leapfrog, the leapfrog's return slot
+ * (SEPDEV_RET_WORDS) and then tnow are appended to a device ring of
+ * `capacity` steps.  capacity 0 frees the ring.  Reading copies the steps
+ * still held, oldest first (*n of them; *lost: steps overwritten since the
+ * last read), and empties the ring; max must be at least the capacity. */
+#define SEPDEV_RING_WORDS (SEPDEV_RET_WORDS + 1)
+int sepdev_ring_alloc(sepdev *d, long capacity);
+int sepdev_ring_read(sepdev *d, double *words, long max, long *n, long *lost);
+
 int sepdev_scalar_alloc(sepdev *d, double init);
 int sepdev_scalar_get(sepdev *d, int slot, double *v);
 int sepdev_scalar_put(sepdev *d, int slot, double v);

@@ -1,0 +1,97 @@
+"""The per-step record (sep_gpu_record_steps / sep_gpu_read_steps): each
+leapfrog's sepret energies and stress and sys->tnow, kept on the device so
+that a resident run needs no host sync per step.
+
+  * a resident graph run of 40 LJ NVT steps records exactly what the
+    synchronised API leaves in sepret and sys->tnow after each of the same
+    40 steps (bit for bit: the same kernels in the same order);
+  * reading empties the record;
+  * with a smaller capacity the record keeps the newest steps, in order,
+    and counts the overwritten ones;
+  * a read with less room than the capacity is refused.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from seplib_amd import sep as S
+from seplib_amd import workloads as W
+from seplib_amd.sim import SepSystem
+
+pytestmark = pytest.mark.gpu
+
+KW = dict(cf=2.5, dt=0.005, skin=0.5)
+
+
+def _workload():
+    return W.lj_sc(10, 0.8442, 0.728, seed=1)
+
+
+def _synchronised_steps(product, nsteps):
+    """sepret fields and tnow after each step of the synchronised API."""
+    s = SepSystem.from_workload(product, _workload(), **KW)
+    out = []
+    try:
+        for _ in range(nsteps):
+            s.reset_retval()
+            s.reset_force()
+            s.force_pairs("AA", 2.5)
+            s.nosehoover(1.0, 10.0)
+            s.leapfrog()
+            r = s.ret
+            out.append((r.epot, r.ecoul, r.ekin, S.mat(r.pot_P).copy(),
+                        S.mat(r.kin_P).copy(), s.sys.tnow))
+    finally:
+        s.close()
+    return out
+
+
+def _read(s, room):
+    rets = (S.sepret * room)()
+    tnow = np.zeros(room)
+    n, lost = C.c_long(0), C.c_long(0)
+    s.check(s._lib().sep_gpu_read_steps(s.atoms, C.cast(rets, C.c_void_p),
+                                        tnow.ctypes.data_as(C.POINTER(C.c_double)), room,
+                                        C.byref(n), C.byref(lost)))
+    return [rets[k] for k in range(n.value)], tnow[:n.value], lost.value
+
+
+def _same(rec, t, ref):
+    e, ec, ek, pp, kp, tn = ref
+    assert (rec.epot, rec.ecoul, rec.ekin) == (e, ec, ek)
+    assert np.array_equal(S.mat(rec.pot_P), pp)
+    assert np.array_equal(S.mat(rec.kin_P), kp)
+    assert t == tn
+
+
+def test_record_matches_synchronised_steps(product):
+    ref = _synchronised_steps(product, 80)
+    s = SepSystem.from_workload(product, _workload(), **KW)
+    lib = s._lib()
+    try:
+        s.gpu_resident(True)
+        ops = [s.op_pairs("AA", 2.5), s.op_nosehoover(1.0, 10.0), s.op_leapfrog()]
+        s.check(lib.sep_gpu_record_steps(s.atoms, 64))
+        s.gpu_run(ops, 40, graph=True)
+        rets, tnow, lost = _read(s, 64)
+        assert len(rets) == 40 and lost == 0
+        for k in range(40):
+            _same(rets[k], tnow[k], ref[k])
+        rets, tnow, lost = _read(s, 64)
+        assert len(rets) == 0 and lost == 0
+        # a smaller record keeps the newest 16 of the next 40 steps
+        s.check(lib.sep_gpu_record_steps(s.atoms, 16))
+        s.gpu_run(ops, 40, graph=True)
+        rets, tnow, lost = _read(s, 16)
+        assert len(rets) == 16 and lost == 24
+        for k in range(16):
+            _same(rets[k], tnow[k], ref[64 + k])
+        n, lost = C.c_long(0), C.c_long(0)
+        lib.sep_gpu_clear_error()
+        assert lib.sep_gpu_read_steps(s.atoms, None, None, 8, C.byref(n),
+                                      C.byref(lost)) == S.SEP_GPU_EARG
+        lib.sep_gpu_clear_error()
+        s.check(lib.sep_gpu_record_steps(s.atoms, 0))
+    finally:
+        s.close()
