@@ -58,6 +58,18 @@ void sep_set_skin(sepsys *sys, double value);
 void sep_save_xyz(seppart *ptr, const char *partnames, const char *file, char *mode,
 		  sepsys *sys);
 
+/* Raw binary dump of npart seppart records; a resident system is
+ * synchronised first.  Replaces include/sepmisc.h:144
+ * (source/sepmisc.c:319-330). */
+void sep_save(seppart *ptr, size_t npart, const char *file);
+
+/* Reads npart raw seppart records and allocates each atom a new neighbour
+ * array of nneighb ints (the pointers read from the file are replaced); a
+ * resident system's device state is replaced by the loaded atoms and its
+ * list is rebuilt at the next pair call.
+ * Replaces include/sepmisc.h:154 (source/sepmisc.c:333-354). */
+void sep_load(seppart *ptr, int nneighb, size_t npart, const char *file);
+
 /* Diagnostics printed as "sep-error -> ..." / "sep-warning -> ..." on
  * stdout.  sep_error does not exit (the reference's -DOCTAVE behaviour,
  * source/sepmisc.c:59-65); it records SEP_GPU_EHOST for sep_gpu_last_error.

@@ -93,6 +93,70 @@ void sep_close(seppart *ptr, size_t npart)
 	free(ptr);
 }
 
+/* Binary dump of the AoS, as the reference: npart raw seppart records
+ * (source/sepmisc.c:319-330).  A resident system is synchronised first. */
+void sep_save(seppart *ptr, size_t npart, const char *file)
+{
+	seph_ctx *c = seph_find(ptr);
+	if (c && c->resident && seph_get_atoms(c) != 0) {
+		sep_warning("sep_save: %s", (char *)sep_gpu_last_error_message());
+		return;
+	}
+	FILE *fout = fopen(file, "wb");
+	if (fout == NULL) {
+		sep_error("%s at line %d: Couldn't open the file: %s\n", __func__, __LINE__, file);
+		return;
+	}
+	fwrite(ptr, sizeof(seppart), npart, fout);
+	fclose(fout);
+}
+
+/* Reads npart raw seppart records into ptr and gives every atom a fresh
+ * neighbour array of nneighb ints (source/sepmisc.c:333-354); the pointer
+ * fields read from the file are overwritten.  A resident system's device
+ * state is replaced by the loaded atoms and its list is rebuilt at the next
+ * pair call (the reference leaves that to the caller's sys->neighb_flag). */
+void sep_load(seppart *ptr, int nneighb, size_t npart, const char *file)
+{
+	for (size_t n = 0; n < npart; n++)
+		free(ptr[n].neighb);
+	FILE *fin = fopen(file, "rb");
+	size_t got = 0;
+	if (fin == NULL)
+		sep_error("%s at line %d: Couldn't open the file: %s\n", __func__, __LINE__, file);
+	else {
+		got = fread(ptr, sizeof(seppart), npart, fin);
+		if (got == 0)
+			sep_error("%s at line %d: Error reading file %s\n", __func__, __LINE__, file);
+		fclose(fin);
+	}
+	for (size_t n = 0; n < npart; n++) {
+		ptr[n].neighb = malloc((nneighb > 0 ? nneighb : 1) * sizeof(int));
+		if (ptr[n].neighb == NULL) {
+			sep_error("%s at line %d: Couldn't allocate memory\n", __func__, __LINE__);
+			return;
+		}
+		ptr[n].neighb[0] = -1;
+	}
+	// half lists written back to these atoms must fit the new arrays
+	seph_set_nneighb(ptr, nneighb > 0 ? nneighb : 1);
+	// a resident system takes the loaded atoms, and its next pair call
+	// rebuilds the list for them
+	seph_ctx *c = seph_find(ptr);
+	if (got && c && c->resident) {
+		sepdev_sys ds;
+		int rc = seph_put_atoms(c);
+		if (!rc)
+			rc = seph_dev_fail(sepdev_get_sys(c->dev, &ds));
+		if (!rc) {
+			ds.neighb_flag = 1;
+			rc = seph_dev_fail(sepdev_put_sys(c->dev, &ds));
+		}
+		if (rc)
+			sep_warning("sep_load: %s", (char *)sep_gpu_last_error_message());
+	}
+}
+
 /* Text format "N\nLx Ly Lz\n" then "type x y z vx vy vz m q" per atom
  * (source/sepinit.c:69-109). */
 seppart *sep_init_xyz(double *lbox, int *npart, const char *file, char verbose)

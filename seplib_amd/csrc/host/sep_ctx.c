@@ -93,6 +93,16 @@ static void free_ctx(seph_ctx *c)
 	free(c);
 }
 
+void seph_set_nneighb(seppart *atoms, int nneighb)
+{
+	seph_entry *e = find_entry(atoms);
+	if (!e)
+		return;
+	e->nneighb = nneighb;
+	if (e->ctx)
+		e->ctx->nneighb = nneighb;
+}
+
 void seph_unregister(seppart *atoms)
 {
 	for (int i = 0; i < g_nreg; i++)
@@ -365,7 +375,9 @@ void seph_ret_unpack(sepret *r, const double *w)
 		}
 }
 
-int seph_get_all(seph_ctx *c, sepsys *sys)
+/* The per-atom state of a resident system back into the AoS; ds gets the
+ * device's system scalars. */
+static int get_atoms(seph_ctx *c, sepdev_sys *ds)
 {
 	int rc;
 	if ((rc = seph_get_state(c)) || (rc = seph_get_f(c)))
@@ -373,14 +385,9 @@ int seph_get_all(seph_ctx *c, sepsys *sys)
 	if ((rc = seph_dev_fail(sepdev_get_vec(c->dev, SEPDEV_XN, c->v3))))
 		return rc;
 	UNPACK3(xn);
-	sepdev_sys ds;
-	if ((rc = seph_dev_fail(sepdev_get_sys(c->dev, &ds))))
+	if ((rc = seph_dev_fail(sepdev_get_sys(c->dev, ds))))
 		return rc;
-	sys->max_dist2 = ds.max_dist2;
-	sys->tnow = ds.tnow;
-	sys->neighb_flag = ds.neighb_flag;
-	sys->nupdate_neighb = ds.nupdate_neighb;
-	if (ds.xn_pending) {
+	if (ds->xn_pending) {
 		/* the reference resets xn and cross_neighb right after the
 		 * leapfrog that raised the flag (source/sepintgr.c:72-84) */
 		for (long i = 0; i < c->n; i++)
@@ -389,6 +396,34 @@ int seph_get_all(seph_ctx *c, sepsys *sys)
 				c->atoms[i].cross_neighb[k] = 0;
 			}
 	}
+	return 0;
+}
+
+int seph_get_atoms(seph_ctx *c)
+{
+	sepdev_sys ds;
+	return get_atoms(c, &ds);
+}
+
+int seph_put_atoms(seph_ctx *c)
+{
+	int rc;
+	if ((rc = seph_put_x(c)) || (rc = seph_put_v(c)) || (rc = seph_put_f(c)) ||
+	    (rc = put_a(c)) || (rc = seph_put_xn_cross(c)) || (rc = seph_put_scalars(c)))
+		return rc;
+	return 0;
+}
+
+int seph_get_all(seph_ctx *c, sepsys *sys)
+{
+	int rc;
+	sepdev_sys ds;
+	if ((rc = get_atoms(c, &ds)))
+		return rc;
+	sys->max_dist2 = ds.max_dist2;
+	sys->tnow = ds.tnow;
+	sys->neighb_flag = ds.neighb_flag;
+	sys->nupdate_neighb = ds.nupdate_neighb;
 	for (int b = 0; b < c->nrets; b++) {
 		double w[SEPDEV_RET_WORDS];
 		if ((rc = seph_dev_fail(sepdev_ret_get(c->dev, c->rets[b].slot, w))))

@@ -62,6 +62,8 @@ typedef struct {
 /* registry */
 void seph_register(seppart *atoms, long n, int nneighb);
 void seph_unregister(seppart *atoms);
+/* The per-atom neighbour arrays were reallocated with nneighb ints. */
+void seph_set_nneighb(seppart *atoms, int nneighb);
 seph_ctx *seph_find(const seppart *atoms);
 /* context with a device; creates the device state on first use */
 seph_ctx *seph_ctx_for(seppart *atoms, const sepsys *sys);
@@ -82,6 +84,10 @@ int seph_get_f(seph_ctx *c);
 int seph_get_state(seph_ctx *c); /* x, v, a, cross counters */
 int seph_put_all(seph_ctx *c, const sepsys *sys);
 int seph_get_all(seph_ctx *c, sepsys *sys);
+/* The per-atom state only: device -> AoS (a resident system), and AoS ->
+ * device (x, v, f, a, xn and crossings, mass, charge, type, molecule). */
+int seph_get_atoms(seph_ctx *c);
+int seph_put_atoms(seph_ctx *c);
 /* The sepret fields of a device return slot (SEPDEV_RET_WORDS) into r;
  * the fields the hot path does not write are left as they are. */
 void seph_ret_unpack(sepret *r, const double *w);

@@ -238,6 +238,8 @@ def bind_sep_names(lib):
     f("sep_get_pressure", C.c_double, P_RET, P_SYS)
     f("sep_get_temperature", C.c_double, P_RET, P_SYS)
     f("sep_save_xyz", None, P_ATOMS, C.c_char_p, C.c_char_p, C.c_char_p, P_SYS)
+    f("sep_save", None, P_ATOMS, C.c_size_t, C.c_char_p)
+    f("sep_load", None, P_ATOMS, C.c_int, C.c_size_t, C.c_char_p)
 
 
 def fn_address(lib, name):

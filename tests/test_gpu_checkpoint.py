@@ -1,0 +1,81 @@
+"""sep_save / sep_load (source/sepmisc.c:319-354) on a resident system.
+
+  * sep_save after resident graph steps writes the device state: loaded
+    back into a host system, x, v, f, a, xn, cross_neighb and crossings
+    equal the synchronised AoS of the resident run bit for bit;
+  * sep_load into a resident system replaces its device state and rebuilds
+    its list: pair forces, energy and virial then equal those of a
+    synchronised system loaded from the same file to 1e-12;
+  * a load with smaller neighbour arrays than sep_init gave is respected
+    by the host half lists written afterwards (no overflow).
+"""
+import numpy as np
+import pytest
+
+from seplib_amd import sep as S
+from seplib_amd import workloads as W
+from seplib_amd.sim import SepSystem
+from tests.parity import FP64_TOL, force_error, rel, tensor_error
+
+pytestmark = pytest.mark.gpu
+
+KW = dict(cf=2.5, dt=0.005, skin=0.4)
+
+
+def test_resident_save_and_load(product, tmp_path):
+    w = W.lj_sc(10, 0.8442, 1.0, seed=6)
+    path = str(tmp_path / "ckpt.bin").encode()
+    lib = product.lib
+    a = SepSystem.from_workload(product, w, **KW)
+    b = SepSystem.from_workload(product, w, **KW)
+    c = SepSystem.from_workload(product, w, **KW)
+    try:
+        a.gpu_resident(True)
+        ops = [a.op_pairs("AA", 2.5), a.op_nosehoover(1.0, 10.0), a.op_leapfrog()]
+        a.gpu_run(ops, 30, graph=True)
+        lib.sep_save(a.atoms, a.n, path)
+        a.gpu_sync()
+        sa = a.state()
+        # the file holds the device state
+        lib.sep_load(c.atoms, S.SEP_NEIGHB, c.n, path)
+        sc = c.state()
+        for k in ("x", "v", "f", "a", "xn", "cross_neighb", "crossings"):
+            assert np.array_equal(sa[k], sc[k]), k
+        # a resident system loaded from it: forces on the loaded positions
+        b.gpu_resident(True)
+        b.force_pairs("AA", 2.5)  # a list for the original positions
+        lib.sep_load(b.atoms, S.SEP_NEIGHB, b.n, path)
+        b.reset_retval()
+        b.reset_force()
+        b.force_pairs("AA", 2.5)
+        b.gpu_sync()
+        c.sys.neighb_flag = 1
+        c.reset_retval()
+        c.reset_force()
+        c.force_pairs("AA", 2.5)
+        assert force_error(np.array(b.view["f"]), np.array(c.view["f"])) <= FP64_TOL
+        rb, rc = b.ret_dict(), c.ret_dict()
+        assert rel(rb["epot"], rc["epot"]) <= FP64_TOL
+        assert tensor_error(rb["pot_P"], rc["pot_P"]) <= FP64_TOL
+    finally:
+        a.close()
+        b.close()
+        c.close()
+
+
+def test_load_with_smaller_neighbour_arrays(product, tmp_path):
+    w = W.lj_sc(8, 0.8442, 1.0, seed=7)
+    path = str(tmp_path / "small.bin").encode()
+    s = SepSystem.from_workload(product, w, **KW)
+    try:
+        s.force_pairs("AA", 2.5)
+        product.lib.sep_save(s.atoms, s.n, path)
+        product.lib.sep_load(s.atoms, 160, s.n, path)
+        s.sys.neighb_flag = 1
+        s.reset_retval()
+        s.reset_force()
+        s.force_pairs("AA", 2.5)
+        half = s.half_pairs()
+        assert len(half) > 0
+    finally:
+        s.close()
