@@ -15,10 +15,10 @@
 //                 by original index (the new order, independent of the
 //                 atomic order that listed them) and move their state to it
 //                 in the spare copy
-//   k_bbuild      one wave per cell of a flagged system: the candidates of
-//                 the distinct neighbour cells, 64 at a time in the lanes,
-//                 tested against each home atom in turn and appended by
-//                 ballot in candidate order
+//   k_bbuild      one workgroup per cell of a flagged system: the
+//                 candidates of the distinct neighbour cells staged in LDS,
+//                 tested 64 at a time against each home atom (ballots kept
+//                 in LDS), then written to the lists in candidate order
 //   k_bstep       persistent: the grid walks a table of tiles (a system and
 //                 a run of its atoms, lanes per atom chosen per system);
 //                 pair forces, Nose-Hoover and leapfrog for the tile's

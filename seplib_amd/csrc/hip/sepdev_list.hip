@@ -626,14 +626,14 @@ __device__ __forceinline__ void sepk_build_cell(
 		// the list is padded with the phantom slot to a multiple of SEPK_PAD
 		if (threadIdx.x < hn) {
 			const int i = threadIdx.x;
-			const int mreg = L.hcnt[par][i];
+			const int len = L.hcnt[par][i];
 			const long t = hs + hb + i;
-			cnt[t] = mreg < cap ? mreg : cap;
-			if (mreg > cap)
+			cnt[t] = len < cap ? len : cap;
+			if (len > cap)
 				sepk_set_error(sys, SEP_GPU_ENEIGHB);
-			mymax = max(mymax, mreg);
-			const int pad = min((mreg + SEPK_PAD - 1) / SEPK_PAD * SEPK_PAD, cap);
-			for (int e = mreg; e < pad; e++)
+			mymax = max(mymax, len);
+			const int pad = min((len + SEPK_PAD - 1) / SEPK_PAD * SEPK_PAD, cap);
+			for (int e = len; e < pad; e++)
 				nbr[sepk_entry(e, t, npad)] = phantom;
 		}
 		if (hb + SEPK_BHOME < h)
