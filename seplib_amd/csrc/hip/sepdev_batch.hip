@@ -57,6 +57,7 @@ struct sepb_sys {
 	int ntile, row0;  // its tiles' partial rows: row0 .. row0 + ntile - 1
 	int ncell[3], cell_off;  // cell_start slots: cell_off .. + ncells
 	double lcell[3], L[3], iL[3];
+	double ilcell[3]; // 1 / lcell
 	double rlist2;
 	double dt, Td, dt_tau2, skin;
 	int cur, p;       // state copy in use, position slot in use
@@ -87,6 +88,8 @@ struct sepb_view {
 	long slots;
 	int nsys;
 	int *cellid, *cslot, *otmp;
+	int *gcount;          // per-cell member counts of the binning (0 between)
+	int *outside;         // per system: an atom outside the box (0 between)
 	int *cell_start;
 	const int *cell_sys;  // system of every global cell index
 	long ncell_tot;
@@ -105,6 +108,7 @@ struct sepdev_batch {
 	sepb_sys *dsys;
 	sepb_sys *hsys; // host copy of the descriptors as last uploaded
 	int *cellid, *cslot, *otmp, *cell_start, *cell_sys;
+	int *gcount, *outside;
 	int *nbr, *nbr_count;
 	int cap;
 	double *partial;
@@ -179,7 +183,7 @@ __global__ __launch_bounds__(SEPB_BIN_THREADS) void k_bbin(sepb_view b)
 		outside = 0;
 	__syncthreads();
 	const sepk_pos *pos = b.st[y->cur].pos[y->p] + off;
-	const double il[3] = {1.0 / y->lcell[0], 1.0 / y->lcell[1], 1.0 / y->lcell[2]};
+	const double il[3] = {y->ilcell[0], y->ilcell[1], y->ilcell[2]};
 	// SEPB_BIN_UNROLL positions per thread are loaded before any is
 	// binned, so a large system pays a few memory round trips, not one
 	// per SEPB_BIN_THREADS atoms
@@ -287,7 +291,8 @@ __device__ __forceinline__ int sepb_offsets(int nc, int *o)
 // system s (fpre[nsys]: all of them), scanned by each workgroup into LDS.
 // Batches of more than SEPB_FMAP systems walk every cell instead.
 #define SEPB_FMAP 1024
-__device__ __forceinline__ void sepb_flag_prefix(const sepb_view &b, int *fpre)
+__device__ __forceinline__ void sepb_flag_prefix(const sepb_view &b, int *fpre,
+						 bool atoms = false)
 {
 	__shared__ int wtot[SEPB_BLOCK / 64];
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -297,7 +302,7 @@ __device__ __forceinline__ void sepb_flag_prefix(const sepb_view &b, int *fpre)
 		int v = 0;
 		if (s < b.nsys) {
 			const sepb_sys *y = b.sys + s;
-			v = y->neighb_flag ? y->ncell[0] * y->ncell[1] * y->ncell[2] : 0;
+			v = !y->neighb_flag ? 0 : atoms ? y->n : y->ncell[0] * y->ncell[1] * y->ncell[2];
 		}
 		int incl = v;
 		for (int o = 1; o < 64; o <<= 1) {
@@ -324,23 +329,29 @@ __device__ __forceinline__ void sepb_flag_prefix(const sepb_view &b, int *fpre)
 	__syncthreads();
 }
 
+// The system of flagged item w (cell or atom, by the prefix fpre): the last
+// system whose items start at or before w.
+__device__ __forceinline__ int sepb_flag_system(const int *fpre, int nsys, long w)
+{
+	int lo = 0, hi = nsys;
+	while (hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if (fpre[mid] <= w)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
 // Work item w of a rebuild kernel -> (system, cell); false if the cell's
 // system is not flagged (only when every cell is walked).
 __device__ __forceinline__ bool sepb_work_cell(const sepb_view &b, const int *fpre, bool mapped,
 					       long w, int &s, int &c)
 {
 	if (mapped) {
-		// the last system whose flagged cells start at or before w
-		int lo = 0, hi = b.nsys;
-		while (hi - lo > 1) {
-			const int mid = (lo + hi) >> 1;
-			if (fpre[mid] <= w)
-				lo = mid;
-			else
-				hi = mid;
-		}
-		s = lo;
-		c = (int)(w - fpre[lo]);
+		s = sepb_flag_system(fpre, b.nsys, w);
+		c = (int)(w - fpre[s]);
 		return true;
 	}
 	s = b.cell_sys[w];
@@ -348,6 +359,122 @@ __device__ __forceinline__ bool sepb_work_cell(const sepb_view &b, const int *fp
 		return false; // the scan's extra entry of a system, or no rebuild
 	c = (int)(w - b.sys[s].cell_off);
 	return true;
+}
+
+// Binning of the flagged systems' atoms across the whole grid, for batches
+// of up to SEPB_FMAP systems (k_bbin, one workgroup per system, serves
+// larger ones): k_bcount gives every atom its cell and its slot among the
+// cell's members (global counters, one atomic add per distinct cell of a
+// wave), k_bscan turns each flagged system's counts into cell starts (one
+// workgroup per system) and k_bscatter lists the members of every cell.
+
+// Flagged atom w -> (system, atom); the atoms are numbered in system order.
+__global__ __launch_bounds__(SEPB_BLOCK) void k_bcount(sepb_view b)
+{
+	__shared__ int apre[SEPB_FMAP + 1];
+	sepb_flag_prefix(b, apre, true);
+	const long total = apre[b.nsys];
+	const int lane = threadIdx.x & 63;
+	const unsigned long long below = (1ull << lane) - 1ull;
+	const long stride = (long)gridDim.x * blockDim.x;
+	// whole waves iterate together (the bound is rounded up to a wave)
+	const long tw = (total + 63) / 64 * 64;
+	for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < tw; w += stride) {
+		const bool act = w < total;
+		int gc = 0, s = 0;
+		long at = 0;
+		if (act) {
+			s = sepb_flag_system(apre, b.nsys, w);
+			sepb_sys *y = b.sys + s;
+			const int t = (int)(w - apre[s]);
+			at = y->off + t;
+			const double il[3] = {y->ilcell[0], y->ilcell[1], y->ilcell[2]};
+			int outside = 0;
+			const int c = sepb_cell(y, il, b.st[y->cur].pos[y->p][at], outside);
+			if (outside)
+				atomicOr(&b.outside[s], 1);
+			b.cellid[at] = c;
+			gc = y->cell_off + c;
+		}
+		unsigned long long todo = __ballot(act);
+		int my = 0;
+		while (todo) {
+			const int lead = __ffsll((long long)todo) - 1;
+			const int cl = __shfl(gc, lead, 64);
+			const unsigned long long same = __ballot(act && gc == cl);
+			int base = 0;
+			if (lane == lead)
+				base = atomicAdd(&b.gcount[cl], __popcll(same));
+			base = __shfl(base, lead, 64);
+			if (act && gc == cl)
+				my = base + __popcll(same & below);
+			todo &= ~same;
+		}
+		if (act)
+			b.cslot[at] = my;
+	}
+}
+
+// One workgroup per flagged system: the exclusive scan of its cells' counts
+// into cell_start (one extra entry: n), the counts and the outside flag
+// zeroed for the next rebuild, y->unclean set from the flag.
+__global__ __launch_bounds__(SEPB_BIN_THREADS) void k_bscan(sepb_view b)
+{
+	__shared__ int wsum[SEPB_BIN_THREADS / 64];
+	sepb_sys *y = b.sys + blockIdx.x;
+	if (!y->neighb_flag)
+		return;
+	const int nc = y->ncell[0] * y->ncell[1] * y->ncell[2];
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	int *cnt = b.gcount + y->cell_off;
+	int *st = b.cell_start + y->cell_off;
+	int carry = 0;
+	for (int base = 0; base < nc; base += SEPB_BIN_THREADS) {
+		const int c = base + threadIdx.x;
+		const int v = c < nc ? cnt[c] : 0;
+		int incl = v;
+		for (int o = 1; o < 64; o <<= 1) {
+			const int u = __shfl_up(incl, o, 64);
+			if (lane >= o)
+				incl += u;
+		}
+		if (lane == 63)
+			wsum[wid] = incl;
+		__syncthreads();
+		int before = carry, tot = 0;
+		for (int w = 0; w < SEPB_BIN_THREADS / 64; w++) {
+			if (w < wid)
+				before += wsum[w];
+			tot += wsum[w];
+		}
+		if (c < nc) {
+			st[c] = before + incl - v;
+			cnt[c] = 0;
+		}
+		carry += tot;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) {
+		st[nc] = carry;
+		y->unclean = b.outside[blockIdx.x];
+		b.outside[blockIdx.x] = 0;
+	}
+}
+
+// Every flagged atom into its cell's member list: otmp[off + start + slot].
+__global__ __launch_bounds__(SEPB_BLOCK) void k_bscatter(sepb_view b)
+{
+	__shared__ int apre[SEPB_FMAP + 1];
+	sepb_flag_prefix(b, apre, true);
+	const long total = apre[b.nsys];
+	const long stride = (long)gridDim.x * blockDim.x;
+	for (long w = (long)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += stride) {
+		const int s = sepb_flag_system(apre, b.nsys, w);
+		const sepb_sys *y = b.sys + s;
+		const int t = (int)(w - apre[s]);
+		const long at = y->off + t;
+		b.otmp[y->off + b.cell_start[y->cell_off + b.cellid[at]] + b.cslot[at]] = t;
+	}
 }
 
 // One wave per cell (grid-stride): the new order of the cell's members and
@@ -970,6 +1097,8 @@ extern "C" void sepdev_batch_destroy(sepdev_batch *b)
 	(void)hipFree(b->cellid);
 	(void)hipFree(b->cslot);
 	(void)hipFree(b->otmp);
+	(void)hipFree(b->gcount);
+	(void)hipFree(b->outside);
 	(void)hipFree(b->cell_start);
 	(void)hipFree(b->cell_sys);
 	(void)hipFree(b->nbr);
@@ -994,6 +1123,8 @@ static sepb_view view_of(const sepdev_batch *b)
 	v.cellid = b->cellid;
 	v.cslot = b->cslot;
 	v.otmp = b->otmp;
+	v.gcount = b->gcount;
+	v.outside = b->outside;
 	v.cell_start = b->cell_start;
 	v.cell_sys = b->cell_sys;
 	v.ncell_tot = b->ncell_tot;
@@ -1100,6 +1231,7 @@ extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *i
 				nc = 1;
 			y.ncell[k] = nc;
 			y.lcell[k] = a.L[k] / nc;
+			y.ilcell[k] = 1.0 / y.lcell[k];
 			y.L[k] = a.L[k];
 			y.iL[k] = 1.0 / a.L[k];
 		}
@@ -1134,6 +1266,8 @@ extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *i
 	rc |= balloc(&b->cellid, slots);
 	rc |= balloc(&b->cslot, slots);
 	rc |= balloc(&b->otmp, slots);
+	rc |= balloc(&b->gcount, cells);
+	rc |= balloc(&b->outside, nsys);
 	rc |= balloc(&b->cell_start, cells);
 	rc |= balloc(&b->cell_sys, cells);
 	rc |= balloc(&b->nbr, (size_t)slots * cap);
@@ -1235,6 +1369,11 @@ extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *i
 		e = hipMemcpy(b->cell_sys, csys, cells * sizeof(int), hipMemcpyHostToDevice);
 	if (e == hipSuccess)
 		e = hipMemset(b->nbr_count, 0, slots * sizeof(int));
+	// the binning's counters start (and stay, between rebuilds) at zero
+	if (e == hipSuccess)
+		e = hipMemset(b->gcount, 0, cells * sizeof(int));
+	if (e == hipSuccess)
+		e = hipMemset(b->outside, 0, nsys * sizeof(int));
 	free(csys);
 	free(hp);
 	free(hv);
@@ -1257,7 +1396,13 @@ extern "C" sepdev_batch *sepdev_batch_create(int nsys, const sepdev_batch_sys *i
 static int launch_rebuild(sepdev_batch *b)
 {
 	const sepb_view v = view_of(b);
-	k_bbin<<<b->nsys, SEPB_BIN_THREADS, 0, b->st>>>(v);
+	if (b->nsys <= SEPB_FMAP) {
+		k_bcount<<<SEPB_BGRID, SEPB_BLOCK, 0, b->st>>>(v);
+		k_bscan<<<b->nsys, SEPB_BIN_THREADS, 0, b->st>>>(v);
+		k_bscatter<<<SEPB_BGRID, SEPB_BLOCK, 0, b->st>>>(v);
+	} else {
+		k_bbin<<<b->nsys, SEPB_BIN_THREADS, 0, b->st>>>(v);
+	}
 	SEPK_LAUNCH_CHECK();
 	k_bsort<<<512, SEPB_BLOCK, 0, b->st>>>(v);
 	SEPK_LAUNCH_CHECK();
