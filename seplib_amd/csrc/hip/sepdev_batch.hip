@@ -8,9 +8,14 @@
 // owns slots off[s] .. off[s] + npad[s], npad = a multiple of 64 above n,
 // slot n being a phantom with a NaN position that pads lists).  Per step:
 //
-//   k_bbin        one workgroup per flagged system: cell of every atom
-//                 (LDS counters), scan of the counts into cell starts, and
-//                 the members of every cell in slot order
+//   k_bcount      every atom of a flagged system: its cell and its slot
+//                 among the cell's members (global counters)
+//   k_bscan       one workgroup per flagged system: scan of the counts
+//                 into cell starts
+//   k_bscatter    the members of every cell in slot order
+//                 (batches of more than SEPB_FMAP systems: k_bbin, one
+//                 workgroup per flagged system with LDS counters, instead
+//                 of these three)
 //   k_bsort       one wave per cell of a flagged system: rank the members
 //                 by original index (the new order, independent of the
 //                 atomic order that listed them) and move their state to it
@@ -27,8 +32,8 @@
 //                 then energies, stress, thermostat, the state copy and
 //                 position slot in use, and the rebuild flag.
 //
-// The first three return at once for systems whose flag is clear, so the
-// five launches replay from one captured graph.  Workgroups never wait for
+// The rebuild launches return at once for systems whose flag is clear, so
+// the launches of a step replay from one captured graph.  Workgroups never wait for
 // each other inside a launch: on this chip a device-scope fence writes back
 // and invalidates an XCD's L2, so every combination across workgroups
 // happens at a kernel boundary instead.  Positions are double
