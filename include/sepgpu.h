@@ -88,6 +88,12 @@ typedef struct {
 } sep_gpu_op;
 
 #define SEP_GPU_RUN_GRAPH 1 /* replay the step from a captured HIP graph */
+/* Run the per-term kernels of the bonded calls on a second stream, beside
+ * the pair kernels of the same step (the GPU form of the reference's
+ * OpenMP sections over independent force terms, source/sepomp.c and
+ * prgs/prg5.c:58-77).  Forces, energies and stress are bit-identical to the
+ * single-stream step: each atom still adds its terms in call order. */
+#define SEP_GPU_RUN_CONCURRENT 2
 
 /* Runs nsteps of { sep_reset_retval(ret); sep_reset_force(); ops[0..nops) }
  * on a resident system without host synchronisation; ret (and each alpha)

@@ -286,19 +286,21 @@ extern "C" int sepdev_lj_table(sepdev *d, const sepdev_ljtab_args *a, int ret_sl
 		return sepk_fail(SEP_GPU_EARG, "ret slot %d", ret_slot);
 	sepk_ljtab c;
 	int rc = fill_table(a, c);
-	if (rc || (rc = sepk_flush_nh(d)) || (rc = sepk_claim_region(d, SEPK_R_PAIR)))
+	int region;
+	if (rc || (rc = sepk_flush_nh(d)) || (rc = sepk_term_region(d, &region)) ||
+	    (rc = sepk_fork_point(d)))
 		return rc;
 	const bool over = d->f_zero_pending != 0;
 	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
 	auto k = over ? k_lj_table<true> : k_lj_table<false>;
 	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c,
-					  sepk_region(d, SEPK_R_PAIR), d->region_rows);
+					  sepk_region(d, region), d->region_rows);
 	SEPK_LAUNCH_CHECK();
 	d->f_zero_pending = 0;
 	sepk_task f;
 	memset(&f, 0, sizeof f);
 	f.mode = SEPK_FIN_PAIR;
-	f.region = SEPK_R_PAIR;
+	f.region = region;
 	f.nblk = (int)nb;
 	f.nv = 7;
 	f.epot_assign = 0; // sep_force_lj adds to epot

@@ -134,12 +134,12 @@ static int gather(sepdev *d, const double *contrib, const int *off, const int *i
 	return 0;
 }
 
-static int bonded_finish(sepdev *d, unsigned nb, int ret_slot, int with_stress)
+static int bonded_finish(sepdev *d, unsigned nb, int ret_slot, int with_stress, int region)
 {
 	sepk_task f;
 	memset(&f, 0, sizeof f);
 	f.mode = SEPK_FIN_BONDED;
-	f.region = SEPK_R_PAIR;
+	f.region = region;
 	f.nblk = (int)nb;
 	f.nv = 7;
 	f.with_stress = with_stress;
@@ -195,15 +195,17 @@ extern "C" int sepdev_bond(sepdev *d, int type, double lbond, double ks,
 {
 	if (d->nb == 0)
 		return 0;
+	int region;
+	hipStream_t st;
 	int rc = sepk_flush_nh(d);
-	if (rc || (rc = sepk_claim_region(d, SEPK_R_PAIR)))
+	if (rc || (rc = sepk_term_region(d, &region)) || (rc = sepk_term_stream(d, 1, &st)))
 		return rc;
 	const unsigned nb = sepk_blocks(d->nb);
-	k_bond<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->iperm, d->blist, d->nb, type, lbond,
-					      ks, L[0], L[1], L[2], d->bcontrib, d->blen,
-					      sepk_region(d, SEPK_R_PAIR), d->region_rows);
+	k_bond<<<nb, SEPK_BLOCK, 0, st>>>(d->s[0], d->iperm, d->blist, d->nb, type, lbond, ks,
+					   L[0], L[1], L[2], d->bcontrib, d->blen,
+					   sepk_region(d, region), d->region_rows);
 	SEPK_LAUNCH_CHECK();
-	if ((rc = bonded_finish(d, nb, ret_slot, 1)))
+	if ((rc = sepk_term_join(d, st)) || (rc = bonded_finish(d, nb, ret_slot, 1, region)))
 		return rc;
 	return gather(d, d->bcontrib, d->bcsr_off, d->bcsr_idx);
 }
@@ -271,15 +273,17 @@ extern "C" int sepdev_angle(sepdev *d, int type, double angle0, double k, int co
 		return 0;
 	// cos(pi - angle0) on the host with the C library, as the reference
 	const double ccon = cos(SEPK_PI - angle0);
+	int region;
+	hipStream_t st;
 	int rc = sepk_flush_nh(d);
-	if (rc || (rc = sepk_claim_region(d, SEPK_R_PAIR)))
+	if (rc || (rc = sepk_term_region(d, &region)) || (rc = sepk_term_stream(d, 2, &st)))
 		return rc;
 	const unsigned nb = sepk_blocks(d->na);
-	k_angle<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->iperm, d->alist, d->na, type, angle0,
-					       k, ccon, cossq, L[0], L[1], L[2], d->acontrib,
-					       d->aval, sepk_region(d, SEPK_R_PAIR), d->region_rows);
+	k_angle<<<nb, SEPK_BLOCK, 0, st>>>(d->s[0], d->iperm, d->alist, d->na, type, angle0, k,
+					    ccon, cossq, L[0], L[1], L[2], d->acontrib, d->aval,
+					    sepk_region(d, region), d->region_rows);
 	SEPK_LAUNCH_CHECK();
-	if ((rc = bonded_finish(d, nb, ret_slot, 0)))
+	if ((rc = sepk_term_join(d, st)) || (rc = bonded_finish(d, nb, ret_slot, 0, region)))
 		return rc;
 	return gather(d, d->acontrib, d->acsr_off, d->acsr_idx);
 }
@@ -349,15 +353,17 @@ extern "C" int sepdev_torsion(sepdev *d, int type, const double *g, const double
 	sepk_rb gg;
 	for (int i = 0; i < 6; i++)
 		gg.g[i] = g[i];
+	int region;
+	hipStream_t st;
 	int rc = sepk_flush_nh(d);
-	if (rc || (rc = sepk_claim_region(d, SEPK_R_PAIR)))
+	if (rc || (rc = sepk_term_region(d, &region)) || (rc = sepk_term_stream(d, 4, &st)))
 		return rc;
 	const unsigned nb = sepk_blocks(d->nd);
-	k_torsion<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->iperm, d->dlist, d->nd, type, gg,
-						 L[0], L[1], L[2], d->dcontrib, d->dval,
-						 sepk_region(d, SEPK_R_PAIR), d->region_rows);
+	k_torsion<<<nb, SEPK_BLOCK, 0, st>>>(d->s[0], d->iperm, d->dlist, d->nd, type, gg, L[0],
+					      L[1], L[2], d->dcontrib, d->dval,
+					      sepk_region(d, region), d->region_rows);
 	SEPK_LAUNCH_CHECK();
-	if ((rc = bonded_finish(d, nb, ret_slot, 0)))
+	if ((rc = sepk_term_join(d, st)) || (rc = bonded_finish(d, nb, ret_slot, 0, region)))
 		return rc;
 	return gather(d, d->dcontrib, d->dcsr_off, d->dcsr_idx);
 }

@@ -222,6 +222,12 @@ extern "C" void sepdev_destroy(sepdev *d)
 			(void)hipEventDestroy(d->t_pair.end[i]);
 		}
 	}
+	if (d->side) {
+		(void)hipStreamSynchronize(d->side);
+		(void)hipEventDestroy(d->fork_ev);
+		(void)hipEventDestroy(d->join_ev);
+		(void)hipStreamDestroy(d->side);
+	}
 	if (d->st)
 		(void)hipStreamDestroy(d->st);
 	free(d);

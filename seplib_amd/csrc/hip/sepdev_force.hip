@@ -381,10 +381,10 @@ static bool fast_image_ok(const sepdev *d, const double *L)
 static int launch_lj(sepdev *d, const sepk_lj &c, int typed, int mixed, bool fast, int ret_slot,
 		     int epot_assign)
 {
-	// first: claiming the region may launch a recorded pair call, which
+	// first: claiming a region may launch a recorded pair call, which
 	// consumes the pending f = 0
-	int rc;
-	if ((rc = sepk_claim_region(d, SEPK_R_PAIR)))
+	int rc, region;
+	if ((rc = sepk_term_region(d, &region)) || (rc = sepk_fork_point(d)))
 		return rc;
 	const bool over = d->f_zero_pending != 0;
 	sepk_lj_kernel k;
@@ -395,7 +395,7 @@ static int launch_lj(sepdev *d, const sepk_lj &c, int typed, int mixed, bool fas
 	if ((rc = sepk_timer_begin(d, &d->t_pair)))
 		return rc;
 	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
-	double *part = sepk_region(d, SEPK_R_PAIR);
+	double *part = sepk_region(d, region);
 	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c, part,
 					  d->region_rows);
 	SEPK_LAUNCH_CHECK();
@@ -413,7 +413,7 @@ static int launch_lj(sepdev *d, const sepk_lj &c, int typed, int mixed, bool fas
 	sepk_task f;
 	memset(&f, 0, sizeof f);
 	f.mode = SEPK_FIN_PAIR;
-	f.region = SEPK_R_PAIR;
+	f.region = region;
 	f.nblk = (int)nb;
 	f.nv = 7;
 	f.epot_assign = epot_assign;
@@ -559,7 +559,8 @@ extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot
 	int rc = sepk_flush_nh(d);
 	if (rc)
 		return rc;
-	if ((rc = sepk_claim_region(d, SEPK_R_PAIR)))
+	int region;
+	if ((rc = sepk_term_region(d, &region)) || (rc = sepk_fork_point(d)))
 		return rc;
 	const bool over = d->f_zero_pending != 0;
 	const bool fast = fast_image_ok(d, a->L);
@@ -567,13 +568,13 @@ extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot
 		      : (fast ? k_coulomb<false, true> : k_coulomb<false, false>);
 	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
 	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c,
-					  sepk_region(d, SEPK_R_PAIR), d->region_rows);
+					  sepk_region(d, region), d->region_rows);
 	SEPK_LAUNCH_CHECK();
 	d->f_zero_pending = 0;
 	sepk_task f;
 	memset(&f, 0, sizeof f);
 	f.mode = SEPK_FIN_PAIR;
-	f.region = SEPK_R_PAIR;
+	f.region = region;
 	f.nblk = (int)nb;
 	f.nv = 7;
 	f.also_ecoul = 1;
@@ -983,6 +984,7 @@ extern "C" int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *
 {
 	if (ret_slot < 0 || ret_slot >= d->nret)
 		return sepk_fail(SEP_GPU_EARG, "ret slot %d", ret_slot);
+	d->fork_ready = 0; // the atoms move
 	if (d->fused_pending)
 		return sepdev_fused_step(d, dt, skin, L, ret_slot);
 	int rc = sepk_materialize_forces(d);

@@ -59,13 +59,17 @@ struct sepk_lj {
 };
 
 // What the finisher does with the combined rows.  Partial rows live in
-// separate regions so that a build, a pair call and a leapfrog of one step
-// can all be finished by a single launch after the leapfrog.
+// separate regions so that a build, the force terms and a leapfrog of one
+// step can all be finished by a single launch after the leapfrog.  Force
+// terms take the first of the SEPK_NTERM term regions that no queued task
+// holds (SEPK_R_PAIR is the first; the fused step always uses it).
 enum { SEPK_FIN_PAIR = 0, SEPK_FIN_BONDED = 1, SEPK_FIN_NH = 2, SEPK_FIN_LF = 3,
        SEPK_FIN_BUILD = 4 };
-enum { SEPK_R_BUILD = 0, SEPK_R_PAIR = 1, SEPK_R_LF = 2, SEPK_R_MISC = 3, SEPK_NREGION = 4 };
+#define SEPK_NTERM 4
+enum { SEPK_R_BUILD = 0, SEPK_R_LF = 1, SEPK_R_MISC = 2, SEPK_R_PAIR = 3,
+       SEPK_NREGION = SEPK_R_PAIR + SEPK_NTERM };
 #define SEPK_MAXNV 10
-#define SEPK_MAXTASK 4
+#define SEPK_MAXTASK 8
 // workgroups of the finisher; the last one to finish combines their rows
 #define SEPK_FIN_BLOCKS 64
 
@@ -201,6 +205,19 @@ struct sepdev {
 	double *fused_alpha;
 	double fused_dt, fused_L[3];
 	double *tscratch;
+
+	// Concurrent force terms (source/sepomp.c: the reference's OpenMP
+	// sections over independent terms), armed by the host around a step.
+	// The per-term kernel of a bonded call runs on `side`, which waits for
+	// fork_ev; st waits for join_ev before that call's gather, so each atom
+	// still receives its forces in call order.  fork_ev is recorded on st
+	// just before the step's first pair kernel (after any list rebuild);
+	// fork_ready is cleared by anything that moves atoms or by a finisher
+	// launch that reads a term region the side stream may write next.
+	hipStream_t side;
+	hipEvent_t fork_ev, join_ev;
+	int fork_armed, fork_ready;
+	unsigned fork_used; // bonded families launched on side since fork_ev
 
 	// graph
 	hipGraph_t graph;
@@ -354,6 +371,16 @@ double *sepk_region(sepdev *d, int region);
 // Finishes queued work first if a queued task still reads `region`, which
 // the caller's next kernel is about to overwrite.
 int sepk_claim_region(sepdev *d, int region);
+// A term region for a force call's partial rows: the first one that no
+// queued task holds (finishing the queue first if all are held).
+int sepk_term_region(sepdev *d, int *region);
+// Concurrent terms: records fork_ev on st if the step is armed and it is
+// not yet recorded; the stream a bonded call's per-term kernel goes to
+// (side once fork_ev is recorded, else st), and the join after it.
+int sepk_fork_point(sepdev *d);
+// family: 1 bonds, 2 angles, 4 torsions (each has one set of per-term slots)
+int sepk_term_stream(sepdev *d, unsigned family, hipStream_t *ks);
+int sepk_term_join(sepdev *d, hipStream_t ks);
 
 // Writes the 6 unique stress components (xx xy xz yy yz zz) into a
 // row-major 3x3 block.

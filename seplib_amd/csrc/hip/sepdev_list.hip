@@ -500,11 +500,13 @@ __device__ __forceinline__ void sepk_build_emit(sepk_build_lds &L, int kn, int h
 	}
 }
 
+// Slice `slice` of `nslice` of cell c: its home atoms
+// [h * slice / nslice, h * (slice + 1) / nslice), SEPK_BHOME at a time.
 template <bool SHIFT, int EXCL>
 __device__ __forceinline__ void sepk_build_cell(
-	sepk_build_lds &L, const sepk_state &s, long c, const int *start,
-	const sepdev_grid &g, long npad, int cap, int phantom, int *nbr, int *cnt,
-	sepdev_sys *sys, int &mymax)
+	sepk_build_lds &L, const sepk_state &s, long c, int slice, int nslice,
+	const int *start, const sepdev_grid &g, long npad, int cap, int phantom, int *nbr,
+	int *cnt, sepdev_sys *sys, int &mymax)
 {
 	const int lane = threadIdx.x & 63;
 	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
@@ -553,8 +555,11 @@ __device__ __forceinline__ void sepk_build_cell(
 	__syncthreads();
 	const int hs = L.hs, h = L.h, T = L.segpre[g.noff];
 	const bool anyshift = L.anyshift;
-	for (int hb = 0; hb < h; hb += SEPK_BHOME) {
-		const int hn = min(h - hb, SEPK_BHOME);
+	// h < 2^24 atoms a cell and nslice <= 64: 32-bit products
+	const int hbeg = nslice == 1 ? 0 : (int)((unsigned)(h * slice) / (unsigned)nslice);
+	const int hend = nslice == 1 ? h : (int)((unsigned)(h * (slice + 1)) / (unsigned)nslice);
+	for (int hb = hbeg; hb < hend; hb += SEPK_BHOME) {
+		const int hn = min(hend - hb, SEPK_BHOME);
 		if (threadIdx.x < SEPK_BHOME)
 			L.hcnt[0][threadIdx.x] = 0;
 		int par = 0;
@@ -636,19 +641,19 @@ __device__ __forceinline__ void sepk_build_cell(
 			for (int e = len; e < pad; e++)
 				nbr[sepk_entry(e, t, npad)] = phantom;
 		}
-		if (hb + SEPK_BHOME < h)
+		if (hb + SEPK_BHOME < hend)
 			__syncthreads(); // the next batch restages the candidates
 	}
 }
 
-// Persistent over cells; the largest list length goes to sys->max_full by an
-// atomic maximum (order-independent, so the result is deterministic).  The
-// permuted state of the workgroup's cells goes home at the end, in one
-// batch, so its memory round trip is paid once.
+// Persistent over (cell, slice) items, nslice per cell; the largest list
+// length goes to sys->max_full by an atomic maximum (order-independent, so
+// the result is deterministic).  The permuted state of the workgroup's cells
+// goes home at the end, in one batch, so its memory round trip is paid once.
 template <int EXCL>
 __global__ __launch_bounds__(SEPK_BT) __attribute__((amdgpu_waves_per_eu(EXCL == 2 ? 4 : 5))) void k_build(
-	sepk_state s, sepk_state home, const int *start, sepdev_grid g, long ncell, long n,
-	long npad, int cap, int *nbr, int *cnt, sepdev_sys *sys, int conditional,
+	sepk_state s, sepk_state home, const int *start, sepdev_grid g, long ncell, int nslice,
+	long n, long npad, int cap, int *nbr, int *cnt, sepdev_sys *sys, int conditional,
 	int has_partners)
 {
 	if (conditional && sys->neighb_flag == 0)
@@ -657,18 +662,24 @@ __global__ __launch_bounds__(SEPK_BT) __attribute__((amdgpu_waves_per_eu(EXCL ==
 	const bool shift = g.shift_ok && sys->unclean == 0;
 	const long per = gridDim.x / SEPK_NXCD;
 	const long xcd = blockIdx.x % SEPK_NXCD, k = blockIdx.x / SEPK_NXCD;
-	const long cper = (ncell + SEPK_NXCD - 1) / SEPK_NXCD;
-	const long lo = xcd * cper, hi = min(lo + cper, ncell);
+	const long nitem = ncell * nslice;
+	const long iper = (nitem + SEPK_NXCD - 1) / SEPK_NXCD;
+	const long ilo = xcd * iper, ihi = min(ilo + iper, nitem);
 	int mymax = 0;
-	for (long c = lo + k; c < hi; c += per) {
+	for (long it = ilo + k; it < ihi; it += per) {
+		// items < 2^31 (the host bounds nslice by the atoms a cell)
+		const long c = nslice == 1 ? it : (long)((unsigned)it / (unsigned)nslice);
+		const int sl = (int)(it - c * nslice);
 		if (shift)
-			sepk_build_cell<true, EXCL>(L, s, c, start, g, npad, cap, (int)n, nbr, cnt,
-						    sys, mymax);
+			sepk_build_cell<true, EXCL>(L, s, c, sl, nslice, start, g, npad, cap, (int)n,
+						    nbr, cnt, sys, mymax);
 		else
-			sepk_build_cell<false, EXCL>(L, s, c, start, g, npad, cap, (int)n, nbr, cnt,
-						     sys, mymax);
+			sepk_build_cell<false, EXCL>(L, s, c, sl, nslice, start, g, npad, cap, (int)n,
+						     nbr, cnt, sys, mymax);
 	}
 	// one wave per cell, SEPK_BT / 64 cells at a time
+	const long cper = (ncell + SEPK_NXCD - 1) / SEPK_NXCD;
+	const long lo = xcd * cper, hi = min(lo + cper, ncell);
 	const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
 	for (long c = lo + k + wv * per; c < hi; c += (SEPK_BT / 64) * per) {
 		const int a = start[c], b = start[c + 1];
@@ -732,6 +743,7 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 		return sepk_fail(SEP_GPU_EARG, "build_list: %ld cells", ncell);
 	if (g.opt == 2 && !d->s[0].partners)
 		return sepk_fail(SEP_GPU_ESTATE, "SEP_EXCL_BONDED needs bonded partners");
+	d->fork_ready = 0; // a rebuild permutes the atoms
 	int rc = sepk_flush_nh(d);
 	if (rc)
 		return rc;
@@ -785,13 +797,28 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 		d->order_tmp, d->cell_start, cur, spare, d->iperm, ncell, d->npad,
 		d->sys, conditional, hp);
 	SEPK_LAUNCH_CHECK();
-	long bb = (ncell + SEPK_NXCD - 1) / SEPK_NXCD * SEPK_NXCD;
+	// Home atoms of a cell split over nslice workgroups when the cells are
+	// few and full (small boxes, dense molecular systems): at least 16 home
+	// atoms a slice, at most SEPK_BHOME, about 1024 items where cells alone
+	// are fewer.  The lists do not depend on the split.
+	const long avg = (d->n + ncell - 1) / ncell;
+	long nslice = (avg + 15) / 16;
+	const long fill = (1024 + ncell - 1) / ncell;
+	if (nslice > fill)
+		nslice = fill;
+	if (nslice < (avg + SEPK_BHOME - 1) / SEPK_BHOME)
+		nslice = (avg + SEPK_BHOME - 1) / SEPK_BHOME;
+	if (nslice < 1)
+		nslice = 1;
+	if (nslice > 64)
+		nslice = 64;
+	long bb = (ncell * nslice + SEPK_NXCD - 1) / SEPK_NXCD * SEPK_NXCD;
 	if (bb > SEPK_BGRID)
 		bb = SEPK_BGRID;
 	auto kb = g.opt == 2 ? k_build<2> : g.opt == 3 ? k_build<3> : k_build<0>;
-	kb<<<(unsigned)bb, SEPK_BT, 0, d->st>>>(spare, cur, d->cell_start, g, ncell, d->n, d->npad,
-						      d->cap, d->nbr, d->nbr_count, d->sys,
-						      conditional, hp);
+	kb<<<(unsigned)bb, SEPK_BT, 0, d->st>>>(spare, cur, d->cell_start, g, ncell, (int)nslice,
+						      d->n, d->npad, d->cap, d->nbr, d->nbr_count,
+						      d->sys, conditional, hp);
 	SEPK_LAUNCH_CHECK();
 	// clears the rebuild flag and the owed xn reset
 	sepk_task f;

@@ -10,7 +10,7 @@
 
 #include "sepdev_impl.h"
 
-#define FIN_THREADS 256
+#define FIN_THREADS (64 * SEPK_MAXTASK)
 
 // ret, alpha and sys point at the finisher's LDS copies of the task's
 // return slot, thermostat variable and the system scalars.
@@ -76,8 +76,8 @@ __device__ static void fin_apply(const sepk_task &k, const double *tot, double *
 	}
 }
 
-// SEPK_FIN_BLOCKS workgroups of SEPK_MAXTASK waves.  Wave w of every
-// workgroup takes task w: a contiguous chunk of its rows, one lane per row
+// SEPK_FIN_BLOCKS workgroups of one wave per queued task (at least two).
+// Wave w of every workgroup takes task w: a contiguous chunk of its rows, one lane per row
 // (SEPK_MAXNV columns loaded at once with clamped addresses, all tasks'
 // loads in flight together), reduced in a fixed order -- lane-strided rows,
 // then a wave butterfly -- to one value per column.  The last workgroup to
@@ -90,6 +90,7 @@ __device__ static double fin_comb(bool mx, double a, double b)
 }
 
 static_assert(FIN_THREADS == 64 * SEPK_MAXTASK, "one finisher wave per task");
+static_assert(sizeof(sepdev_sys) <= 64 * sizeof(int), "the second wave copies the scalars");
 
 __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, sepk_fin f)
 {
@@ -155,7 +156,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, 
 		}
 	}
 	const int nsys = (int)(sizeof(sepdev_sys) / sizeof(int));
-	for (int w = threadIdx.x; w < f.ntask * SEPDEV_RET_WORDS; w += FIN_THREADS) {
+	for (int w = threadIdx.x; w < f.ntask * SEPDEV_RET_WORDS; w += blockDim.x) {
 		const int u = w / SEPDEV_RET_WORDS, word = w % SEPDEV_RET_WORDS;
 		if (rmap[u] == u)
 			sret[u][word] = f.t[u].ret[word];
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, 
 				  amap[u] >= 0 ? &salpha[amap[u]] : nullptr, &ssys);
 		}
 	__syncthreads();
-	for (int w = threadIdx.x; w < f.ntask * SEPDEV_RET_WORDS; w += FIN_THREADS) {
+	for (int w = threadIdx.x; w < f.ntask * SEPDEV_RET_WORDS; w += blockDim.x) {
 		const int u = w / SEPDEV_RET_WORDS, word = w % SEPDEV_RET_WORDS;
 		if (rmap[u] == u)
 			f.t[u].ret[word] = sret[u][word];
@@ -244,8 +245,79 @@ int sepk_flush_tasks(sepdev *d)
 	f.l2 = d->fin_l2;
 	f.count = d->fin_count;
 	d->npending = 0;
-	k_finish<<<SEPK_FIN_BLOCKS, FIN_THREADS, 0, d->st>>>(d->partials, f);
+	// one wave per task, at least two (the system scalars are copied by the
+	// second wave)
+	const unsigned threads = 64u * (unsigned)(f.ntask > 2 ? f.ntask : 2);
+	k_finish<<<SEPK_FIN_BLOCKS, threads, 0, d->st>>>(d->partials, f);
 	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+int sepk_term_region(sepdev *d, int *region)
+{
+	int rc = sepk_flush_pair(d);
+	if (rc)
+		return rc;
+	for (int r = SEPK_R_PAIR; r < SEPK_R_PAIR + SEPK_NTERM; r++) {
+		bool held = false;
+		for (int i = 0; i < d->npending; i++)
+			held = held || d->pending[i].region == r;
+		if (!held) {
+			*region = r;
+			return 0;
+		}
+	}
+	// the finisher reads the term regions: the side stream waits for it
+	d->fork_ready = 0;
+	*region = SEPK_R_PAIR;
+	return sepk_flush_tasks(d);
+}
+
+int sepk_fork_point(sepdev *d)
+{
+	if (!d->fork_armed || d->fork_ready)
+		return 0;
+	SEPK_CHECK(hipEventRecord(d->fork_ev, d->st));
+	d->fork_ready = 1;
+	d->fork_used = 0;
+	return 0;
+}
+
+int sepk_term_stream(sepdev *d, unsigned family, hipStream_t *ks)
+{
+	*ks = d->st;
+	if (!d->fork_armed || !d->fork_ready)
+		return 0;
+	// a second call of the family since the fork rewrites the family's
+	// per-term slots: it waits for the gather of the first
+	if (d->fork_used & family) {
+		SEPK_CHECK(hipEventRecord(d->fork_ev, d->st));
+		d->fork_used = 0;
+	}
+	d->fork_used |= family;
+	SEPK_CHECK(hipStreamWaitEvent(d->side, d->fork_ev, 0));
+	*ks = d->side;
+	return 0;
+}
+
+int sepk_term_join(sepdev *d, hipStream_t ks)
+{
+	if (ks == d->st)
+		return 0;
+	SEPK_CHECK(hipEventRecord(d->join_ev, ks));
+	SEPK_CHECK(hipStreamWaitEvent(d->st, d->join_ev, 0));
+	return 0;
+}
+
+extern "C" int sepdev_concurrent_terms(sepdev *d, int on)
+{
+	if (on && !d->side) {
+		SEPK_CHECK(hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking));
+		SEPK_CHECK(hipEventCreateWithFlags(&d->fork_ev, hipEventDisableTiming));
+		SEPK_CHECK(hipEventCreateWithFlags(&d->join_ev, hipEventDisableTiming));
+	}
+	d->fork_armed = on != 0;
+	d->fork_ready = 0;
 	return 0;
 }
 

@@ -82,21 +82,44 @@ static int fusable(const sep_gpu_op *ops, int nops)
 	return nops == 2 || ops[1].op == SEP_GPU_OP_NOSEHOOVER;
 }
 
+/* A step whose bonded terms can overlap its pair terms: a pair call
+ * (which fixes the atom order for the step) before a bonded call. */
+static int overlappable(const sep_gpu_op *ops, int nops)
+{
+	int pair = 0;
+	for (int i = 0; i < nops; i++) {
+		const int op = ops[i].op;
+		if (op == SEP_GPU_OP_PAIRS || op == SEP_GPU_OP_LJ || op == SEP_GPU_OP_COULOMB)
+			pair = 1;
+		else if (pair && op >= SEP_GPU_OP_BOND && op <= SEP_GPU_OP_TORSION)
+			return 1;
+	}
+	return 0;
+}
+
 static int run_ops(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
 		   int nops);
 
 static int run_one_step(seppart *atoms, sepsys *sys, sepret *ret,
-			const sep_gpu_op *ops, int nops)
+			const sep_gpu_op *ops, int nops, int flags)
 {
 	seph_ctx *c = seph_ctx_for(atoms, sys);
 	if (!c)
 		return sep_gpu_last_error();
 	const int fuse = c->resident && fusable(ops, nops);
+	const int conc = c->resident && (flags & SEP_GPU_RUN_CONCURRENT) && overlappable(ops, nops);
 	if (fuse)
 		sepdev_fuse_step(c->dev, 1);
-	int rc = run_ops(atoms, sys, ret, ops, nops);
+	int rc = conc ? seph_dev_fail(sepdev_concurrent_terms(c->dev, 1)) : 0;
+	if (!rc)
+		rc = run_ops(atoms, sys, ret, ops, nops);
 	if (fuse) {
 		const int rc2 = seph_dev_fail(sepdev_fuse_step(c->dev, 0));
+		if (!rc)
+			rc = rc2;
+	}
+	if (conc) {
+		const int rc2 = seph_dev_fail(sepdev_concurrent_terms(c->dev, 0));
 		if (!rc)
 			rc = rc2;
 	}
@@ -157,12 +180,12 @@ int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
 		return 0;
 	sep_gpu_clear_error();
 	/* the first step runs eagerly: it binds slots and sizes every buffer */
-	int rc = run_one_step(atoms, sys, ret, ops, nops);
+	int rc = run_one_step(atoms, sys, ret, ops, nops, flags);
 	if (rc || nsteps == 1)
 		return rc;
 	if (!(flags & SEP_GPU_RUN_GRAPH)) {
 		for (long s = 1; s < nsteps; s++)
-			if ((rc = run_one_step(atoms, sys, ret, ops, nops)))
+			if ((rc = run_one_step(atoms, sys, ret, ops, nops, flags)))
 				return rc;
 		return 0;
 	}
@@ -173,9 +196,9 @@ int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
 	if (rest >= 2) {
 		if ((rc = seph_dev_fail(sepdev_graph_begin(c->dev))))
 			return rc;
-		rc = run_one_step(atoms, sys, ret, ops, nops);
+		rc = run_one_step(atoms, sys, ret, ops, nops, flags);
 		if (!rc)
-			rc = run_one_step(atoms, sys, ret, ops, nops);
+			rc = run_one_step(atoms, sys, ret, ops, nops, flags);
 		int rc2 = seph_dev_fail(sepdev_graph_end(c->dev));
 		if (rc || rc2)
 			return rc ? rc : rc2;
@@ -184,7 +207,7 @@ int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
 			return rc;
 	}
 	if (rest % 2)
-		rc = run_one_step(atoms, sys, ret, ops, nops);
+		rc = run_one_step(atoms, sys, ret, ops, nops, flags);
 	return rc;
 }
 

@@ -182,6 +182,9 @@ int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *L,
  * other use of the forces launches it first); off launches a recorded call
  * that is still waiting */
 int sepdev_fuse_step(sepdev *d, int on);
+/* Arms (on = 1) or disarms concurrent force terms for the calls of one step:
+ * bonded per-term kernels overlap the pair kernels on a second stream. */
+int sepdev_concurrent_terms(sepdev *d, int on);
 
 /* ---- misc */
 int sepdev_synchronize(sepdev *d);

@@ -35,6 +35,7 @@ SEP_GPU_FP64, SEP_GPU_MIXED = 0, 1
 (OP_PAIRS, OP_LJ, OP_COULOMB, OP_BOND, OP_ANGLE, OP_COSSQ, OP_TORSION,
  OP_NOSEHOOVER, OP_LEAPFROG) = range(1, 10)
 SEP_GPU_RUN_GRAPH = 1
+SEP_GPU_RUN_CONCURRENT = 2
 
 d3 = C.c_double * 3
 i3 = C.c_int * 3

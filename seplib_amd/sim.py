@@ -235,10 +235,11 @@ class SepSystem:
     def gpu_precision(self, mode):
         self.check(self._lib().sep_gpu_set_precision(self.atoms, self.sysp, mode))
 
-    def gpu_run(self, ops, nsteps, graph=False):
+    def gpu_run(self, ops, nsteps, graph=False, concurrent=False):
         arr = (S.sep_gpu_op * len(ops))(*ops)
+        flags = (S.SEP_GPU_RUN_GRAPH if graph else 0) | (S.SEP_GPU_RUN_CONCURRENT if concurrent else 0)
         self.check(self._lib().sep_gpu_run(self.atoms, self.sysp, self.retp, arr, len(ops),
-                                           nsteps, S.SEP_GPU_RUN_GRAPH if graph else 0))
+                                           nsteps, flags))
 
     def gpu_pairs(self):
         lib = self._lib()

@@ -1,0 +1,109 @@
+"""Concurrent force terms (SEP_GPU_RUN_CONCURRENT): the per-term kernels of
+a step's bonded calls on a second stream beside its pair kernels, the GPU
+form of the reference's OpenMP sections over independent terms
+(source/sepomp.c, prgs/prg5.c:58-77).
+
+Each atom still adds its terms in call order, so the two-stream run must
+equal the one-stream run bit for bit; the one-stream run is held against
+the oracle by test_gpu_water.py and test_gpu_golden.py.
+
+  * water, 4000 molecules: 30 resident steps of LJ, Coulomb, bond, angle,
+    Nose-Hoover and leapfrog, eager and replayed from a graph, in two call
+    orders (bonded calls after both pair calls, and between them);
+  * the reference's butane input (prgs/prg2.c terms): LJ, bond, harmonic
+    angle, cosine-squared angle and Ryckaert-Bellemans torsion, where the
+    second angle call reuses the angle family's per-term slots and so must
+    wait for the first one's gather;
+  * a step with no pair call before its bonded calls runs on one stream.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from seplib_amd import sep as S
+from seplib_amd import water as WAT
+from seplib_amd.sim import SepSystem
+from tests.golden import runs as R
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _final(s):
+    s.gpu_sync()
+    st = s.state()
+    r = s.ret
+    return dict(x=st["x"], v=st["v"], f=st["f"], xn=st["xn"], crossings=st["crossings"],
+                ret=R.ret_vec(s), alpha=s.alpha.value, tnow=st["tnow"],
+                nupdate=st["nupdate_neighb"], epot=r.epot, ecoul=r.ecoul, ekin=r.ekin)
+
+
+def _run(api, w, make_ops, steps, graph, concurrent, cf, dt, nneighb):
+    s = SepSystem.from_workload(api, w, cf=cf, dt=dt, nneighb=nneighb)
+    try:
+        s.gpu_resident(True)
+        s.gpu_run(make_ops(s), steps, graph=graph, concurrent=concurrent)
+        return _final(s)
+    finally:
+        s.close()
+
+
+def _assert_same(a, b):
+    for k in a:
+        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+
+
+@pytest.fixture(scope="module")
+def water():
+    return WAT.system(4000)
+
+
+def _water_ops_pairs_first(s):
+    return WAT.step_ops(s)
+
+
+def _water_ops_interleaved(s):
+    o = WAT.step_ops(s)
+    return [o[0], o[2], o[3], o[1], o[4], o[5]]
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+@pytest.mark.parametrize("make_ops", [_water_ops_pairs_first, _water_ops_interleaved],
+                         ids=["pairs_first", "interleaved"])
+def test_water_two_streams_bit_identical(product, water, make_ops, graph):
+    kw = dict(cf=WAT.RC_COUL, dt=WAT.DT, nneighb=WAT.NNEIGHB)
+    one = _run(product, water, make_ops, 30, graph, False, **kw)
+    two = _run(product, water, make_ops, 30, graph, True, **kw)
+    assert one["nupdate"] > 0  # the run crossed list rebuilds
+    _assert_same(one, two)
+
+
+def _butane_ops(s):
+    p = R.BUTANE
+    return [s.op_pairs("CC", 2.5, "sep_lj_shift", S.SEP_EXCL_SAME_MOL),
+            s.op_bonded(S.OP_BOND, 0, [p["lbond"], p["kspring"]]),
+            s.op_bonded(S.OP_ANGLE, 0, [p["angle"], p["kangle"]]),
+            s.op_bonded(S.OP_COSSQ, 0, [p["angle"], p["kangle"]]),
+            s.op_bonded(S.OP_TORSION, 0, p["rb"]),
+            s.op_nosehoover(4.0, 0.1),
+            s.op_leapfrog()]
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_butane_two_streams_bit_identical(product, graph):
+    w = R.workload_from(np.load(os.path.join(GOLD, "g3_butane.npz"), allow_pickle=False))
+    kw = dict(cf=2.5, dt=0.001, nneighb=S.SEP_NEIGHB)
+    one = _run(product, w, _butane_ops, 40, graph, False, **kw)
+    two = _run(product, w, _butane_ops, 40, graph, True, **kw)
+    _assert_same(one, two)
+
+
+def test_bonded_before_pairs_stays_on_one_stream(product, water):
+    def ops(s):
+        o = WAT.step_ops(s)
+        return [o[2], o[3], o[0], o[1], o[4], o[5]]
+    kw = dict(cf=WAT.RC_COUL, dt=WAT.DT, nneighb=WAT.NNEIGHB)
+    _assert_same(_run(product, water, ops, 10, True, False, **kw),
+                 _run(product, water, ops, 10, True, True, **kw))
