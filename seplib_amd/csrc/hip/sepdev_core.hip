@@ -83,7 +83,7 @@ __global__ void k_init_identity(sepk_state s, int *iperm, long n, long npad)
 	// first of them, and a NaN position fails every cutoff test
 	const double nan = __builtin_nan("");
 	s.pos[t] = t < n ? sepk_pos{0.0, 0.0, 0.0, 0.0} : sepk_pos{nan, nan, nan, 0.0};
-	s.qpos[t] = make_int4(0, 0, 0, t < n ? 0 : 1);
+	s.qpos[t] = make_int4(0, 0, 0, t < n ? 0 : SEPK_QPHANTOM);
 	s.m[t] = 1.0;
 	s.type[t] = t < n ? 'A' : 0;
 	s.mol[t] = -1;
@@ -105,7 +105,7 @@ __global__ void k_init_alt(sepk_pos *pos, int4 *qpos, long n, long npad)
 		return;
 	const double nan = __builtin_nan("");
 	pos[t] = t < n ? sepk_pos{0.0, 0.0, 0.0, 0.0} : sepk_pos{nan, nan, nan, 0.0};
-	qpos[t] = make_int4(0, 0, 0, t < n ? 0 : 1);
+	qpos[t] = make_int4(0, 0, 0, t < n ? 0 : SEPK_QPHANTOM);
 }
 
 extern "C" sepdev *sepdev_create(long npart, int *err)
@@ -460,6 +460,8 @@ extern "C" int sepdev_put_scalars(sepdev *d, const double *mass, const double *c
 	const long n = d->n;
 	if (mass)
 		d->sumv2_valid = 0;
+	if (charge)
+		d->qpos_stale = 1; // the quantized records carry the charges
 	const size_t bytes = (size_t)n * (2 * sizeof(double) + sizeof(int) + 1);
 	if (bytes > d->stage_bytes)
 		return sepk_fail(SEP_GPU_EARG, "staging overflow");

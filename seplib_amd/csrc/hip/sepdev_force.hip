@@ -189,7 +189,7 @@ __device__ __forceinline__ void sepk_lj_pairs(
 		// the stress identity uses the quantized positions x~ = u L / 2^32
 		// and the image count k of the pair, nonzero when the unwrapped
 		// difference borrows differently from the wrapped one
-		int4 qi = make_int4(0, 0, 0, 1);
+		int4 qi = make_int4(0, 0, 0, SEPK_QPHANTOM);
 		if (act) {
 			qi = s.qpos[t];
 			const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
@@ -211,8 +211,8 @@ __device__ __forceinline__ void sepk_lj_pairs(
 					if (TYPED && !((ti == c.t0 && tj[u] == c.t1) ||
 						       (ti == c.t1 && tj[u] == c.t0)))
 						continue;
-					if (qj[u].w)
-						continue; // the phantom
+					if (qj[u].w == SEPK_QPHANTOM)
+						continue;
 					const unsigned ux = (unsigned)qi.x, vx = (unsigned)qj[u].x;
 					const unsigned uy = (unsigned)qi.y, vy = (unsigned)qj[u].y;
 					const unsigned uz = (unsigned)qi.z, vz = (unsigned)qj[u].z;
@@ -327,8 +327,8 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 typedef void (*sepk_lj_kernel)(sepk_state, const int *, const int *, long, long,
 			       sepk_lj, double *, long);
 
-// qpos from pos for every slot (phantoms get w = 1), after a position upload
-// or a change of box.
+// qpos from pos for every slot (sepk_quant), after a position or charge
+// upload or a change of box.
 __global__ void k_quantize(sepk_state s, long npad, double q0, double q1, double q2)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -493,9 +493,77 @@ extern "C" int sepdev_fuse_step(sepdev *d, int on)
 struct sepk_coul {
 	double cf, cf2, icf, icf2;
 	double L0, L1, L2, iL0, iL1, iL2;
+	double qd0, qd1, qd2; // quantization step per axis (mixed form)
+	float cff, icff, icf2f;
 };
 
-template <bool OVERWRITE, bool FAST>
+// The mixed form of the pair loop below: positions and charges from the
+// 16-byte quantized records (integer minimum image, charge as FP32 in w),
+// r^2 and the cutoff test in FP64 from the exact integer differences, the
+// pair force and energy in FP32 (hardware reciprocal square root, factored
+// forms of the shifted-force terms), sums in FP64.  The charge test is the reference's |q| < DBL_EPSILON; the phantom's
+// NaN charge fails it too.
+__device__ __forceinline__ void sepk_coul_pairs_mixed(
+	const sepk_state &s, const int *__restrict__ nbr, const int *__restrict__ cnt,
+	long npad, const sepk_coul &c, long t, int sub, int4 qi, double &fx, double &fy,
+	double &fz, double (&acc)[7])
+{
+	const float qif = __int_as_float(qi.w);
+	const float epsf = (float)DBL_EPSILON;
+	const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
+	const size_t stride = (size_t)npad * SEPK_SUB;
+	const int4 *lp = sepk_lane_list(nbr, t, sub);
+	for (int b = 0; b < nbat; b++, lp += stride) {
+		const int4 jb = *lp;
+		const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
+		int4 qj[SEPK_BATCH];
+#pragma unroll
+		for (int u = 0; u < SEPK_BATCH; u++)
+			qj[u] = s.qpos[(unsigned)j[u]];
+#pragma unroll
+		for (int u = 0; u < SEPK_BATCH; u++) {
+			const float qjf = __int_as_float(qj[u].w);
+			if (!(fabsf(qjf) >= epsf))
+				continue;
+			const unsigned ux = (unsigned)qi.x, vx = (unsigned)qj[u].x;
+			const unsigned uy = (unsigned)qi.y, vy = (unsigned)qj[u].y;
+			const unsigned uz = (unsigned)qi.z, vz = (unsigned)qj[u].z;
+			const int ix = (int)(ux - vx), iy = (int)(uy - vy), iz = (int)(uz - vz);
+			const double dx = (double)ix * c.qd0;
+			const double dy = (double)iy * c.qd1;
+			const double dz = (double)iz * c.qd2;
+			const double r2 = dx * dx + dy * dy + dz * dz;
+			if (!(r2 < c.cf2))
+				continue;
+			// 1/r^2 - 1/rc^2 = (rc - r)(rc + r) / (r^2 rc^2) and
+			// 1/r - 1/rc + (r - rc)/rc^2 = (rc - r)^2 / (r rc^2): the
+			// reference's expressions without their cancellation near
+			// the cutoff, which FP32 would carry into the energy sum
+			const float r2f = (float)r2;
+			const float ri = __builtin_amdgcn_rsqf(r2f);
+			const float rf = r2f * ri;
+			const float qq = qjf * qif;
+			const float dc = c.cff - rf;
+			const float qs = qq * c.icf2f * ri;
+			const double ft = (double)(qs * (dc * (c.cff + rf)) * (ri * ri));
+			const double e = (double)(qs * (dc * dc));
+			fx = fma(ft, dx, fx);
+			fy = fma(ft, dy, fy);
+			fz = fma(ft, dz, fz);
+			acc[0] += e;
+			const bool bx = (ux < vx) != (ix < 0), by = (uy < vy) != (iy < 0),
+				   bz = (uz < vz) != (iz < 0);
+			if (bx | by | bz) {
+				const double kx = bx ? (ix < 0 ? 1.0 : -1.0) : 0.0;
+				const double ky = by ? (iy < 0 ? 1.0 : -1.0) : 0.0;
+				const double kz = bz ? (iz < 0 ? 1.0 : -1.0) : 0.0;
+				sepk_pair_cross(ft, dx, dy, dz, kx, ky, kz, c.L0, c.L1, c.L2, acc);
+			}
+		}
+	}
+}
+
+template <bool OVERWRITE, bool FAST, bool MIXED>
 __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt,
 	long n, long npad, sepk_coul c, double *partial, long rows)
@@ -510,6 +578,20 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 	if (act) {
 		pi = s.pos[t];
 		act = !(fabs(pi.q) < DBL_EPSILON);
+	}
+	if (MIXED) {
+		int4 qi = make_int4(0, 0, 0, SEPK_QPHANTOM);
+		if (act) {
+			qi = s.qpos[t];
+			sepk_coul_pairs_mixed(s, nbr, cnt, npad, c, t, sub, qi, fx, fy, fz, acc);
+		}
+		// the stress identity on the quantized positions
+		const double px = act ? (double)(unsigned)qi.x * c.qd0 : 0.0;
+		const double py = act ? (double)(unsigned)qi.y * c.qd1 : 0.0;
+		const double pz = act ? (double)(unsigned)qi.z * c.qd2 : 0.0;
+		sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, px, py, pz, acc,
+					      s.f, partial, lb, rows);
+		return;
 	}
 	if (act) {
 		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
@@ -555,17 +637,20 @@ extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot
 	if (ret_slot < 0 || ret_slot >= d->nret)
 		return sepk_fail(SEP_GPU_EARG, "ret slot %d", ret_slot);
 	sepk_coul c = {a->cf, a->cf2, a->icf, a->icf2, a->L[0], a->L[1], a->L[2],
-		       1.0 / a->L[0], 1.0 / a->L[1], 1.0 / a->L[2]};
+		       1.0 / a->L[0], 1.0 / a->L[1], 1.0 / a->L[2],
+		       a->L[0] / SEPK_QSCALE, a->L[1] / SEPK_QSCALE, a->L[2] / SEPK_QSCALE,
+		       (float)a->cf, (float)a->icf, (float)a->icf2};
 	int rc = sepk_flush_nh(d);
-	if (rc)
+	if (rc || (a->mixed && (rc = sepk_fresh_qpos(d, a->L))))
 		return rc;
 	int region;
 	if ((rc = sepk_term_region(d, &region)) || (rc = sepk_fork_point(d)))
 		return rc;
 	const bool over = d->f_zero_pending != 0;
 	const bool fast = fast_image_ok(d, a->L);
-	auto k = over ? (fast ? k_coulomb<true, true> : k_coulomb<true, false>)
-		      : (fast ? k_coulomb<false, true> : k_coulomb<false, false>);
+	auto k = a->mixed ? (over ? k_coulomb<true, false, true> : k_coulomb<false, false, true>)
+		 : over   ? (fast ? k_coulomb<true, true, false> : k_coulomb<true, false, false>)
+			  : (fast ? k_coulomb<false, true, false> : k_coulomb<false, false, false>);
 	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
 	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, c,
 					  sepk_region(d, region), d->region_rows);

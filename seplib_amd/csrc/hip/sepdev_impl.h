@@ -38,7 +38,8 @@ struct __attribute__((aligned(32))) sepk_pos {
 struct sepk_state {
 	sepk_pos *pos;
 	// the mixed-form pair kernels' copy of pos: each coordinate mod L as an
-	// unsigned 32-bit fraction of the box (sepk_quant), w = 1 on phantoms
+	// unsigned 32-bit fraction of the box (sepk_quant), w the FP32 charge
+	// bits (SEPK_QPHANTOM on phantoms)
 	int4 *qpos;
 	double *v, *f, *a, *xn;
 	int *cn, *cr;
@@ -303,17 +304,20 @@ __device__ __forceinline__ double sepk_wrap_fast(double d, double L, double invL
 // 32-bit fraction rint(x 2^32 / L) mod 2^32 (resolution L / 2^32, 1.6e-8 at
 // L = 67), so that (int)(qi - qj) (L / 2^32) is the minimum image of
 // xi - xj to within that resolution, with no wrap arithmetic; one 16-byte
-// record per neighbour gather instead of the 32-byte FP64 one.  The phantom
-// (NaN) gets w = 1.
+// record per neighbour gather instead of the 32-byte FP64 one.  w holds the
+// charge as FP32 bits (the mixed Coulomb kernel's charges); the phantom
+// (NaN position) gets SEPK_QPHANTOM, a NaN pattern no FP32 conversion of a
+// double produces.
 #define SEPK_QSCALE 4294967296.0 // 2^32
+#define SEPK_QPHANTOM (-1)
 
 __device__ __forceinline__ int4 sepk_quant(const sepk_pos &p, double s0, double s1, double s2)
 {
 	if (!(p.x == p.x))
-		return make_int4(0, 0, 0, 1);
+		return make_int4(0, 0, 0, SEPK_QPHANTOM);
 	return make_int4((int)(unsigned)(long long)rint(p.x * s0),
 			 (int)(unsigned)(long long)rint(p.y * s1),
-			 (int)(unsigned)(long long)rint(p.z * s2), 0);
+			 (int)(unsigned)(long long)rint(p.z * s2), __float_as_int((float)p.q));
 }
 
 // Per-block reduction of NV values per thread into column-major partial

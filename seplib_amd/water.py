@@ -24,8 +24,9 @@ call order, so both runs give the same bits.
     python -m seplib_amd.water [--molecules 32000] [--steps 2000]
 
 prints one JSON line: atom-steps/s for FP64 and the mixed form, each with
-the terms on one stream and on two, and whether the two-stream run's final
-state equals the one-stream run's bit for bit.
+the terms on one stream and on two, whether the two-stream run's final
+state equals the one-stream run's bit for bit, and the mixed form's force
+and energy differences from FP64 on the starting state.
 """
 import argparse
 import json
@@ -85,6 +86,31 @@ def timed(api, w, steps, warmup, precision, concurrent):
         s.close()
 
 
+def agreement(api, w):
+    """Mixed against FP64 on the starting state: max |dF_i| / RMS|F| of the
+    total force, and the relative differences of epot and ecoul."""
+    out = {}
+    for prec in (S.SEP_GPU_FP64, S.SEP_GPU_MIXED):
+        s = SepSystem.from_workload(api, w, cf=RC_COUL, dt=DT, nneighb=NNEIGHB)
+        try:
+            s.gpu_precision(prec)
+            s.reset_retval()
+            s.reset_force()
+            s.force_pairs("OO", RC_LJ, "sep_lj_shift", S.SEP_EXCL_SAME_MOL)
+            s.coulomb_sf(RC_COUL, S.SEP_EXCL_SAME_MOL)
+            s.stretch_harmonic(0, LBOND, KBOND)
+            s.angle_harmonic(0, THETA0, KANGLE)
+            out[prec] = (np.array(s.view["f"]), s.ret.epot, s.ret.ecoul)
+        finally:
+            s.close()
+    f64, e64, c64 = out[S.SEP_GPU_FP64]
+    f32, e32, c32 = out[S.SEP_GPU_MIXED]
+    rms = np.sqrt(np.mean(np.sum(f64 * f64, axis=1)))
+    return {"force_err_max": float(np.abs(f32 - f64).max() / rms),
+            "epot_rel": float(abs(e32 - e64) / abs(e64)),
+            "ecoul_rel": float(abs(c32 - c64) / abs(c64))}
+
+
 def same(a, b):
     return all(np.array_equal(np.asarray(x), np.asarray(y)) for x, y in zip(a, b))
 
@@ -114,6 +140,7 @@ def main(argv=None):
         print("%s: one stream %.1f us/step, two streams %.1f us/step, identical %s" % (
             name, 1e6 * secs1 / args.steps, 1e6 * secs2 / args.steps,
             out[name]["bit_identical"]), file=sys.stderr, flush=True)
+    out["mixed_vs_fp64"] = agreement(api, w)
     print(json.dumps(out), flush=True)
     return 0
 

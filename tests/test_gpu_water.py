@@ -13,7 +13,8 @@ reference default skin 0.25, 9^3 cells.
     total force is compared instead);
   * five resident NVT steps (Nose-Hoover T 3.86, dt 0.0005) of the whole
     step against the oracle's synchronised loop: energies to 1e-10;
-  * the mixed form's total force and energy against FP64 to 1e-5.
+  * the mixed form's total force, Coulomb force, energies and stress
+    against FP64 to 1e-5.
 """
 import numpy as np
 import pytest
@@ -125,4 +126,8 @@ def test_water_96k_mixed_against_fp64(product, water):
     f32, r32, _ = _terms(product, water, S.SEP_GPU_MIXED)
     tot64, tot32 = f64.sum(axis=0), f32.sum(axis=0)
     assert force_error(tot32, tot64) <= 1e-5, force_error(tot32, tot64)
+    # the Coulomb term alone (its kernel has its own mixed form)
+    assert force_error(f32[3], f64[3]) <= 1e-5, force_error(f32[3], f64[3])
     assert abs(r32[-1][0] - r64[-1][0]) <= 1e-5 * abs(r64[-1][0])
+    assert abs(r32[-1][2] - r64[-1][2]) <= 1e-5 * abs(r64[-1][2])
+    assert tensor_error(r32[-1][3:12], r64[-1][3:12]) <= 1e-5
