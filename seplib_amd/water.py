@@ -63,8 +63,9 @@ def step_ops(s):
 
 
 def timed(api, w, steps, warmup, precision, concurrent):
-    """Seconds for `steps` graph-replayed steps after `warmup`, and the final
-    positions, velocities, forces and sepret words."""
+    """Seconds for `steps` graph-replayed steps after `warmup`, the final
+    positions, velocities, forces and sepret words, and the list rebuilds
+    of the whole run."""
     s = SepSystem.from_workload(api, w, cf=RC_COUL, dt=DT, nneighb=NNEIGHB)
     try:
         s.gpu_precision(precision)
@@ -81,7 +82,7 @@ def timed(api, w, steps, warmup, precision, concurrent):
         r = s.ret
         final = (st["x"].copy(), st["v"].copy(), np.array(s.view["f"]),
                  np.array([r.epot, r.ecoul, r.ekin]), s.alpha.value)
-        return secs, final
+        return secs, final, st["nupdate_neighb"]
     finally:
         s.close()
 
@@ -130,13 +131,14 @@ def main(argv=None):
                        "replay" % (args.molecules, n),
            "steps": args.steps}
     for name, prec in (("fp64", S.SEP_GPU_FP64), ("mixed", S.SEP_GPU_MIXED)):
-        secs1, fin1 = timed(api, w, args.steps, args.warmup, prec, False)
-        secs2, fin2 = timed(api, w, args.steps, args.warmup, prec, True)
+        secs1, fin1, nup = timed(api, w, args.steps, args.warmup, prec, False)
+        secs2, fin2, _ = timed(api, w, args.steps, args.warmup, prec, True)
         out[name] = {"one_stream_atom_steps_per_s": n * args.steps / secs1,
                      "two_streams_atom_steps_per_s": n * args.steps / secs2,
                      "one_stream_us_per_step": 1e6 * secs1 / args.steps,
                      "two_streams_us_per_step": 1e6 * secs2 / args.steps,
-                     "bit_identical": bool(same(fin1, fin2))}
+                     "bit_identical": bool(same(fin1, fin2)),
+                     "rebuilds": int(nup)}
         print("%s: one stream %.1f us/step, two streams %.1f us/step, identical %s" % (
             name, 1e6 * secs1 / args.steps, 1e6 * secs2 / args.steps,
             out[name]["bit_identical"]), file=sys.stderr, flush=True)
