@@ -31,6 +31,9 @@ class ReferenceAPI:
         # RTLD_LOCAL: the reference's symbols must not interpose libsep.so's
         self.lib = C.CDLL(path, mode=os.RTLD_LOCAL)
         S.bind_api(self.lib, "sep_")
+        S.bind_omp(self.lib, "sep_")
+        self.lib.sep_omp_make_neighb.restype = None
+        self.lib.sep_omp_make_neighb.argtypes = [S.P_ATOMS, C.c_double, S.P_SYS, C.c_uint]
         S.bind_sep_names(self.lib)
         self.lib.sep_set_omp.argtypes = [C.c_uint, S.P_SYS]
 
@@ -72,6 +75,9 @@ class ReferenceAPI:
     def read_topology(self, ptr, path, sysp):
         self.lib.sep_read_topology_file(ptr, path.encode(), sysp, b"q")
 
+    def omp_make_neighb(self, ptr, cf, sysp, opt, nneighb):
+        self.lib.sep_omp_make_neighb(ptr, cf, sysp, opt)
+
     def set_omp(self, nthreads, sysp):
         self.lib.sep_set_omp(nthreads, sysp)
 
@@ -87,6 +93,10 @@ class OracleAPI:
             raise FileNotFoundError(path + " (run make -C oracle)")
         self.lib = C.CDLL(path, mode=os.RTLD_LOCAL)
         S.bind_api(self.lib, "orc_")
+        S.bind_omp(self.lib, "orc_")
+        self.lib.orc_omp_make_neighb.restype = C.c_int
+        self.lib.orc_omp_make_neighb.argtypes = [S.P_ATOMS, C.c_double, S.P_SYS, C.c_uint,
+                                                 C.c_int]
         L = self.lib
 
         def f(name, restype, *args):
@@ -144,3 +154,6 @@ class OracleAPI:
         rc = self.lib.orc_read_topology(ptr, path.encode(), sysp)
         if rc:
             raise RuntimeError("oracle could not read " + path)
+
+    def omp_make_neighb(self, ptr, cf, sysp, opt, nneighb):
+        return self.lib.orc_omp_make_neighb(ptr, cf, sysp, opt, nneighb)

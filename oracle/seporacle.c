@@ -27,6 +27,9 @@
  *   Nose-Hoover, leapfrog -> golden step sequences (bitwise), free-particle
  *                          recurrences, wrap/crossing cases, skin/2 trigger
  *   topology reader     -> the reference's own .top inputs (golden vectors)
+ *   Model II terms (orc_omp_*) -> the compiled reference's sep_omp_* run
+ *                          live (bitwise), and the same forces as the
+ *                          sep_* term of each (the cos^2 form for the angle)
  */
 #include "seporacle.h"
 
@@ -806,6 +809,187 @@ void orc_torsion_ryckaert(seppart *p, int type, const double *g, sepsys *s,
 		}
 		r->epot += g[0] + (g[1] + (g[2] + (g[3] + (g[4] + g[5] * cc) * cc) * cc) * cc) * cc;
 		m->dihedrals[n] = SEP_PI - acos(cc);
+	}
+}
+
+/* ------------------------------------------------------------------ */
+/* OpenMP Model II terms (source/sepomp.c:35-328)                      */
+
+/* source/sepomp.c:16-33: the list for opt when the rebuild flag is set;
+ * returns -1 for a cutoff above sys->cf or a list error */
+int orc_omp_make_neighb(seppart *p, double cf, sepsys *s, unsigned opt, int nneighb)
+{
+	if (cf > s->cf)
+		return -1;
+	if (s->neighb_flag == 1) {
+		if (orc_build_list(p, s, opt, nneighb))
+			return -1;
+		s->neighb_flag = 0;
+	}
+	return 0;
+}
+
+/* Pair terms over the half lists: atom i1 of the types, each listed i2 of
+ * the other type, force ft r added to i1 and taken from i2, in list order
+ * (source/sepomp.c:35-79, 81-133, 136-176). */
+static int omp_types_ok(const seppart *p, long a, long b, const char *t)
+{
+	return (p[a].type == t[0] && p[b].type == t[1]) ||
+	       (p[a].type == t[1] && p[b].type == t[0]);
+}
+
+static void omp_add_pair(double **ftot, long a, long b, double ft, const double *r)
+{
+	double f[3];
+	for (int k = 0; k < 3; k++)
+		f[k] = ft * r[k];
+	for (int k = 0; k < 3; k++) {
+		ftot[a][k] += f[k];
+		ftot[b][k] -= f[k];
+	}
+}
+
+void orc_omp_pairs(double **ftot, const seppart *p, const char *types, double cf,
+		   orc_pot_fn fun, const sepsys *s)
+{
+	const double cf2 = cf * cf;
+	for (long i1 = 0; i1 < s->npart; i1++) {
+		if (p[i1].type != types[0] && p[i1].type != types[1])
+			continue;
+		for (int n = 0; p[i1].neighb[n] != -1; n++) {
+			const long i2 = p[i1].neighb[n];
+			if (!omp_types_ok(p, i1, i2, types))
+				continue;
+			double r[3];
+			const double r2 = separation(p, i1, i2, s->length, r);
+			if (r2 < cf2)
+				omp_add_pair(ftot, i1, i2, fun(r2, 'f'), r);
+		}
+	}
+}
+
+void orc_omp_pairs_lj(double **ftot, const seppart *p, const char *types,
+		      const double *param, const sepsys *s)
+{
+	const double cf = param[0], sigma = param[1], epsilon = param[2];
+	const double cf2 = cf * cf;
+	for (long i1 = 0; i1 < s->npart; i1++) {
+		if (p[i1].type != types[0] && p[i1].type != types[1])
+			continue;
+		for (int n = 0; p[i1].neighb[n] != -1; n++) {
+			const long i2 = p[i1].neighb[n];
+			if (!omp_types_ok(p, i1, i2, types))
+				continue;
+			double r[3];
+			const double r2 = separation(p, i1, i2, s->length, r);
+			if (r2 < cf2) {
+				/* source/sepomp.c:112-115, left to right */
+				const double rri = sigma / r2;
+				const double rri3 = rri * rri * rri;
+				const double ft = 48.0 / epsilon * rri3 * (rri3 - 0.5) * rri;
+				omp_add_pair(ftot, i1, i2, ft, r);
+			}
+		}
+	}
+}
+
+void orc_omp_coulomb(double **ftot, const seppart *p, double cf, const sepsys *s)
+{
+	const double cf2 = cf * cf;
+	const double icf2 = 1.0 / cf2;
+	for (long i1 = 0; i1 < s->npart; i1++)
+		for (int n = 0; p[i1].neighb[n] != -1; n++) {
+			const long i2 = p[i1].neighb[n];
+			double r[3];
+			const double r2 = separation(p, i1, i2, s->length, r);
+			if (r2 < cf2) {
+				const double zizj = p[i2].z * p[i1].z;
+				const double rr = sqrt(r2);
+				omp_add_pair(ftot, i1, i2, zizj * (1.0 / r2 - icf2) / rr, r);
+			}
+		}
+}
+
+/* source/sepomp.c:179-212 */
+void orc_omp_bond(double **ftot, const seppart *p, int type, double lbond, double ks,
+		  const sepsys *s)
+{
+	const sepmolinfo *m = s->molptr;
+	for (unsigned n = 0; n < m->num_bonds; n++) {
+		if ((int)m->blist[3 * n + 2] != type)
+			continue;
+		const long a = m->blist[3 * n], b = m->blist[3 * n + 1];
+		double r[3];
+		const double dist = sqrt(separation(p, a, b, s->length, r));
+		const double ft = -ks * (dist - lbond) / dist;
+		for (int k = 0; k < 3; k++) {
+			const double f = ft * r[k];
+			ftot[a][k] += f;
+			ftot[b][k] -= f;
+		}
+	}
+}
+
+/* source/sepomp.c:215-261: the cosine-squared force */
+void orc_omp_angle(double **ftot, const seppart *p, int type, double angle0, double k,
+		   const sepsys *s)
+{
+	const double ccon = cos(SEP_PI - angle0);
+	const sepmolinfo *m = s->molptr;
+	for (unsigned n = 0; n < m->num_angles; n++) {
+		if ((int)m->alist[4 * n + 3] != type)
+			continue;
+		const long a = m->alist[4 * n], b = m->alist[4 * n + 1], c = m->alist[4 * n + 2];
+		double d1[3], d2[3];
+		angle_vectors(p, a, b, c, s->length, d1, d2);
+		const double c11 = dot3(d1, d1), c12 = dot3(d1, d2), c22 = dot3(d2, d2);
+		const double cd = sqrt(c11 * c22);
+		const double f = -k * (c12 / cd - ccon);
+		for (int q = 0; q < 3; q++) {
+			const double f1 = f * ((c12 / c11) * d1[q] - d2[q]) / cd;
+			const double f2 = f * (d1[q] - (c12 / c22) * d2[q]) / cd;
+			ftot[a][q] += f1;
+			ftot[b][q] += (-f1 - f2);
+			ftot[c][q] += f2;
+		}
+	}
+}
+
+/* source/sepomp.c:264-328 */
+void orc_omp_torsion(double **ftot, const seppart *p, int type, const double *g,
+		     const sepsys *s)
+{
+	const sepmolinfo *m = s->molptr;
+	for (unsigned n = 0; n < m->num_dihedrals; n++) {
+		if ((int)m->dlist[5 * n + 4] != type)
+			continue;
+		const long a = m->dlist[5 * n], b = m->dlist[5 * n + 1];
+		const long c = m->dlist[5 * n + 2], d = m->dlist[5 * n + 3];
+		double d1[3], d2[3], d3[3];
+		for (int k = 0; k < 3; k++) {
+			d1[k] = min_image(p[b].x[k] - p[a].x[k], s->length[k]);
+			d2[k] = min_image(p[c].x[k] - p[b].x[k], s->length[k]);
+			d3[k] = min_image(p[d].x[k] - p[c].x[k], s->length[k]);
+		}
+		const double c11 = dot3(d1, d1), c12 = dot3(d1, d2), c13 = dot3(d1, d3);
+		const double c22 = dot3(d2, d2), c23 = dot3(d2, d3), c33 = dot3(d3, d3);
+		const double ca = c13 * c22 - c12 * c23;
+		const double cb1 = c11 * c22 - c12 * c12;
+		const double cb2 = c22 * c33 - c23 * c23;
+		const double cd = sqrt(cb1 * cb2);
+		const double cc = ca / cd;
+		const double f = -(g[1] + (2. * g[2] + (3. * g[3] + (4. * g[4] + 5. * g[5] * cc) * cc) * cc) * cc);
+		const double t1 = ca, t2 = c11 * c23 - c12 * c13, t3 = -cb1;
+		const double t4 = cb2, t5 = c13 * c23 - c12 * c33, t6 = -ca;
+		const double cr1 = c12 / c22, cr2 = c23 / c22;
+		for (int k = 0; k < 3; k++) {
+			const double f1 = f * c22 * (t1 * d1[k] + t2 * d2[k] + t3 * d3[k]) / (cd * cb1);
+			const double f2 = f * c22 * (t4 * d1[k] + t5 * d2[k] + t6 * d3[k]) / (cd * cb2);
+			ftot[a][k] += f1;
+			ftot[b][k] += (-(1.0 + cr1) * f1 + cr2 * f2);
+			ftot[c][k] += (cr1 * f1 - (1.0 + cr2) * f2);
+			ftot[d][k] += f2;
+		}
 	}
 }
 

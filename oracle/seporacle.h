@@ -57,6 +57,22 @@ void orc_angle_cossq(seppart *p, int type, double angle0, double k,
 		     sepsys *s, sepret *r);
 void orc_torsion_ryckaert(seppart *p, int type, const double *g, sepsys *s,
 			  sepret *r);
+/* OpenMP Model II terms (source/sepomp.c:35-328): forces added to ftot
+ * (npart row pointers of 3 doubles) only -- no energy, stress or per-term
+ * geometry -- the pair terms over the half lists already in p[i].neighb */
+int orc_omp_make_neighb(seppart *p, double cf, sepsys *s, unsigned opt, int nneighb);
+void orc_omp_pairs(double **ftot, const seppart *p, const char *types, double cf,
+		   orc_pot_fn fun, const sepsys *s);
+void orc_omp_pairs_lj(double **ftot, const seppart *p, const char *types,
+		      const double *param, const sepsys *s);
+void orc_omp_coulomb(double **ftot, const seppart *p, double cf, const sepsys *s);
+void orc_omp_bond(double **ftot, const seppart *p, int type, double lbond, double ks,
+		  const sepsys *s);
+void orc_omp_angle(double **ftot, const seppart *p, int type, double angle0, double k,
+		   const sepsys *s);
+void orc_omp_torsion(double **ftot, const seppart *p, int type, const double *g,
+		     const sepsys *s);
+
 /* .top reader (source/sepmol.c:22-369); returns 0 or -1 */
 int orc_read_topology(seppart *p, const char *file, sepsys *s);
 

@@ -285,7 +285,7 @@ out:
 enum { BONDED_BOND, BONDED_ANGLE, BONDED_COSSQ, BONDED_TORSION };
 
 static void bonded_call(seppart *ptr, int kind, int type, const double *prm,
-			sepsys *sys, sepret *ret)
+			sepsys *sys, sepret *ret, int values)
 {
 	seph_ctx *c = seph_ctx_for(ptr, sys);
 	int rc = 0;
@@ -326,8 +326,9 @@ static void bonded_call(seppart *ptr, int kind, int type, const double *prm,
 		goto out;
 	if ((rc = sync_ret_end(c, ret, 0, kind == BONDED_BOND)))
 		goto out;
-	/* per-term geometry is written only for terms of this type */
-	{
+	/* per-term geometry is written only for terms of this type (the Model
+	 * II terms leave it alone, source/sepomp.c:179-328) */
+	if (values) {
 		sepmolinfo *m = sys->molptr;
 		const unsigned nterm = kind == BONDED_BOND ? c->topo_nb
 				       : kind == BONDED_TORSION ? c->topo_nd : c->topo_na;
@@ -359,27 +360,27 @@ void sep_stretch_harmonic(sepatom *aptr, int type, const double lbond,
 			  const double ks, sepsys *sys, sepret *ret)
 {
 	const double p[2] = {lbond, ks};
-	bonded_call(aptr, BONDED_BOND, type, p, sys, ret);
+	bonded_call(aptr, BONDED_BOND, type, p, sys, ret, 1);
 }
 
 void sep_angle_harmonic(sepatom *ptr, int type, const double angle0,
 			const double k, sepsys *sys, sepret *ret)
 {
 	const double p[2] = {angle0, k};
-	bonded_call(ptr, BONDED_ANGLE, type, p, sys, ret);
+	bonded_call(ptr, BONDED_ANGLE, type, p, sys, ret, 1);
 }
 
 void sep_angle_cossq(sepatom *ptr, int type, const double angle0,
 		     const double k, sepsys *sys, sepret *ret)
 {
 	const double p[2] = {angle0, k};
-	bonded_call(ptr, BONDED_COSSQ, type, p, sys, ret);
+	bonded_call(ptr, BONDED_COSSQ, type, p, sys, ret, 1);
 }
 
 void sep_torsion_Ryckaert(sepatom *ptr, int type, const double g[6],
 			  sepsys *sys, sepret *ret)
 {
-	bonded_call(ptr, BONDED_TORSION, type, g, sys, ret);
+	bonded_call(ptr, BONDED_TORSION, type, g, sys, ret, 1);
 }
 
 /* ---- Nose-Hoover: source/sepintgr.c:149-168 --------------------------- */
@@ -515,4 +516,192 @@ int sep_gpu_lj_adjoint(seppart *ptr, sepsys *sys, int ntypes, const char *types,
 	for (int k = 0; k < 3; k++)
 		ta.L[k] = sys->length[k];
 	return seph_dev_fail(sepdev_lj_adjoint(c->dev, &ta, dLdF, dLdx, dLdtheta));
+}
+
+/* ---- OpenMP Model II: source/sepomp.c:16-328 ---------------------------- */
+
+/* A Model II term runs through the synchronised path of its sep_*
+ * counterpart with the atoms' forces parked: the path then leaves exactly
+ * the term's forces in f, which go to ftot before the parked forces come
+ * back.  Energies land in a scratch sepret and are dropped, as the
+ * reference computes none. */
+typedef struct {
+	seppart *ptr;
+	long n;
+	double *saved;
+	int err_before;
+} omp_term;
+
+static int omp_begin(omp_term *t, seppart *ptr, sepsys *sys, const char *who)
+{
+	seph_ctx *c = seph_ctx_for(ptr, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	if (c->resident) {
+		sep_warning("%s: a resident system runs concurrent terms through sep_gpu_run", who);
+		return seph_fail(SEP_GPU_ESTATE, "%s on a resident system", who);
+	}
+	t->ptr = ptr;
+	t->n = sys->npart;
+	t->err_before = sep_gpu_last_error();
+	t->saved = malloc(sizeof(double) * 3 * (size_t)t->n);
+	if (!t->saved)
+		return seph_fail(SEP_GPU_EHOST, "%s: out of host memory", who);
+	for (long i = 0; i < t->n; i++)
+		for (int k = 0; k < 3; k++) {
+			t->saved[3 * i + k] = ptr[i].f[k];
+			ptr[i].f[k] = 0.0;
+		}
+	return 0;
+}
+
+/* rc: the term's own status; a void sep_* call reports through the error
+ * state instead */
+static void omp_end(omp_term *t, double **ftot, int rc)
+{
+	const int ok = rc == 0 && sep_gpu_last_error() == t->err_before;
+	for (long i = 0; i < t->n; i++)
+		for (int k = 0; k < 3; k++) {
+			if (ok)
+				ftot[i][k] += t->ptr[i].f[k];
+			t->ptr[i].f[k] = t->saved[3 * i + k];
+		}
+	free(t->saved);
+}
+
+void sep_omp_make_neighb(seppart *ptr, double cf, sepsys *sys, const unsigned opt)
+{
+	if (cf > sys->cf) {
+		sep_warning("cutoff for an interaction cannot be larger than maximum cutoff");
+		seph_fail(SEP_GPU_EARG, "cutoff %f above the system cutoff %f", cf, sys->cf);
+		return;
+	}
+	if (sys->neighb_flag != 1)
+		return;
+	seph_ctx *c = seph_ctx_for(ptr, sys);
+	int rc = c ? 0 : sep_gpu_last_error();
+	if (!rc && c->resident)
+		rc = seph_fail(SEP_GPU_ESTATE, "sep_omp_make_neighb on a resident system");
+	if (!rc && ((rc = seph_put_x(c)) || (rc = seph_put_scalars(c))))
+		;
+	if (!rc && needs_partners(opt))
+		rc = seph_put_partners(c);
+	if (!rc) {
+		sepdev_grid g;
+		if (sys->neighb_update == SEP_NEIGHBLIST) {
+			const double rl = sys->cf + sys->skin;
+			seph_grid_own(sys, rl, opt, 0, &g);
+			g.rlist2 = rl * rl;
+		} else {
+			seph_grid_reference(sys, opt, &g);
+		}
+		rc = seph_build(c, sys, &g, 0);
+	}
+	if (rc) {
+		sep_warning("sep_omp_make_neighb: %s", (char *)sep_gpu_last_error_message());
+		return;
+	}
+	sys->neighb_flag = 0;
+}
+
+/* The pair terms use the list as it stands, as the reference's do: a set
+ * rebuild flag is held off for the call. */
+static void omp_pairs(double **ftot, seppart *ptr, sepsys *sys, pair_call *pc,
+		      const char *who)
+{
+	omp_term t;
+	if (omp_begin(&t, ptr, sys, who))
+		return;
+	sepret r;
+	sep_reset_retval(&r);
+	const unsigned flag = sys->neighb_flag;
+	sys->neighb_flag = 0;
+	const int rc = run_pairs(ptr, sys, &r, pc);
+	sys->neighb_flag = flag;
+	if (rc)
+		sep_warning("%s: %s", who, (char *)sep_gpu_last_error_message());
+	omp_end(&t, ftot, rc);
+}
+
+void sep_omp_pairs(double **ftot, const seppart *ptr, const char *types, double cf,
+		   double (*fun)(double, char), const sepsys *sys)
+{
+	const int pot = seph_identify_potential(fun);
+	if (pot == SEPH_POT_NONE) {
+		sep_warning("sep_omp_pairs: the device path evaluates sep_lj, sep_lj_shift and sep_wca only");
+		seph_fail(SEP_GPU_EUNSUPPORTED, "potential function not available on the device");
+		return;
+	}
+	pair_call pc;
+	memset(&pc, 0, sizeof pc);
+	pc.types = types;
+	pc.cf = cf;
+	pc.opt = SEP_ALL;
+	pc.lj.eps48 = 48.0;
+	pc.lj.eps4 = 4.0;
+	pc.lj.awh = 0.5;
+	pc.lj.aw = 1.0;
+	pc.lj.sig2 = 1.0;
+	omp_pairs(ftot, (seppart *)ptr, (sepsys *)sys, &pc, "sep_omp_pairs");
+}
+
+void sep_omp_pairs_lj(double **ftot, const seppart *ptr, const char *types,
+		      const double *param, const sepsys *sys)
+{
+	/* source/sepomp.c:112-118: rri = sigma/r^2, ft = 48/eps rri^3 (rri^3 -
+	 * 1/2) rri -- the pair kernel's form with sig2 = sigma, eps48 = 48/eps */
+	pair_call pc;
+	memset(&pc, 0, sizeof pc);
+	pc.types = types;
+	pc.cf = param[0];
+	pc.opt = SEP_ALL;
+	pc.lj.eps48 = 48.0 / param[2];
+	pc.lj.eps4 = 4.0 / param[2];
+	pc.lj.awh = 0.5;
+	pc.lj.aw = 1.0;
+	pc.lj.sig2 = param[1];
+	omp_pairs(ftot, (seppart *)ptr, (sepsys *)sys, &pc, "sep_omp_pairs_lj");
+}
+
+void sep_omp_coulomb(double **ftot, seppart *ptr, double cf, sepsys *sys)
+{
+	omp_term t;
+	if (omp_begin(&t, ptr, sys, "sep_omp_coulomb"))
+		return;
+	sepret r;
+	sep_reset_retval(&r);
+	sep_coulomb_sf(ptr, cf, sys, &r, SEP_ALL);
+	omp_end(&t, ftot, 0);
+}
+
+static void omp_bonded(double **ftot, seppart *ptr, int kind, int type, const double *prm,
+		       sepsys *sys, const char *who)
+{
+	omp_term t;
+	if (omp_begin(&t, ptr, sys, who))
+		return;
+	sepret r;
+	sep_reset_retval(&r);
+	bonded_call(ptr, kind, type, prm, sys, &r, 0);
+	omp_end(&t, ftot, 0);
+}
+
+void sep_omp_bond(double **ftot, seppart *aptr, int type, const double lbond,
+		  const double ks, sepsys *sys)
+{
+	const double p[2] = {lbond, ks};
+	omp_bonded(ftot, aptr, BONDED_BOND, type, p, sys, "sep_omp_bond");
+}
+
+void sep_omp_angle(double **ftot, seppart *ptr, int type, const double angle0,
+		   const double k, sepsys *sys)
+{
+	/* source/sepomp.c:217-250: the cosine-squared force */
+	const double p[2] = {angle0, k};
+	omp_bonded(ftot, ptr, BONDED_COSSQ, type, p, sys, "sep_omp_angle");
+}
+
+void sep_omp_torsion(double **ftot, seppart *ptr, int type, const double g[6], sepsys *sys)
+{
+	omp_bonded(ftot, ptr, BONDED_TORSION, type, g, sys, "sep_omp_torsion");
 }

@@ -169,6 +169,31 @@ def bind_api(lib, p):
         f(nm, C.c_double, C.c_double, C.c_char)
 
 
+P_MAT = C.POINTER(C.POINTER(C.c_double))
+
+
+def bind_omp(lib, p):
+    """The OpenMP Model II term signatures (include/sepomp.h) with prefix p."""
+    def f(name, *args):
+        fn = getattr(lib, p + name)
+        fn.restype = None
+        fn.argtypes = list(args)
+    f("omp_pairs", P_MAT, P_ATOMS, C.c_char_p, C.c_double, C.c_void_p, P_SYS)
+    f("omp_pairs_lj", P_MAT, P_ATOMS, C.c_char_p, P_D, P_SYS)
+    f("omp_coulomb", P_MAT, P_ATOMS, C.c_double, P_SYS)
+    f("omp_bond", P_MAT, P_ATOMS, C.c_int, C.c_double, C.c_double, P_SYS)
+    f("omp_angle", P_MAT, P_ATOMS, C.c_int, C.c_double, C.c_double, P_SYS)
+    f("omp_torsion", P_MAT, P_ATOMS, C.c_int, P_D, P_SYS)
+
+
+def row_pointers(a):
+    """A double ** over the rows of a C-contiguous float64 (n, 3) array."""
+    base = a.ctypes.data
+    rows = (C.POINTER(C.c_double) * len(a))(
+        *[C.cast(base + i * a.strides[0], C.POINTER(C.c_double)) for i in range(len(a))])
+    return C.cast(rows, P_MAT), rows
+
+
 def load_product(path=PRODUCT_LIB):
     """The MI355X library; raises if it has not been built."""
     if not os.path.exists(path):
@@ -176,6 +201,9 @@ def load_product(path=PRODUCT_LIB):
     lib = C.CDLL(path)
     bind_api(lib, "sep_")
     bind_sep_names(lib)
+    bind_omp(lib, "sep_")
+    lib.sep_omp_make_neighb.restype = None
+    lib.sep_omp_make_neighb.argtypes = [P_ATOMS, C.c_double, P_SYS, C.c_uint]
     f = lambda n, r, *a: (setattr(getattr(lib, n), "restype", r),
                           setattr(getattr(lib, n), "argtypes", list(a)))
     f("sep_gpu_last_error", C.c_int)

@@ -64,6 +64,9 @@ class ProductAPI:
     def torsion(self, ptr, type_, g, sysp, retp):
         self.lib.sep_torsion_Ryckaert(ptr, type_, g, sysp, retp)
 
+    def omp_make_neighb(self, ptr, cf, sysp, opt, nneighb):
+        self.lib.sep_omp_make_neighb(ptr, cf, sysp, opt)
+
     def read_topology(self, ptr, path, sysp):
         self.lib.sep_read_topology_file(ptr, path.encode(), sysp, b"q")
 
@@ -197,6 +200,17 @@ class SepSystem:
 
     def angle_cossq(self, type_, angle0, k):
         self.api.fn("angle_cossq")(self.atoms, type_, angle0, k, self.sysp, self.retp)
+
+    def omp_make_neighb(self, cf, opt=S.SEP_ALL):
+        return self.api.omp_make_neighb(self.atoms, cf, self.sysp, opt, self.nneighb)
+
+    def omp_term(self, name, *args):
+        """One OpenMP Model II term (sep_omp_<name>) into a fresh N x 3
+        array, returned."""
+        f = np.zeros((self.n, 3))
+        mat, _rows = S.row_pointers(f)
+        self.api.fn("omp_" + name)(mat, self.atoms, *args, self.sysp)
+        return f
 
     def torsion(self, type_, g):
         self.api.torsion(self.atoms, type_, (C.c_double * 6)(*g), self.sysp, self.retp)

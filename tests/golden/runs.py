@@ -4,6 +4,7 @@ and on the product and compare with the stored vectors).  Each driver takes
 any library with the seppart ABI as ``api`` and returns plain arrays; the
 call sequences follow the reference's own programs prgs/prg0.c (LJ),
 prgs/prg2.c (butane) and prgs/prg3.c (water)."""
+import ctypes as C
 import hashlib
 
 import numpy as np
@@ -230,3 +231,37 @@ def ka_workload():
 def run_ka(api, w, prepare=None):
     calls = [(lambda t: (lambda s: s.force_lj(t, KA[t])))(t) for t in ("AA", "AB", "BB")]
     return term_run(api, w, 2.5, S.SEP_LLIST_NEIGHBLIST, calls, skin=0.3, prepare=prepare)
+
+
+# ---- OpenMP Model II terms (source/sepomp.c, prgs/prg5.c) -----------------------
+
+def model2_water(s):
+    p = WATER
+    return [("pairs", b"OO", 2.5, s.api.pot("sep_lj_shift")),
+            ("pairs_lj", b"OO", (C.c_double * 3)(2.5, 1.0, 1.0)),
+            ("coulomb", p["cf"]),
+            ("bond", 0, p["lbond"], p["kspring"]),
+            ("angle", 0, p["angle"], p["kangle"])]
+
+
+def model2_butane(s):
+    p = BUTANE
+    return [("pairs", b"CC", 2.5, s.api.pot("sep_lj_shift")),
+            ("pairs_lj", b"CC", (C.c_double * 3)(2.5, 1.1, 0.9)),
+            ("bond", 0, p["lbond"], p["kspring"]),
+            ("angle", 0, p["angle"], p["kangle"]),
+            ("torsion", 0, (C.c_double * 6)(*p["rb"]))]
+
+
+def run_model2(api, w, cf, opt, terms, nneighb=S.SEP_NEIGHB):
+    """sep_omp_make_neighb(cf, opt), then each Model II term into its own
+    zeroed array: the arrays, and the atoms' f afterwards (which the terms
+    must leave alone)."""
+    s = SepSystem.from_workload(api, w, cf=cf, dt=0.001, nneighb=nneighb)
+    try:
+        f0 = np.array(s.view["f"])
+        s.omp_make_neighb(cf, opt)
+        out = [s.omp_term(t[0], *t[1:]) for t in terms(s)]
+        return np.array(out), f0, np.array(s.view["f"])
+    finally:
+        s.close()

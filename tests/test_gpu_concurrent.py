@@ -14,7 +14,11 @@ the oracle by test_gpu_water.py and test_gpu_golden.py.
     angle, cosine-squared angle and Ryckaert-Bellemans torsion, where the
     second angle call reuses the angle family's per-term slots and so must
     wait for the first one's gather;
-  * a step with no pair call before its bonded calls runs on one stream.
+  * a step with no pair call before its bonded calls runs on one stream;
+  * the Model II entry points themselves (sep_omp_make_neighb and the
+    sep_omp_* terms into caller-owned arrays) against the oracle to 1e-12
+    on water and butane, leaving the atoms' forces alone, and refused on a
+    resident system.
 """
 import os
 
@@ -107,3 +111,42 @@ def test_bonded_before_pairs_stays_on_one_stream(product, water):
     kw = dict(cf=WAT.RC_COUL, dt=WAT.DT, nneighb=WAT.NNEIGHB)
     _assert_same(_run(product, water, ops, 10, True, False, **kw),
                  _run(product, water, ops, 10, True, True, **kw))
+
+
+# ---- the Model II entry points themselves (sep_omp_*, include/sepomp.h) ----------
+
+def _model2_case(name):
+    if name == "water":
+        return WAT.system(1500), R.WATER["cf"], R.model2_water
+    w = R.workload_from(np.load(os.path.join(GOLD, "g3_butane.npz"), allow_pickle=False))
+    return w, 2.5, R.model2_butane
+
+
+@pytest.mark.parametrize("name", ["water", "butane"])
+def test_model2_terms_match_oracle(product, oracle, name):
+    """sep_omp_make_neighb, then sep_omp_pairs, _pairs_lj, _coulomb, _bond,
+    _angle and _torsion into caller arrays: against the oracle's
+    restatement (pinned bitwise to the compiled reference) to 1e-12, and
+    the atoms' own forces left as they were."""
+    from tests.parity import FP64_TOL, force_error
+    w, cf, terms = _model2_case(name)
+    a, f0, f1 = R.run_model2(product, w, cf, S.SEP_EXCL_SAME_MOL, terms)
+    b, _, _ = R.run_model2(oracle, w, cf, S.SEP_EXCL_SAME_MOL, terms)
+    assert np.array_equal(f0, f1)
+    for k in range(len(a)):
+        assert np.abs(b[k]).max() > 0, k
+        assert force_error(a[k], b[k]) <= FP64_TOL, (name, k, force_error(a[k], b[k]))
+
+
+def test_model2_refused_on_resident_system(product):
+    w, cf, terms = _model2_case("butane")
+    s = SepSystem.from_workload(product, w, cf=cf, dt=0.001, nneighb=S.SEP_NEIGHB)
+    try:
+        s.gpu_resident(True)
+        product.lib.sep_gpu_clear_error()
+        f = s.omp_term("bond", 0, R.BUTANE["lbond"], R.BUTANE["kspring"])
+        assert product.lib.sep_gpu_last_error() == S.SEP_GPU_ESTATE
+        assert not f.any()
+        product.lib.sep_gpu_clear_error()
+    finally:
+        s.close()

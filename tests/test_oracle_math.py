@@ -397,3 +397,49 @@ def test_potential_functions_two_body(oracle, fun, cf, shift):
                 assert s.ret.epot == 0.0 and np.abs(f).max() == 0.0, (fun, r)
         finally:
             s.close()
+
+
+def _model2_cases():
+    import os
+    from tests.golden import runs as R
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "g3_butane.npz"),
+                allow_pickle=False)
+    water_calls = [
+        lambda s: s.force_pairs("OO", 2.5, "sep_lj_shift", S.SEP_EXCL_SAME_MOL),
+        lambda s: s.force_lj("OO", [2.5, 1.0, 1.0, 1.0], S.SEP_EXCL_SAME_MOL),
+        lambda s: s.coulomb_sf(R.WATER["cf"], S.SEP_EXCL_SAME_MOL),
+        lambda s: s.stretch_harmonic(0, R.WATER["lbond"], R.WATER["kspring"]),
+        lambda s: s.angle_cossq(0, R.WATER["angle"], R.WATER["kangle"]),
+    ]
+    b = R.BUTANE
+    butane_calls = [
+        lambda s: s.force_pairs("CC", 2.5, "sep_lj_shift", S.SEP_EXCL_SAME_MOL),
+        # Model II's (sigma, eps) = (1.1, 0.9) is sep_force_lj's
+        # sigma^2 = 1.1, eps = 1 / 0.9 with aw = 1
+        lambda s: s.force_lj("CC", [2.5, 1.0 / 0.9, math.sqrt(1.1), 1.0],
+                             S.SEP_EXCL_SAME_MOL),
+        lambda s: s.stretch_harmonic(0, b["lbond"], b["kspring"]),
+        lambda s: s.angle_cossq(0, b["angle"], b["kangle"]),
+        lambda s: s.torsion(0, b["rb"]),
+    ]
+    return [(W.water(300, 3.16, 3.86, seed=31), R.WATER["cf"], R.model2_water, water_calls),
+            (R.workload_from(g), 2.5, R.model2_butane, butane_calls)]
+
+
+def test_model2_terms_are_the_sep_terms(oracle):
+    """Each OpenMP Model II term (orc_omp_*, source/sepomp.c) gives the
+    forces of the oracle's own sep_* term over the same list (another
+    accumulation order: 1e-12), the angle being the cosine-squared form and
+    sep_omp_pairs_lj the LJ force with sigma entering unsquared and
+    epsilon inverted; the atoms' f is left alone."""
+    from tests.golden import runs as R
+    for w, cf, terms, calls in _model2_cases():
+        m2, f0, f1 = R.run_model2(oracle, w, cf, S.SEP_EXCL_SAME_MOL, terms)
+        assert np.array_equal(f0, f1)
+        for k, call in enumerate(calls):
+            # each sep_* term on its own, from f = 0, after the first call
+            # has built the list (the list path of Coulomb builds none)
+            ref = R.term_run(oracle, w, cf, S.SEP_LLIST_NEIGHBLIST, [call],
+                             prepare=calls[0])[0][0]
+            assert np.abs(ref).max() > 0, k
+            assert force_error(m2[k], ref) <= 1e-12, (k, force_error(m2[k], ref))

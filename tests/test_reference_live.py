@@ -35,3 +35,24 @@ def test_water_generated_bitwise(reference, oracle):
     for update in (S.SEP_LLIST_NEIGHBLIST, S.SEP_BRUTE):
         for x, y in zip(R.run_water(reference, w, update), R.run_water(oracle, w, update)):
             assert np.array_equal(x, y)
+
+
+def _butane():
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "g3_butane.npz"),
+                allow_pickle=False)
+    return R.workload_from(g)
+
+
+def test_model2_terms_bitwise(reference, oracle):
+    """The OpenMP Model II terms (source/sepomp.c) after sep_omp_make_neighb:
+    a generated water box and the reference's butane input."""
+    cases = [(W.water(400, 3.16, 3.86, seed=24), R.WATER["cf"], S.SEP_EXCL_SAME_MOL,
+              R.model2_water),
+             (_butane(), 2.5, S.SEP_EXCL_SAME_MOL, R.model2_butane)]
+    for w, cf, opt, terms in cases:
+        a, _, fa = R.run_model2(reference, w, cf, opt, terms)
+        b, _, fb = R.run_model2(oracle, w, cf, opt, terms)
+        assert np.abs(a).max() > 0
+        assert np.array_equal(a, b)
+        assert np.array_equal(fa, fb)
