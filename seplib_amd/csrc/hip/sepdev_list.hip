@@ -354,10 +354,10 @@ __device__ __forceinline__ void sepk_copy_home(const sepk_state &src, const sepk
 // its d - L / d + L, and d + 0 is d.  Otherwise the wrap is evaluated per
 // candidate.
 #define SEPK_BT 256      // threads of a build workgroup (4 waves)
-#define SEPK_BCHUNK 640  // candidates staged at a time
+#define SEPK_BCHUNK 384  // candidates staged at a time
 #define SEPK_BWORDS (SEPK_BCHUNK / 64)
 #define SEPK_BHOME 64    // home atoms staged at a time
-#define SEPK_BGRID 1280  // build workgroups at most (a multiple of 8)
+#define SEPK_BGRID 1792  // build workgroups at most (a multiple of 8)
 static_assert(SEPK_BCHUNK % 64 == 0, "candidates are tested 64 at a time");
 
 struct sepk_build_lds {
@@ -651,7 +651,7 @@ __device__ __forceinline__ void sepk_build_cell(
 // the result is deterministic).  The permuted state of the workgroup's cells
 // goes home at the end, in one batch, so its memory round trip is paid once.
 template <int EXCL>
-__global__ __launch_bounds__(SEPK_BT) __attribute__((amdgpu_waves_per_eu(EXCL == 2 ? 4 : 5))) void k_build(
+__global__ __launch_bounds__(SEPK_BT) __attribute__((amdgpu_waves_per_eu(EXCL == 2 ? 4 : 7))) void k_build(
 	sepk_state s, sepk_state home, const int *start, sepdev_grid g, long ncell, int nslice,
 	long n, long npad, int cap, int *nbr, int *cnt, sepdev_sys *sys, int conditional,
 	int has_partners)
