@@ -560,10 +560,10 @@ __global__ __launch_bounds__(SEPB_BLOCK) void k_bsort(sepb_view b)
 // (home atom, 64-candidate word) finds the word's first entry number from
 // the atom's running length and the earlier words' popcounts and writes
 // the set bits in order, so every list is in candidate order.
-#define SEPB_BCHUNK 640
+#define SEPB_BCHUNK 384
 #define SEPB_BWORDS (SEPB_BCHUNK / 64)
 #define SEPB_BHOME 64
-#define SEPB_BGRID 1280 // build workgroups (5 per CU)
+#define SEPB_BGRID 1792 // build workgroups (7 per CU)
 static_assert(SEPB_BCHUNK % 64 == 0, "candidates are tested 64 at a time");
 
 struct sepb_build_lds {
@@ -644,7 +644,7 @@ __device__ __forceinline__ void sepb_build_emit(sepb_build_lds &L, int kn, int h
 	}
 }
 
-__global__ __launch_bounds__(SEPB_BLOCK) void k_bbuild(sepb_view b)
+__global__ __launch_bounds__(SEPB_BLOCK) __attribute__((amdgpu_waves_per_eu(7))) void k_bbuild(sepb_view b)
 {
 	__shared__ sepb_build_lds L;
 	__shared__ int fpre[SEPB_FMAP + 1];
