@@ -927,7 +927,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 // the pair loop and its rows as in k_lj, then the atom's integrator step
 // from its summed force, with the new positions to o.pos / o.qpos.
 template <bool MIXED, bool FAST>
-__global__ __launch_bounds__(SEPK_BLOCK) void k_lj_step(
+__global__ __launch_bounds__(SEPK_BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED || FAST ? 7 : 5))) void k_lj_step(
 	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt, long n,
 	long npad, sepk_lj c, const sepdev_sys *sys, int nh, const double *alpha, double Td,
 	double dt_tau2, double dt, double qs0, double qs1, double qs2, sepk_lfout o,
