@@ -582,8 +582,10 @@ void sep_omp_make_neighb(seppart *ptr, double cf, sepsys *sys, const unsigned op
 	int rc = c ? 0 : sep_gpu_last_error();
 	if (!rc && c->resident)
 		rc = seph_fail(SEP_GPU_ESTATE, "sep_omp_make_neighb on a resident system");
-	if (!rc && ((rc = seph_put_x(c)) || (rc = seph_put_scalars(c))))
-		;
+	if (!rc)
+		rc = seph_put_x(c);
+	if (!rc)
+		rc = seph_put_scalars(c);
 	if (!rc && needs_partners(opt))
 		rc = seph_put_partners(c);
 	if (!rc) {
