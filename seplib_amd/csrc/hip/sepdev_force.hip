@@ -467,6 +467,16 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 		d->fused_epot_assign = a->epot_assign;
 		return 0;
 	}
+	if (d->pc_armed && a->mixed && !d->pc_pending && !d->fused_pending) {
+		// recorded: a Coulomb call right after launches both terms
+		d->pc_pending = 1;
+		d->pc_lj = c;
+		d->pc_typed = a->typed;
+		d->pc_fast = fast;
+		d->pc_ret = ret_slot;
+		d->pc_epot_assign = a->epot_assign;
+		return 0;
+	}
 	return launch_lj(d, c, a->typed, a->mixed, fast, ret_slot, a->epot_assign);
 }
 
@@ -474,6 +484,11 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 // step's leapfrog needs its forces or its rows).
 int sepk_flush_pair(sepdev *d)
 {
+	if (d->pc_pending) {
+		d->pc_pending = 0;
+		return launch_lj(d, d->pc_lj, d->pc_typed, 1, d->pc_fast != 0, d->pc_ret,
+				 d->pc_epot_assign);
+	}
 	if (!d->fused_pending)
 		return 0;
 	d->fused_pending = 0;
@@ -484,6 +499,12 @@ int sepk_flush_pair(sepdev *d)
 extern "C" int sepdev_fuse_step(sepdev *d, int on)
 {
 	d->fuse_armed = on != 0;
+	return on ? 0 : sepk_flush_pair(d);
+}
+
+extern "C" int sepdev_fuse_pair_coulomb(sepdev *d, int on)
+{
+	d->pc_armed = on != 0;
 	return on ? 0 : sepk_flush_pair(d);
 }
 
@@ -630,6 +651,204 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 				      s.f, partial, lb, rows);
 }
 
+// A pair call (LJ, typed or not) and the shifted-force Coulomb call right
+// after it, mixed form, in one walk of the list: one gather of each
+// neighbour's record serves both terms.  Each term keeps its own force sum,
+// energy, crossing stress and partial rows, accumulated entry by entry in
+// the order of its own kernel (k_lj, k_coulomb), and the atom's force is
+// written as the two calls would leave it: the LJ sum (over or added to f),
+// then the Coulomb sum added.  The results are those of the two launches,
+// bit for bit.
+template <bool TYPED, bool OVERWRITE>
+__global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
+	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt, long n,
+	long npad, sepk_lj c, sepk_coul cc, double *part_lj, double *part_c, long rows)
+{
+	const long lb = sepk_xcd_block();
+	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
+	const int sub = threadIdx.x % SEPK_SUB;
+	double fl[3] = {0.0, 0.0, 0.0}, fc[3] = {0.0, 0.0, 0.0};
+	double al[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+	double ac[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+	bool act_l = t < n, act_c = t < n;
+	signed char ti = 0;
+	if (act_l && TYPED) {
+		ti = s.type[t];
+		act_l = ti == c.t0 || ti == c.t1;
+	}
+	if (act_c)
+		act_c = !(fabs(s.pos[t].q) < DBL_EPSILON);
+	int4 qi = make_int4(0, 0, 0, SEPK_QPHANTOM);
+	if (act_l || act_c) {
+		qi = s.qpos[t];
+		const float qif = __int_as_float(qi.w);
+		const float epsf = (float)DBL_EPSILON;
+		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
+		const size_t stride = (size_t)npad * SEPK_SUB;
+		const int4 *lp = sepk_lane_list(nbr, t, sub);
+		for (int b = 0; b < nbat; b++, lp += stride) {
+			const int4 jb = *lp;
+			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
+			int4 qj[SEPK_BATCH];
+			signed char tj[SEPK_BATCH];
+#pragma unroll
+			for (int u = 0; u < SEPK_BATCH; u++) {
+				qj[u] = s.qpos[(unsigned)j[u]];
+				if (TYPED)
+					tj[u] = s.type[(unsigned)j[u]];
+			}
+#pragma unroll
+			for (int u = 0; u < SEPK_BATCH; u++) {
+				const unsigned ux = (unsigned)qi.x, vx = (unsigned)qj[u].x;
+				const unsigned uy = (unsigned)qi.y, vy = (unsigned)qj[u].y;
+				const unsigned uz = (unsigned)qi.z, vz = (unsigned)qj[u].z;
+				const int ix = (int)(ux - vx), iy = (int)(uy - vy), iz = (int)(uz - vz);
+				const double dx = (double)ix * c.qd0;
+				const double dy = (double)iy * c.qd1;
+				const double dz = (double)iz * c.qd2;
+				const double r2 = dx * dx + dy * dy + dz * dz;
+				const bool bx = (ux < vx) != (ix < 0), by = (uy < vy) != (iy < 0),
+					   bz = (uz < vz) != (iz < 0);
+				const double kx = bx ? (ix < 0 ? 1.0 : -1.0) : 0.0;
+				const double ky = by ? (iy < 0 ? 1.0 : -1.0) : 0.0;
+				const double kz = bz ? (iz < 0 ? 1.0 : -1.0) : 0.0;
+				// the LJ term as in sepk_lj_pairs
+				const bool tl = !TYPED || (ti == c.t0 && tj[u] == c.t1) ||
+						(ti == c.t1 && tj[u] == c.t0);
+				if (act_l && tl && qj[u].w != SEPK_QPHANTOM && r2 < c.cf2) {
+					const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
+					const float q3 = q * q * q;
+					const double ft = (double)(fmaf(c.Af, q3, c.nBf) * q3 * q);
+					const double e = (double)fmaf(fmaf(c.Cf, q3, c.nDf), q3, c.ushiftf);
+					fl[0] = fma(ft, dx, fl[0]);
+					fl[1] = fma(ft, dy, fl[1]);
+					fl[2] = fma(ft, dz, fl[2]);
+					al[0] += e;
+					if (bx | by | bz)
+						sepk_pair_cross(ft, dx, dy, dz, kx, ky, kz, c.L0, c.L1, c.L2, al);
+				}
+				// the Coulomb term as in sepk_coul_pairs_mixed
+				const float qjf = __int_as_float(qj[u].w);
+				if (act_c && fabsf(qjf) >= epsf && r2 < cc.cf2) {
+					const float r2f = (float)r2;
+					const float ri = __builtin_amdgcn_rsqf(r2f);
+					const float rf = r2f * ri;
+					const float qq = qjf * qif;
+					const float dc = cc.cff - rf;
+					const float qs = qq * cc.icf2f * ri;
+					const double ft = (double)(qs * (dc * (cc.cff + rf)) * (ri * ri));
+					const double e = (double)(qs * (dc * dc));
+					fc[0] = fma(ft, dx, fc[0]);
+					fc[1] = fma(ft, dy, fc[1]);
+					fc[2] = fma(ft, dz, fc[2]);
+					ac[0] += e;
+					if (bx | by | bz)
+						sepk_pair_cross(ft, dx, dy, dz, kx, ky, kz, cc.L0, cc.L1, cc.L2, ac);
+				}
+			}
+		}
+	}
+	// the two epilogues of sepk_pair_epilogue, the second adding to the first
+#pragma unroll
+	for (int o = 1; o < SEPK_SUB; o <<= 1)
+#pragma unroll
+		for (int k = 0; k < 3; k++) {
+			fl[k] += __shfl_xor(fl[k], o, SEPK_WAVE);
+			fc[k] += __shfl_xor(fc[k], o, SEPK_WAVE);
+		}
+	if (t < n && sub == 0) {
+		for (int k = 0; k < 3; k++) {
+			double v = OVERWRITE ? fl[k] : s.f[k * npad + t];
+			if (!OVERWRITE && act_l)
+				v += fl[k];
+			if (act_c)
+				v += fc[k];
+			s.f[k * npad + t] = v;
+		}
+	}
+	const double px = (double)(unsigned)qi.x * c.qd0;
+	const double py = (double)(unsigned)qi.y * c.qd1;
+	const double pz = (double)(unsigned)qi.z * c.qd2;
+	if (act_l && sub == 0) {
+		const double gx = 2.0 * fl[0], gy = 2.0 * fl[1], gz = 2.0 * fl[2];
+		al[1] = fma(gx, px, al[1]);
+		al[2] = fma(gx, py, al[2]);
+		al[3] = fma(gx, pz, al[3]);
+		al[4] = fma(gy, py, al[4]);
+		al[5] = fma(gy, pz, al[5]);
+		al[6] = fma(gz, pz, al[6]);
+	}
+	if (act_c && sub == 0) {
+		const double gx = 2.0 * fc[0], gy = 2.0 * fc[1], gz = 2.0 * fc[2];
+		ac[1] = fma(gx, px, ac[1]);
+		ac[2] = fma(gx, py, ac[2]);
+		ac[3] = fma(gx, pz, ac[3]);
+		ac[4] = fma(gy, py, ac[4]);
+		ac[5] = fma(gy, pz, ac[5]);
+		ac[6] = fma(gz, pz, ac[6]);
+	}
+	sepk_block_partial<7>(al, 0u, part_lj, lb, rows);
+	__syncthreads(); // the partial helper's LDS is reused
+	sepk_block_partial<7>(ac, 0u, part_c, lb, rows);
+}
+
+// The recorded pair call and this Coulomb call as one k_lj_coul launch,
+// each with its own term region and finishing task, queued in call order.
+static int launch_lj_coul(sepdev *d, const sepk_coul &cc, int ret_slot)
+{
+	d->pc_pending = 0;
+	int rc;
+	// two distinct free term regions (the queue is finished first if not)
+	int r1 = -1, r2 = -1;
+	for (int r = SEPK_R_PAIR; r < SEPK_R_PAIR + SEPK_NTERM && r2 < 0; r++) {
+		bool held = false;
+		for (int i = 0; i < d->npending; i++)
+			held = held || d->pending[i].region == r;
+		if (!held) {
+			if (r1 < 0)
+				r1 = r;
+			else
+				r2 = r;
+		}
+	}
+	if (r2 < 0) {
+		d->fork_ready = 0;
+		if ((rc = sepk_flush_tasks(d)))
+			return rc;
+		r1 = SEPK_R_PAIR;
+		r2 = SEPK_R_PAIR + 1;
+	}
+	if ((rc = sepk_fork_point(d)))
+		return rc;
+	const bool over = d->f_zero_pending != 0;
+	auto k = d->pc_typed ? (over ? k_lj_coul<true, true> : k_lj_coul<true, false>)
+			     : (over ? k_lj_coul<false, true> : k_lj_coul<false, false>);
+	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
+	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, d->pc_lj, cc,
+					  sepk_region(d, r1), sepk_region(d, r2), d->region_rows);
+	SEPK_LAUNCH_CHECK();
+	d->f_zero_pending = 0;
+	d->last_fused = 0;
+	sepk_task f;
+	memset(&f, 0, sizeof f);
+	f.mode = SEPK_FIN_PAIR;
+	f.region = r1;
+	f.nblk = (int)nb;
+	f.nv = 7;
+	f.epot_assign = d->pc_epot_assign;
+	sepk_task_ret(d, f, d->pc_ret);
+	if ((rc = sepk_queue_task(d, f)))
+		return rc;
+	memset(&f, 0, sizeof f);
+	f.mode = SEPK_FIN_PAIR;
+	f.region = r2;
+	f.nblk = (int)nb;
+	f.nv = 7;
+	f.also_ecoul = 1;
+	sepk_task_ret(d, f, ret_slot);
+	return sepk_queue_task(d, f);
+}
+
 extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot)
 {
 	if (!d->list_built)
@@ -643,6 +862,8 @@ extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot
 	int rc = sepk_flush_nh(d);
 	if (rc || (a->mixed && (rc = sepk_fresh_qpos(d, a->L))))
 		return rc;
+	if (d->pc_pending && a->mixed)
+		return launch_lj_coul(d, c, ret_slot);
 	int region;
 	if ((rc = sepk_term_region(d, &region)) || (rc = sepk_fork_point(d)))
 		return rc;

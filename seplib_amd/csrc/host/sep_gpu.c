@@ -97,6 +97,17 @@ static int overlappable(const sep_gpu_op *ops, int nops)
 	return 0;
 }
 
+/* A pair call followed directly by a Coulomb call: one list walk serves
+ * both in the mixed form. */
+static int pair_then_coulomb(const sep_gpu_op *ops, int nops)
+{
+	for (int i = 0; i + 1 < nops; i++)
+		if ((ops[i].op == SEP_GPU_OP_PAIRS || ops[i].op == SEP_GPU_OP_LJ) &&
+		    ops[i + 1].op == SEP_GPU_OP_COULOMB)
+			return 1;
+	return 0;
+}
+
 static int run_ops(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
 		   int nops);
 
@@ -110,9 +121,17 @@ static int run_one_step(seppart *atoms, sepsys *sys, sepret *ret,
 	const int conc = c->resident && (flags & SEP_GPU_RUN_CONCURRENT) && overlappable(ops, nops);
 	if (fuse)
 		sepdev_fuse_step(c->dev, 1);
+	const int pcf = c->resident && c->precision == SEP_GPU_MIXED && pair_then_coulomb(ops, nops);
 	int rc = conc ? seph_dev_fail(sepdev_concurrent_terms(c->dev, 1)) : 0;
+	if (!rc && pcf)
+		rc = seph_dev_fail(sepdev_fuse_pair_coulomb(c->dev, 1));
 	if (!rc)
 		rc = run_ops(atoms, sys, ret, ops, nops);
+	if (pcf) {
+		const int rc2 = seph_dev_fail(sepdev_fuse_pair_coulomb(c->dev, 0));
+		if (!rc)
+			rc = rc2;
+	}
 	if (fuse) {
 		const int rc2 = seph_dev_fail(sepdev_fuse_step(c->dev, 0));
 		if (!rc)

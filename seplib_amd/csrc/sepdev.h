@@ -185,6 +185,9 @@ int sepdev_fuse_step(sepdev *d, int on);
 /* Arms (on = 1) or disarms concurrent force terms for the calls of one step:
  * bonded per-term kernels overlap the pair kernels on a second stream. */
 int sepdev_concurrent_terms(sepdev *d, int on);
+/* Arms (on = 1) or disarms, for the calls of one step, the single list
+ * walk of a mixed pair call followed by a Coulomb call. */
+int sepdev_fuse_pair_coulomb(sepdev *d, int on);
 
 /* ---- misc */
 int sepdev_synchronize(sepdev *d);
