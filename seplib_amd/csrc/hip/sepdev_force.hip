@@ -467,10 +467,11 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 		d->fused_epot_assign = a->epot_assign;
 		return 0;
 	}
-	if (d->pc_armed && a->mixed && !d->pc_pending && !d->fused_pending) {
+	if (d->pc_armed && !d->pc_pending && !d->fused_pending) {
 		// recorded: a Coulomb call right after launches both terms
 		d->pc_pending = 1;
 		d->pc_lj = c;
+		d->pc_mixed = a->mixed;
 		d->pc_typed = a->typed;
 		d->pc_fast = fast;
 		d->pc_ret = ret_slot;
@@ -486,7 +487,7 @@ int sepk_flush_pair(sepdev *d)
 {
 	if (d->pc_pending) {
 		d->pc_pending = 0;
-		return launch_lj(d, d->pc_lj, d->pc_typed, 1, d->pc_fast != 0, d->pc_ret,
+		return launch_lj(d, d->pc_lj, d->pc_typed, d->pc_mixed, d->pc_fast != 0, d->pc_ret,
 				 d->pc_epot_assign);
 	}
 	if (!d->fused_pending)
@@ -652,14 +653,14 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 }
 
 // A pair call (LJ, typed or not) and the shifted-force Coulomb call right
-// after it, mixed form, in one walk of the list: one gather of each
+// after it, in either form, in one walk of the list: one gather of each
 // neighbour's record serves both terms.  Each term keeps its own force sum,
 // energy, crossing stress and partial rows, accumulated entry by entry in
 // the order of its own kernel (k_lj, k_coulomb), and the atom's force is
 // written as the two calls would leave it: the LJ sum (over or added to f),
 // then the Coulomb sum added.  The results are those of the two launches,
 // bit for bit.
-template <bool TYPED, bool OVERWRITE>
+template <bool TYPED, bool OVERWRITE, bool MIXED, bool FAST>
 __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt, long n,
 	long npad, sepk_lj c, sepk_coul cc, double *part_lj, double *part_c, long rows)
@@ -676,10 +677,58 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 		ti = s.type[t];
 		act_l = ti == c.t0 || ti == c.t1;
 	}
+	sepk_pos pi = {0.0, 0.0, 0.0, 0.0};
+	if (act_l || act_c)
+		pi = s.pos[t];
 	if (act_c)
-		act_c = !(fabs(s.pos[t].q) < DBL_EPSILON);
+		act_c = !(fabs(pi.q) < DBL_EPSILON);
 	int4 qi = make_int4(0, 0, 0, SEPK_QPHANTOM);
-	if (act_l || act_c) {
+	if (!MIXED && (act_l || act_c)) {
+		// the FP64 loops of sepk_lj_pairs and k_coulomb on 32-byte records
+		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
+		const size_t stride = (size_t)npad * SEPK_SUB;
+		const int4 *lp = sepk_lane_list(nbr, t, sub);
+		const char *pb = (const char *)s.pos;
+		for (int b = 0; b < nbat; b++, lp += stride) {
+			const int4 jb = *lp;
+			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
+			sepk_pos pj[SEPK_BATCH];
+			signed char tj[SEPK_BATCH];
+#pragma unroll
+			for (int u = 0; u < SEPK_BATCH; u++) {
+				pj[u] = sepk_rec(pb, j[u]);
+				if (TYPED)
+					tj[u] = s.type[(unsigned)j[u]];
+			}
+#pragma unroll
+			for (int u = 0; u < SEPK_BATCH; u++) {
+				double kx, ky, kz;
+				const double dx = sepk_image<FAST>(pi.x - pj[u].x, c.L0, c.iL0, kx);
+				const double dy = sepk_image<FAST>(pi.y - pj[u].y, c.L1, c.iL1, ky);
+				const double dz = sepk_image<FAST>(pi.z - pj[u].z, c.L2, c.iL2, kz);
+				const double r2 = dx * dx + dy * dy + dz * dz;
+				const bool tl = !TYPED || (ti == c.t0 && tj[u] == c.t1) ||
+						(ti == c.t1 && tj[u] == c.t0);
+				if (act_l && tl && r2 < c.cf2) {
+					const double q = c.sig2 * sepk_recip(r2);
+					const double q3 = q * q * q;
+					const double ft = fma(c.A, q3, c.nB) * q3 * q;
+					const double e = fma(fma(c.C, q3, c.nD), q3, c.ushift);
+					sepk_pair_accumulate(ft, e, dx, dy, dz, kx, ky, kz, c.L0, c.L1, c.L2,
+							     fl[0], fl[1], fl[2], al);
+				}
+				if (act_c && !(fabs(pj[u].q) < DBL_EPSILON) && r2 < cc.cf2) {
+					const double qq = pj[u].q * pi.q;
+					const double r = sqrt(r2);
+					const double ft = qq * (1.0 / r2 - cc.icf2) / r;
+					const double e = qq * (1.0 / r + (r - cc.cf) * cc.icf2 - cc.icf);
+					sepk_pair_accumulate(ft, e, dx, dy, dz, kx, ky, kz, cc.L0, cc.L1,
+							     cc.L2, fc[0], fc[1], fc[2], ac);
+				}
+			}
+		}
+	}
+	if (MIXED && (act_l || act_c)) {
 		qi = s.qpos[t];
 		const float qif = __int_as_float(qi.w);
 		const float epsf = (float)DBL_EPSILON;
@@ -766,9 +815,10 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 			s.f[k * npad + t] = v;
 		}
 	}
-	const double px = (double)(unsigned)qi.x * c.qd0;
-	const double py = (double)(unsigned)qi.y * c.qd1;
-	const double pz = (double)(unsigned)qi.z * c.qd2;
+	// the stress identity's positions: quantized (mixed) or FP64
+	const double px = MIXED ? (double)(unsigned)qi.x * c.qd0 : pi.x;
+	const double py = MIXED ? (double)(unsigned)qi.y * c.qd1 : pi.y;
+	const double pz = MIXED ? (double)(unsigned)qi.z * c.qd2 : pi.z;
 	if (act_l && sub == 0) {
 		const double gx = 2.0 * fl[0], gy = 2.0 * fl[1], gz = 2.0 * fl[2];
 		al[1] = fma(gx, px, al[1]);
@@ -821,8 +871,18 @@ static int launch_lj_coul(sepdev *d, const sepk_coul &cc, int ret_slot)
 	if ((rc = sepk_fork_point(d)))
 		return rc;
 	const bool over = d->f_zero_pending != 0;
-	auto k = d->pc_typed ? (over ? k_lj_coul<true, true> : k_lj_coul<true, false>)
-			     : (over ? k_lj_coul<false, true> : k_lj_coul<false, false>);
+	typedef void (*kfn)(sepk_state, const int *, const int *, long, long, sepk_lj, sepk_coul,
+			    double *, double *, long);
+	static const kfn table[2][2][2][2] = {
+		{{{k_lj_coul<false, false, false, false>, k_lj_coul<false, false, false, true>},
+		  {k_lj_coul<false, false, true, false>, k_lj_coul<false, false, true, true>}},
+		 {{k_lj_coul<false, true, false, false>, k_lj_coul<false, true, false, true>},
+		  {k_lj_coul<false, true, true, false>, k_lj_coul<false, true, true, true>}}},
+		{{{k_lj_coul<true, false, false, false>, k_lj_coul<true, false, false, true>},
+		  {k_lj_coul<true, false, true, false>, k_lj_coul<true, false, true, true>}},
+		 {{k_lj_coul<true, true, false, false>, k_lj_coul<true, true, false, true>},
+		  {k_lj_coul<true, true, true, false>, k_lj_coul<true, true, true, true>}}}};
+	const kfn k = table[d->pc_typed ? 1 : 0][over ? 1 : 0][d->pc_mixed ? 1 : 0][d->pc_fast ? 1 : 0];
 	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
 	k<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad, d->pc_lj, cc,
 					  sepk_region(d, r1), sepk_region(d, r2), d->region_rows);
@@ -862,7 +922,10 @@ extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot
 	int rc = sepk_flush_nh(d);
 	if (rc || (a->mixed && (rc = sepk_fresh_qpos(d, a->L))))
 		return rc;
-	if (d->pc_pending && a->mixed)
+	// the recorded pair call joins this one when both read the same
+	// records with the same image rule
+	if (d->pc_pending && (a->mixed != 0) == (d->pc_mixed != 0) &&
+	    (a->mixed || (fast_image_ok(d, a->L) ? 1 : 0) == (d->pc_fast ? 1 : 0)))
 		return launch_lj_coul(d, c, ret_slot);
 	int region;
 	if ((rc = sepk_term_region(d, &region)) || (rc = sepk_fork_point(d)))

@@ -216,12 +216,12 @@ struct sepdev {
 	// fork_ready is cleared by anything that moves atoms or by a finisher
 	// launch that reads a term region the side stream may write next.
 	// Pair call and Coulomb call in one list walk (k_lj_coul), armed by
-	// the host around a step that makes the two calls in a row, mixed
-	// form: the pair call is recorded, and the Coulomb call launches both
+	// the host around a step that makes the two calls in a row: the pair
+	// call is recorded, and the Coulomb call launches both
 	// (anything else first launches the pair call on its own).
 	int pc_armed, pc_pending;
 	sepk_lj pc_lj;
-	int pc_typed, pc_fast, pc_ret, pc_epot_assign;
+	int pc_mixed, pc_typed, pc_fast, pc_ret, pc_epot_assign;
 
 	hipStream_t side;
 	hipEvent_t fork_ev, join_ev;

@@ -98,7 +98,7 @@ static int overlappable(const sep_gpu_op *ops, int nops)
 }
 
 /* A pair call followed directly by a Coulomb call: one list walk serves
- * both in the mixed form. */
+ * both. */
 static int pair_then_coulomb(const sep_gpu_op *ops, int nops)
 {
 	for (int i = 0; i + 1 < nops; i++)
@@ -121,7 +121,7 @@ static int run_one_step(seppart *atoms, sepsys *sys, sepret *ret,
 	const int conc = c->resident && (flags & SEP_GPU_RUN_CONCURRENT) && overlappable(ops, nops);
 	if (fuse)
 		sepdev_fuse_step(c->dev, 1);
-	const int pcf = c->resident && c->precision == SEP_GPU_MIXED && pair_then_coulomb(ops, nops);
+	const int pcf = c->resident && pair_then_coulomb(ops, nops);
 	int rc = conc ? seph_dev_fail(sepdev_concurrent_terms(c->dev, 1)) : 0;
 	if (!rc && pcf)
 		rc = seph_dev_fail(sepdev_fuse_pair_coulomb(c->dev, 1));

@@ -186,7 +186,7 @@ int sepdev_fuse_step(sepdev *d, int on);
  * bonded per-term kernels overlap the pair kernels on a second stream. */
 int sepdev_concurrent_terms(sepdev *d, int on);
 /* Arms (on = 1) or disarms, for the calls of one step, the single list
- * walk of a mixed pair call followed by a Coulomb call. */
+ * walk of a pair call followed by a Coulomb call. */
 int sepdev_fuse_pair_coulomb(sepdev *d, int on);
 
 /* ---- misc */

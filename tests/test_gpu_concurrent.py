@@ -15,9 +15,9 @@ the oracle by test_gpu_water.py and test_gpu_golden.py.
     second angle call reuses the angle family's per-term slots and so must
     wait for the first one's gather;
   * a step with no pair call before its bonded calls runs on one stream;
-  * the mixed water step's pair and Coulomb calls in one list walk (a
+  * the water step's pair and Coulomb calls in one list walk (a
     sep_gpu_run step) against the same calls made one at a time on the
-    resident system: the same bits after 20 steps;
+    resident system: the same bits after 20 steps, FP64 and mixed;
   * the Model II entry points themselves (sep_omp_make_neighb and the
     sep_omp_* terms into caller-owned arrays) against the oracle to 1e-12
     on water and butane, leaving the atoms' forces alone, and refused on a
@@ -155,13 +155,14 @@ def test_model2_refused_on_resident_system(product):
         s.close()
 
 
-def test_water_pair_coulomb_walk_bit_identical(product, water):
+@pytest.mark.parametrize("precision", [S.SEP_GPU_FP64, S.SEP_GPU_MIXED], ids=["fp64", "mixed"])
+def test_water_pair_coulomb_walk_bit_identical(product, water, precision):
     kw = dict(cf=WAT.RC_COUL, dt=WAT.DT, nneighb=WAT.NNEIGHB)
     out = []
     for fused in (True, False):
         s = SepSystem.from_workload(product, water, **kw)
         try:
-            s.gpu_precision(S.SEP_GPU_MIXED)
+            s.gpu_precision(precision)
             s.gpu_resident(True)
             if fused:
                 s.gpu_run(WAT.step_ops(s), 20, graph=True)
