@@ -744,7 +744,12 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 	if (g.opt == 2 && !d->s[0].partners)
 		return sepk_fail(SEP_GPU_ESTATE, "SEP_EXCL_BONDED needs bonded partners");
 	d->fork_ready = 0; // a rebuild permutes the atoms
-	int rc = sepk_flush_nh(d);
+	// a recorded pair call reads the current list: launch it before the
+	// list can be resized or rebuilt
+	int rc = sepk_flush_pair(d);
+	if (rc)
+		return rc;
+	rc = sepk_flush_nh(d);
 	if (rc)
 		return rc;
 
