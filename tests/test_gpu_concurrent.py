@@ -17,7 +17,8 @@ the oracle by test_gpu_water.py and test_gpu_golden.py.
   * a step with no pair call before its bonded calls runs on one stream;
   * the water step's pair and Coulomb calls in one list walk (a
     sep_gpu_run step) against the same calls made one at a time on the
-    resident system: the same bits after 20 steps, FP64 and mixed;
+    resident system: the same bits after 20 steps, FP64 and mixed; and
+    with SEP_BRUTE, where the Coulomb call builds its own list;
   * the Model II entry points themselves (sep_omp_make_neighb and the
     sep_omp_* terms into caller-owned arrays) against the oracle to 1e-12
     on water and butane, leaving the atoms' forces alone, and refused on a
@@ -180,4 +181,33 @@ def test_water_pair_coulomb_walk_bit_identical(product, water, precision):
         finally:
             s.close()
     assert out[0]["nupdate"] > 0
+    _assert_same(out[0], out[1])
+
+
+def test_water_brute_pair_then_coulomb_order(product, water):
+    """SEP_BRUTE: each pair call builds its own list, so a recorded pair call
+    has to run before the Coulomb call's build replaces the list.  A
+    sep_gpu_run step equals the same calls made one at a time."""
+    kw = dict(cf=WAT.RC_COUL, dt=WAT.DT, nneighb=WAT.NNEIGHB, update=S.SEP_BRUTE)
+    out = []
+    for run in (True, False):
+        s = SepSystem.from_workload(product, water, **kw)
+        try:
+            s.gpu_resident(True)
+            if run:
+                s.gpu_run(WAT.step_ops(s), 5, graph=False)
+            else:
+                for _ in range(5):
+                    s.reset_retval()
+                    s.reset_force()
+                    s.force_pairs("OO", WAT.RC_LJ, "sep_lj_shift", S.SEP_EXCL_SAME_MOL)
+                    s.coulomb_sf(WAT.RC_COUL, S.SEP_EXCL_SAME_MOL)
+                    s.stretch_harmonic(0, WAT.LBOND, WAT.KBOND)
+                    s.angle_harmonic(0, WAT.THETA0, WAT.KANGLE)
+                    s.nosehoover(WAT.TEMP, WAT.TAU)
+                    s.leapfrog()
+            out.append(_final(s))
+        finally:
+            s.close()
+    assert np.abs(out[0]["f"]).max() > 0
     _assert_same(out[0], out[1])
