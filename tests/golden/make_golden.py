@@ -2,7 +2,8 @@
 from its own sources (oracle/_ref/libsepref.so, strict FP flags; build it
 with ``make -C oracle``).  Run from the repository root:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py g6 abi     # only those named
 
 Fixtures (numpy .npz, no pickles; inputs and outputs together, so the tests
 never read the reference tree):
@@ -26,6 +27,15 @@ never read the reference tree):
                    (test/prg1.xyz/.top, L=13.99) with prgs/prg2.c's terms.
   g5_ka.npz        Kob-Andersen 80:20 N=500 at rho 1.2: sep_force_lj for
                    AA, AB and BB.
+  g6_excl.npz      exclusion variants and list modes: 250 butane molecules
+                   of test/prg1.* and the water input of test/prg2.*, each
+                   case one force call after a reset (runs.G6_CASES):
+                   sep_force_pairs with SEP_EXCL_BONDED / SEP_EXCL_SAME_MOL /
+                   SEP_ALL through the linked-cell list, the O(N^2) list
+                   (SEP_NEIGHBLIST) and SEP_BRUTE, and sep_coulomb_sf brute
+                   force with and without SEP_EXCL_BONDED; the force, the
+                   sepret fields and, for list modes, the size and hash of
+                   the host half-list multiset.
   abi_layout.json  sizes and field offsets of the structs of sepstrct.h as
                    compiled from the reference's own header (abi_probe.c).
 """
@@ -94,6 +104,26 @@ def g5(ref):
                         type=w["type"], L=w["L"], forces=f, rets=r)
 
 
+def g6(ref, orc):
+    wb = R.read_xyz(os.path.join(REF, "test", "prg1.xyz"))
+    wb = R.g6_butane(wb, R.read_top(os.path.join(REF, "test", "prg1.top")))
+    ww = R.read_xyz(os.path.join(REF, "test", "prg2.xyz"))
+    ww.update(R.read_top(os.path.join(REF, "test", "prg2.top")))
+    out = {}
+    for tag, system, update, call in R.G6_CASES:
+        w = wb if system == "butane" else ww
+        f, r, npairs, h, _ = R.run_g6_case(ref, orc.lib, w, system, update, call)
+        out["f_" + tag] = f
+        out["r_" + tag] = r
+        out["pairs_" + tag] = np.array([npairs, h], dtype=np.int64)
+    keys = ("x", "v", "m", "q", "type", "L", "bonds", "angles", "dihedrals")
+    for name, w in (("butane", wb), ("water", ww)):
+        for k in keys:
+            if k in w:
+                out[name + "_" + k] = w[k]
+    np.savez_compressed(os.path.join(HERE, "g6_excl.npz"), **out)
+
+
 def abi():
     src = os.path.join(HERE, "abi_probe.c")
     with tempfile.TemporaryDirectory() as d:
@@ -111,12 +141,12 @@ def main():
         sys.exit("oracle/_ref/libsepref.so is missing: run make -C oracle")
     ref = ReferenceAPI()
     orc = OracleAPI()
-    abi()
-    g1(ref, orc)
-    g4(ref)
-    g2(ref)
-    g3(ref)
-    g5(ref)
+    writers = {"abi": abi, "g1": lambda: g1(ref, orc), "g4": lambda: g4(ref),
+               "g2": lambda: g2(ref), "g3": lambda: g3(ref), "g5": lambda: g5(ref),
+               "g6": lambda: g6(ref, orc)}
+    names = sys.argv[1:] or list(writers)
+    for name in names:
+        writers[name]()
     for name in sorted(os.listdir(HERE)):
         if name.endswith((".npz", ".json")):
             print("%-20s %8d bytes" % (name, os.path.getsize(os.path.join(HERE, name))))

@@ -265,3 +265,76 @@ def run_model2(api, w, cf, opt, terms, nneighb=S.SEP_NEIGHB):
         return np.array(out), f0, np.array(s.view["f"])
     finally:
         s.close()
+
+
+# ---- exclusion variants and list modes --------------------------------------
+# source/sepprfrc.c:517-603 with :703-740 (linked cells, SEP_EXCL_BONDED:
+# bond, angle or dihedral partners, each scan stopping at the first -1 in
+# either atom), :306-343 (the O(N^2) list of SEP_NEIGHBLIST, whose bonded
+# test is sep_bond_share alone), :19-92 (SEP_BRUTE, bonds alone) and
+# source/sepcoulomb.c:21-93 (brute-force Coulomb, which drops a pair only
+# when sep_bonded is exactly 1).
+
+G6_BUTANE_MOL = 250  # 1000 atoms of the reference's butane input
+
+
+def g6_butane(w_full, top):
+    return subset(w_full, top, G6_BUTANE_MOL, 4)
+
+
+def _pairs_call(types, opt):
+    return lambda s: s.force_pairs(types, 2.5, "sep_lj_shift", opt)
+
+
+def _coulomb_call(opt):
+    return lambda s: s.coulomb_sf(WATER["cf"], opt)
+
+
+# tag, system ("butane" or "water"), neighbour update mode, call
+G6_CASES = (
+    ("butane_llist_bonded", "butane", S.SEP_LLIST_NEIGHBLIST, _pairs_call("CC", S.SEP_EXCL_BONDED)),
+    ("butane_llist_samemol", "butane", S.SEP_LLIST_NEIGHBLIST, _pairs_call("CC", S.SEP_EXCL_SAME_MOL)),
+    ("butane_nlist_all", "butane", S.SEP_NEIGHBLIST, _pairs_call("CC", S.SEP_ALL)),
+    ("butane_nlist_bonded", "butane", S.SEP_NEIGHBLIST, _pairs_call("CC", S.SEP_EXCL_BONDED)),
+    ("butane_nlist_samemol", "butane", S.SEP_NEIGHBLIST, _pairs_call("CC", S.SEP_EXCL_SAME_MOL)),
+    ("butane_brute_bonded", "butane", S.SEP_BRUTE, _pairs_call("CC", S.SEP_EXCL_BONDED)),
+    ("water_llist_bonded", "water", S.SEP_LLIST_NEIGHBLIST, _pairs_call("OH", S.SEP_EXCL_BONDED)),
+    ("water_nlist_bonded", "water", S.SEP_NEIGHBLIST, _pairs_call("OH", S.SEP_EXCL_BONDED)),
+    ("water_nlist_all", "water", S.SEP_NEIGHBLIST, _pairs_call("OO", S.SEP_ALL)),
+    ("water_brute_bonded", "water", S.SEP_BRUTE, _pairs_call("OH", S.SEP_EXCL_BONDED)),
+    ("water_brute_coulomb_bonded", "water", S.SEP_BRUTE, _coulomb_call(S.SEP_EXCL_BONDED)),
+    ("water_brute_coulomb_all", "water", S.SEP_BRUTE, _coulomb_call(S.SEP_ALL)),
+)
+
+
+def g6_workload(g, system):
+    """The G6 input of one system (butane or water) with its topology, from
+    the loaded fixture."""
+    w = {k[len(system) + 1:]: np.array(g[k]) for k in g.files if k.startswith(system + "_")}
+    w["mol"] = np.full(len(w["m"]), -1)
+    return w
+
+
+def g6_cf(system):
+    return 2.5 if system == "butane" else WATER["cf"]
+
+
+def run_g6_case(api, orc_lib, w, system, update, call, prepare=None):
+    """One force call after a reset: the force, the sepret vector and, for
+    the list modes, the size and hash of the host half-list multiset
+    (read by the oracle's orc_half_pairs)."""
+    s = SepSystem.from_workload(api, w, cf=g6_cf(system), dt=0.001, update=update)
+    try:
+        if prepare:
+            prepare(s)
+        s.reset_retval()
+        s.reset_force()
+        call(s)
+        f = np.array(s.view["f"])
+        r = ret_vec(s)
+        if update == S.SEP_BRUTE:
+            return f, r, -1, -1, None
+        keys = pair_keys(orc_lib, s, cap_per_atom=2048)
+        return f, r, len(keys), pair_hash(keys), keys
+    finally:
+        s.close()

@@ -71,3 +71,19 @@ def test_g5_ka_force_lj(oracle):
         assert np.abs(g["forces"][k]).max() > 0.0
         assert np.array_equal(f[k], g["forces"][k]), k
         assert np.array_equal(r[k], g["rets"][k]), k
+
+
+@pytest.mark.parametrize("case", R.G6_CASES, ids=[c[0] for c in R.G6_CASES])
+def test_g6_exclusions_and_list_modes(oracle, case):
+    """SEP_EXCL_BONDED through the linked-cell list (bond, angle and
+    dihedral partners), the O(N^2) list and SEP_BRUTE (bonds only), and the
+    brute-force Coulomb rule (excluded only when exactly one partnership
+    holds): pair multiset, forces and sepret bit for bit."""
+    tag, system, update, call = case
+    g = _load("g6_excl.npz")
+    f, r, npairs, h, _ = R.run_g6_case(oracle, oracle.lib, R.g6_workload(g, system), system,
+                                       update, call)
+    assert np.abs(g["f_" + tag]).max() > 0.0
+    assert [npairs, h] == list(g["pairs_" + tag])
+    assert np.array_equal(f, g["f_" + tag])
+    assert np.array_equal(r, g["r_" + tag])

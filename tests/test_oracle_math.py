@@ -443,3 +443,131 @@ def test_model2_terms_are_the_sep_terms(oracle):
                              prepare=calls[0])[0][0]
             assert np.abs(ref).max() > 0, k
             assert force_error(m2[k], ref) <= 1e-12, (k, force_error(m2[k], ref))
+
+
+# ---- list modes and exclusion rules ------------------------------------------
+# source/sepprfrc.c:306-343 (the O(N^2) list), :517-603 with :703-740 (linked
+# cells excluding bond, angle and dihedral partners), :19-92 (brute force,
+# bond partners), source/sepcoulomb.c:21-93 (brute-force Coulomb, excluding a
+# pair when exactly one partnership holds).  The expected sets come from the
+# topology rows and numpy distances, not from the oracle's predicates.
+
+def _g6():
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "g6_excl.npz"), allow_pickle=False)
+    from tests.golden import runs as R
+    return g, R
+
+
+@pytest.mark.parametrize("L,seed", [(8.0, 3), (12.5, 4)], ids=["2cells", "4cells"])
+def test_neighblist_mode_is_the_brute_force_set(oracle, L, seed):
+    """SEP_NEIGHBLIST lists every pair within the list cutoff exactly once
+    (also where the linked-cell stencil repeats pairs), the same set as the
+    linked-cell list on a grid of at least 3 cells."""
+    from tests.golden.runs import pair_keys
+    n = 300 if L < 10 else 900
+    w = _random_box(n, L, seed)
+    keys = {}
+    for update in (S.SEP_NEIGHBLIST, S.SEP_LLIST_NEIGHBLIST):
+        s = SepSystem.from_workload(oracle, w, cf=2.5, dt=0.005, skin=0.5, update=update)
+        try:
+            s.reset_retval()
+            s.reset_force()
+            s.force_pairs("AA", 2.5, "sep_lj_shift")
+            keys[update] = pair_keys(oracle.lib, s, cap_per_atom=1024)
+        finally:
+            s.close()
+    ref = _pairs_numpy(w["x"], w["L"], 3.0 * 3.0)
+    assert len(ref) > n
+    assert np.array_equal(keys[S.SEP_NEIGHBLIST], ref)
+    if L > 10:
+        assert np.array_equal(keys[S.SEP_LLIST_NEIGHBLIST], ref)
+    else:
+        assert len(keys[S.SEP_LLIST_NEIGHBLIST]) > len(ref)
+
+
+def _topology_pairs(w, rows, n):
+    """Keys i * n + j (i < j) of every atom pair that shares a row of the
+    topology table (bonds: 2 atoms, angles: 3, dihedrals: 4)."""
+    out = set()
+    for r in rows:
+        atoms = [int(a) for a in r[1:-1]]
+        for a in range(len(atoms)):
+            for b in range(a + 1, len(atoms)):
+                i, j = sorted((atoms[a], atoms[b]))
+                out.add(i * n + j)
+    return out
+
+
+def _within(keys, x, L, rl2):
+    n = len(x)
+    i, j = keys // n, keys % n
+    d = x[i] - x[j]
+    for k in range(3):
+        d[:, k] = _wrap(d[:, k], L[k])
+    return keys[d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2] < rl2]
+
+
+def test_exclusion_sets_follow_the_topology(oracle):
+    """Butane (1000 atoms of the reference's input): the O(N^2) list with
+    SEP_EXCL_BONDED drops exactly the bonded pairs within the list cutoff;
+    the linked-cell list with SEP_EXCL_BONDED drops every pair that shares a
+    bond, an angle or a dihedral (for butane: every same-molecule pair, so
+    it equals SEP_EXCL_SAME_MOL)."""
+    g, R = _g6()
+    w = R.g6_workload(g, "butane")
+    n = len(w["m"])
+    keys = {}
+    for tag, system, update, call in R.G6_CASES:
+        if system == "butane" and update != S.SEP_BRUTE:
+            keys[tag] = R.run_g6_case(oracle, oracle.lib, w, system, update, call)[4]
+    full = keys["butane_nlist_all"]
+    rl2 = (2.5 + 0.25) ** 2  # sys.cf + the default skin
+    assert np.array_equal(full, _pairs_numpy(w["x"], w["L"], rl2))
+    bonded = _within(np.array(sorted(_topology_pairs(w, w["bonds"], n))), w["x"], w["L"], rl2)
+    assert len(bonded) == len(w["bonds"])
+    assert np.array_equal(keys["butane_nlist_bonded"], np.setdiff1d(full, bonded))
+    shared = _topology_pairs(w, w["bonds"], n) | _topology_pairs(w, w["angles"], n) | \
+        _topology_pairs(w, w["dihedrals"], n)
+    shared = _within(np.array(sorted(shared)), w["x"], w["L"], rl2)
+    assert len(shared) == 6 * len(w["dihedrals"])
+    assert np.array_equal(keys["butane_llist_bonded"], np.setdiff1d(full, shared))
+    assert np.array_equal(keys["butane_llist_samemol"], keys["butane_llist_bonded"])
+    assert np.array_equal(keys["butane_nlist_samemol"], keys["butane_llist_bonded"])
+
+
+def test_brute_coulomb_bonded_rule(oracle):
+    """Water (the reference's input): brute-force Coulomb with
+    SEP_EXCL_BONDED excludes a pair only when exactly one partnership holds.
+    O-H shares a bond and an angle (two), so it stays; H-H shares only the
+    angle, so it goes.  Checked against numpy O(N^2) sums with that mask."""
+    g, R = _g6()
+    w = R.g6_workload(g, "water")
+    n = len(w["m"])
+    cf = R.WATER["cf"]
+    mol = np.empty(n, dtype=np.int64)
+    for r in w["bonds"]:
+        mol[r[1]] = mol[r[2]] = r[0]
+    is_h = w["type"] == b"H"
+    assert is_h.sum() == 2 * (~is_h).sum()
+    for opt in (S.SEP_ALL, S.SEP_EXCL_BONDED):
+        tag = "water_brute_coulomb_" + ("all" if opt == S.SEP_ALL else "bonded")
+        case = [c for c in R.G6_CASES if c[0] == tag][0]
+        f, r, _, _, _ = R.run_g6_case(oracle, oracle.lib, w, "water", case[2], case[3])
+        fn = np.zeros((n, 3))
+        e = 0.0
+        for i in range(n):
+            d = w["x"][i] - w["x"]
+            for k in range(3):
+                d[:, k] = _wrap(d[:, k], w["L"][k])
+            r2 = (d ** 2).sum(axis=1)
+            mask = (r2 < cf * cf) & (np.arange(n) != i)
+            if opt == S.SEP_EXCL_BONDED:
+                mask &= ~(is_h[i] & is_h & (mol == mol[i]))
+            rr = np.sqrt(r2[mask])
+            zz = w["q"][i] * w["q"][mask]
+            fn[i] = ((zz * (1.0 / r2[mask] - 1.0 / cf ** 2) / rr)[:, None] * d[mask]).sum(0)
+            e += 0.5 * (zz * (1.0 / rr + (rr - cf) / cf ** 2 - 1.0 / cf)).sum()
+        assert force_error(f, fn) <= 1e-12, opt
+        assert rel(r[2], e) <= 1e-12, opt  # ecoul
