@@ -16,8 +16,11 @@ the device every step with no host copy.
 The timed region is K steps of the mixed form (FP32 pair arithmetic on
 positions quantized to 2^-32 of the box, FP64 accumulation, FP64 positions
 and integrator) and K steps of the plain FP64 form; ``value`` is the mixed
-form's atom-steps/s.  The mixed form's force
-error against FP64 is measured on the final state.  The step's dominant
+form's atom-steps/s.  After the timed runs, the precision study of the
+same workload (seplib_amd/precision.py): 10^4 NVT steps of the mixed form
+with its pair forces checked against FP64 forces on the same positions
+every 100 steps, then 10^4 NVE steps per form for the energy drift.  The
+step's dominant
 kernel -- pair forces, the Nose-Hoover term and the leapfrog fused in one
 launch -- gives the roofline: its average launch duration (``achieved``)
 comes from HIP events on the library stream around 200 back-to-back
@@ -28,10 +31,16 @@ FETCH_SIZE / WRITE_SIZE counter summary committed under profiles/, scaled by
 the counters' reading for a plain device copy of known size.
 
 cpu_baseline: the reference's own C (oracle/_ref/libsepref_fast.so, built
-from its sources with the shipped -Ofast flags) with OpenMP on the host,
-over a bounded 200-step sample of the same workload.
+from its sources with the shipped -Ofast flags) with OpenMP on the host's
+cores (at most 16, the host share of one GPU here; OMP_PROC_BIND=close,
+OMP_PLACES=cores) over a bounded 200-step sample of the same workload, and
+on one thread over 20 steps; the CPU model and core counts are recorded.
+Where the compiled reference is absent, the repository's serial C
+restatement of the same path (oracle/liboracle.so) on one thread, kind
+"port".
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--copy-calib]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-study]
+                    [--copy-calib]
 
 --copy-calib first runs a plain 256 MiB device-to-device copy kernel (for
 the counter calibration of a profiled run).
@@ -101,42 +110,62 @@ def run_gpu(api, w, precision, steps, warmup, time_kernel=True):
     return secs, kern_ms, mean.value, s
 
 
-def force_error(api, state_src):
-    """Mixed vs FP64 forces on one state: max|dF|/RMS|F| and RMS|dF|/RMS|F|."""
-    import numpy as np
+def pair_call(s):
     from seplib_amd import sep as S
-    st = state_src.state()
-    out = {}
-    for prec in (S.SEP_GPU_FP64, S.SEP_GPU_MIXED):
-        s = make_system(api, state_src.workload, prec)
-        s.set_state(st)
-        s.sys.neighb_flag = 1
-        s.reset_retval()
-        s.reset_force()
-        s.force_pairs("AA", RC, "sep_lj_shift", S.SEP_ALL)
-        out[prec] = (np.array(s.view["f"]), s.ret.epot)
-        s.close()
-    f64, f32 = out[S.SEP_GPU_FP64][0], out[S.SEP_GPU_MIXED][0]
-    rms = np.sqrt((f64 ** 2).sum(axis=1).mean())
-    d = np.sqrt(((f32 - f64) ** 2).sum(axis=1))
-    e64, e32 = out[S.SEP_GPU_FP64][1], out[S.SEP_GPU_MIXED][1]
-    return float(d.max() / rms), float(np.sqrt((d ** 2).mean()) / rms), abs(e32 - e64) / abs(e64)
+    s.force_pairs("AA", RC, "sep_lj_shift", S.SEP_ALL)
 
 
-def cpu_baseline(w, steps=200, warm=20):
-    """The compiled reference with OpenMP on the host, bounded sample."""
-    from seplib_amd import sep as S
-    from oracle.checkers import REFERENCE_FAST_LIB, ReferenceAPI
-    if not ReferenceAPI.available(REFERENCE_FAST_LIB):
-        return None
-    api = ReferenceAPI(REFERENCE_FAST_LIB)
+def nve_ops(s):
+    return [s.op_pairs("AA", RC, "sep_lj_shift"), s.op_leapfrog()]
+
+
+def final_state_force_error(api, state_src):
+    """Mixed vs FP64 pair forces on one state: max|dF|/RMS|F|, RMS|dF|/RMS|F|
+    and the relative energy difference."""
+    from seplib_amd import precision as P
+    sampler = P.ForceSampler(lambda prec: make_system(api, state_src.workload, prec),
+                             pair_call)
     try:
-        ncpu = len(os.sched_getaffinity(0))
+        emax, erms = sampler.sample(state_src.state())
+        return emax, erms, sampler.worst_epot
+    finally:
+        sampler.close()
+
+
+def cpu_info():
+    """CPU model, physical cores and logical CPUs of the host (/proc/cpuinfo),
+    and the CPUs this process may run on."""
+    model, cores, logical = None, set(), 0
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    logical += 1
+                elif k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                    cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
     except AttributeError:
-        ncpu = os.cpu_count() or 1
-    threads = max(1, min(16, ncpu))
+        allowed = os.cpu_count() or 1
+    return dict(model=model, physical_cores=len(cores) or None, logical_cpus=logical or None,
+                allowed_cpus=allowed)
+
+
+def _cpu_rate(api, w, threads, steps, warm):
+    from seplib_amd import sep as S
     s = make_system(api, w)
-    api.set_omp(threads, s.sysp)
+    if threads is not None:
+        api.set_omp(threads, s.sysp)
 
     def one():
         s.reset_retval()
@@ -151,31 +180,44 @@ def cpu_baseline(w, steps=200, warm=20):
         one()
     secs = time.perf_counter() - t0
     s.close()
-    return dict(value=len(w["m"]) * steps / secs, unit="atom-steps/s", cores=threads,
-                kind="reference",
-                sample="%d NVT steps of the same N=256000 workload after %d warm-up steps, "
-                       "reference C (shipped -Ofast flags) with %d OpenMP threads, "
-                       "%.1f s" % (steps, warm, threads, secs))
+    return len(w["m"]) * steps / secs, secs
 
 
-def energy_drift(api, w, precision, nvt_steps, nve_steps, every=100):
-    """NVT equilibration, then NVE (no thermostat) with the total energy
-    epot + ekin read every `every` steps: relative drift |E_end - E_0| / |E_0|
-    and the largest excursion max_t |E(t) - E_0| / |E_0|."""
-    s = make_system(api, w, precision)
-    s.gpu_resident(True)
-    s.gpu_run(step_ops(s), nvt_steps, graph=True)
-    nve = [s.op_pairs("AA", RC, "sep_lj_shift"), s.op_leapfrog()]
-    energies = []
-    for k in range(nve_steps // every):
-        s.gpu_run(nve, every, graph=True)
-        s.gpu_sync()
-        energies.append(s.ret.epot + s.ret.ekin)
-    s.close()
-    e0 = energies[0]
-    drift = abs(energies[-1] - e0) / abs(e0)
-    excursion = max(abs(e - e0) for e in energies) / abs(e0)
-    return drift, excursion, e0 / len(w["m"])
+def cpu_baseline(w, steps=200, warm=20, steps1=20, warm1=2):
+    """The compiled reference with OpenMP on the host's cores (bounded
+    sample) and on one thread; the repository's C restatement on one thread
+    where the compiled reference is absent."""
+    from oracle.checkers import REFERENCE_FAST_LIB, OracleAPI, ReferenceAPI
+    info = cpu_info()
+    n = len(w["m"])
+    if ReferenceAPI.available(REFERENCE_FAST_LIB):
+        os.environ.setdefault("OMP_PROC_BIND", "close")
+        os.environ.setdefault("OMP_PLACES", "cores")
+        api = ReferenceAPI(REFERENCE_FAST_LIB)
+        threads = max(1, min(16, info["allowed_cpus"]))
+        value, secs = _cpu_rate(api, w, threads, steps, warm)
+        v1, secs1 = _cpu_rate(api, w, 1, steps1, warm1)
+        return dict(value=value, unit="atom-steps/s", cores=threads, kind="reference",
+                    sample="%d NVT steps of the same N=%d workload after %d warm-up steps, "
+                           "reference C (shipped -Ofast flags) with %d OpenMP threads "
+                           "(OMP_PROC_BIND=close, OMP_PLACES=cores), %.1f s; one thread: "
+                           "%d steps, %.1f s" % (steps, n, warm, threads, secs, steps1, secs1),
+                    single_thread=dict(value=v1, steps=steps1, secs=round(secs1, 2)),
+                    cpu=info)
+    api = OracleAPI()
+    v1, secs1 = _cpu_rate(api, w, None, steps1, warm1)
+    return dict(value=v1, unit="atom-steps/s", cores=1, kind="port",
+                sample="%d NVT steps of the same N=%d workload after %d warm-up steps, the "
+                       "repository's serial C restatement of the path (oracle/liboracle.so), "
+                       "%.1f s (the compiled reference is not built here)"
+                       % (steps1, n, warm1, secs1),
+                single_thread=dict(value=v1, steps=steps1, secs=round(secs1, 2)), cpu=info)
+
+
+def precision_study(api, w, nvt_steps, nve_steps, every=100):
+    from seplib_amd import precision as P
+    return P.study(lambda prec: make_system(api, w, prec), pair_call, step_ops, nve_ops,
+                   nvt_steps, nve_steps, every)
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lj_lj256k_traffic.json")
@@ -211,8 +253,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--copy-calib", action="store_true")
-    ap.add_argument("--drift", action="store_true",
-                    help="also measure the NVE energy drift (1e4 NVT + 1e4 NVE steps per form)")
+    ap.add_argument("--no-study", action="store_true",
+                    help="skip the precision study (1e4 NVT steps of the mixed form with force "
+                         "checks every 100 steps, then 1e4 NVE steps per form for the drift)")
+    ap.add_argument("--study-steps", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -239,9 +283,11 @@ def main():
         log("%s: %.3f s, %.4g atom-steps/s, pair kernel %.2f us, nbar %.2f, rebuilds %d, "
             "epot/N %.6f" % (name, secs, n * args.steps / secs, kms * 1e3, nbar,
                              s.sys.nupdate_neighb, s.ret.epot / n))
-    ferr_max, ferr_rms, eerr = force_error(api, res["mixed"]["sys"])
-    log("mixed vs fp64 forces: max %.3e rms %.3e (relative to RMS|F|), epot %.3e" %
-        (ferr_max, ferr_rms, eerr))
+    ferr_max, ferr_rms, eerr = final_state_force_error(api, res["mixed"]["sys"])
+    log("mixed vs fp64 forces on the final state: max %.3e rms %.3e (relative to RMS|F|), "
+        "epot %.3e" % (ferr_max, ferr_rms, eerr))
+    for r in res.values():
+        r["sys"].close()
 
     m = res["mixed"]
     value = n * args.steps / m["secs"]
@@ -265,17 +311,29 @@ def main():
                     kernel_us=round(m["kms"] * 1e3, 3),
                     algorithmic_bytes=round(bytes_per_atom * n),
                     bytes_per_atom=round(bytes_per_atom, 2))
-    drift = None
-    if args.drift:
-        drift = {}
-        for name, prec in (("fp64", S.SEP_GPU_FP64), ("mixed", S.SEP_GPU_MIXED)):
-            d, x, e0 = energy_drift(api, w, prec, 10000, 10000)
-            drift[name] = dict(drift=d, max_excursion=x, e0_per_atom=e0)
-            log("%s NVE over 1e4 steps after 1e4 NVT: drift %.3e, max excursion %.3e "
-                "(E0/N %.6f)" % (name, d, x, e0))
+    study = None
+    if not args.no_study:
+        k = args.study_steps
+        t0 = time.perf_counter()
+        st = precision_study(api, w, k, k)
+        study = {"nvt_steps": k, "nve_steps": k, "every": 100,
+                 "force_samples": st["force_samples"],
+                 "force_rel_err_max": st["force_err_max"],
+                 "force_rel_err_rms": st["force_err_rms"],
+                 "epot_rel_err_max": st["epot_err_max"],
+                 "nve_drift": {f: {"drift": st[f]["drift"],
+                                   "max_excursion": st[f]["max_excursion"],
+                                   "e0_per_atom": st[f]["e0"] / n,
+                                   "rebuilds": st[f]["rebuilds"]} for f in ("mixed", "fp64")}}
+        log("precision study (%.1f s): mixed trajectory, %d samples: force error max %.3e "
+            "rms %.3e; NVE drift over %d steps: mixed %.3e, fp64 %.3e"
+            % (time.perf_counter() - t0, st["force_samples"], st["force_err_max"],
+               st["force_err_rms"], k, st["mixed"]["drift"], st["fp64"]["drift"]))
     cpu = None if args.no_cpu else cpu_baseline(w, steps=args.cpu_steps)
     if cpu:
-        log("cpu baseline: %.4g atom-steps/s on %d threads" % (cpu["value"], cpu["cores"]))
+        log("cpu baseline (%s): %.4g atom-steps/s on %d threads, %.4g on one thread; %s"
+            % (cpu["kind"], cpu["value"], cpu["cores"], cpu["single_thread"]["value"],
+               cpu["cpu"]["model"]))
     out = {
         "metric": "atom-steps/s LJ N=256k, FP32+FP64-acc vs FP64 GPU at ≤1e-5 force rel err",
         "value": round(value, 1),
@@ -301,8 +359,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    if drift is not None:
-        out["nve_energy_drift"] = drift
+    if study is not None:
+        out["precision_study"] = study
     print(json.dumps(out), flush=True)
 
 

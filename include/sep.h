@@ -18,6 +18,7 @@
 #include "sepmisc.h"
 #include "sepret.h"
 #include "sepcoulomb.h"
+#include "separray.h"
 #include "sepomp.h"
 #include "sepgpu.h"
 

@@ -97,9 +97,25 @@ typedef struct {
 
 /* Runs nsteps of { sep_reset_retval(ret); sep_reset_force(); ops[0..nops) }
  * on a resident system without host synchronisation; ret (and each alpha)
- * hold the last step's values after sep_gpu_sync. */
+ * hold the last step's values after sep_gpu_sync.  At the end it reads the
+ * device error word once (one small copy): a step that fails on the device
+ * (SEP_GPU_ENEIGHB: a list build overflowed; SEP_GPU_ECELL: an atom outside
+ * the cell grid) and every step after it change nothing, so the system stays
+ * at the step before the failure (sys->tnow tells which), and the code is
+ * returned.  After SEP_GPU_ENEIGHB the list has been grown (unless a
+ * sep_gpu_list_limit forbids it), so calling again continues the run. */
 int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret,
 		const sep_gpu_op *ops, int nops, long nsteps, int flags);
+
+/* Bound on the device neighbour list: at most `entries` neighbours per atom
+ * in the full list (both directions; rounded down to a multiple of 16, at
+ * least 16), 0 for none.  The reference sizes each atom's half list once,
+ * with the nneighb of sep_init, and stops at "Too many neighbours"
+ * (source/sepprfrc.c:499-501); this library grows its list instead, unless
+ * a bound is set: then a build that needs more raises SEP_GPU_ENEIGHB.  The
+ * list is grown to the bound if it is smaller; the current list stays in
+ * use until the next build.  No counterpart in the reference. */
+int sep_gpu_list_limit(seppart *atoms, sepsys *sys, int entries);
 
 /* ---- multi-type LJ and force matching --------------------------------- */
 /* The pair forces of sep_force_lj (include/sepprfrc.h:69, source/

@@ -52,10 +52,8 @@ void sep_omp_angle(double **ftot, seppart *ptr, int type, const double angle0,
 void sep_omp_torsion(double **ftot, seppart *ptr, int type, const double g[6],
 		     sepsys *sys);
 
-/* reference include/separray.h:39,90,206 (source/separray.c:31-52,215-222,
- * 419-425): the N x 3 arrays the Model II terms add into */
-double **sep_matrix(size_t nrow, size_t ncol);
-void sep_matrix_set(double **a, size_t nrow, size_t ncol, double value);
-void sep_free_matrix(double **ptr, size_t nrow);
+/* the N x 3 arrays the Model II terms add into: sep_matrix, sep_matrix_set
+ * and sep_free_matrix, declared where the reference declares them */
+#include "separray.h"
 
 #endif

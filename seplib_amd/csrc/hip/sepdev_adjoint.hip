@@ -115,7 +115,8 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_table(
 		fy += __shfl_xor(fy, o, SEPK_WAVE);
 		fz += __shfl_xor(fz, o, SEPK_WAVE);
 	}
-	if (t < n && sub == 0) {
+	// f stays as it is while the device error word is set
+	if (t < n && sub == 0 && !*s.err) {
 		if (OVERWRITE) {
 			s.f[t] = fx;
 			s.f[npad + t] = fy;

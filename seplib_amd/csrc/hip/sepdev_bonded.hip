@@ -109,7 +109,7 @@ __global__ void k_bonded_gather(sepk_state s, long n, long npad, const double *c
 				const int *off, const int *idx)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n)
+	if (t >= n || *s.err)
 		return;
 	const int o = s.orig[t];
 	double f[3];
@@ -159,6 +159,8 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_bond(
 	double lbond, double ks, double L0, double L1, double L2, double *contrib,
 	double *blen, double *partial, long rows)
 {
+	if (*s.err)
+		return; // the whole block: a step after a device error changes nothing
 	const int n = blockIdx.x * blockDim.x + threadIdx.x;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	if (n < nb) {
@@ -222,6 +224,8 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_angle(
 	double angle0, double kth, double ccon, int cossq, double L0, double L1,
 	double L2, double *contrib, double *aval, double *partial, long rows)
 {
+	if (*s.err)
+		return; // the whole block: a step after a device error changes nothing
 	const int n = blockIdx.x * blockDim.x + threadIdx.x;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	if (n < na) {
@@ -299,6 +303,8 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_torsion(
 	sepk_rb g, double L0, double L1, double L2, double *contrib, double *dval,
 	double *partial, long rows)
 {
+	if (*s.err)
+		return; // the whole block: a step after a device error changes nothing
 	const int n = blockIdx.x * blockDim.x + threadIdx.x;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	if (n < nd) {

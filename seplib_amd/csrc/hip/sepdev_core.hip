@@ -167,6 +167,7 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 		sepdev_destroy(d);
 		return nullptr;
 	}
+	d->s[0].err = d->s[1].err = &d->sys->error;
 	// list lengths start at zero so that a pair kernel that runs before any
 	// build reads no list entries
 	(void)hipMemsetAsync(d->nbr_count, 0, d->npad * sizeof(int), d->st);
@@ -750,6 +751,8 @@ extern "C" int sepdev_graph_begin(sepdev *d)
 	d->capturing = 1;
 	return 0;
 }
+
+extern "C" int sepdev_capturing(sepdev *d) { return d->capturing; }
 
 extern "C" int sepdev_graph_end(sepdev *d)
 {

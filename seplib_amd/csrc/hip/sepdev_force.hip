@@ -18,10 +18,10 @@ int sepk_timer_end(sepdev *d, sepk_timer *t);
 
 // ---- lazy force reset ------------------------------------------------------
 
-__global__ void k_zero_forces(double *f, long count)
+__global__ void k_zero_forces(double *f, long count, const int *err)
 {
 	long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t < count)
+	if (t < count && !*err)
 		f[t] = 0.0;
 }
 
@@ -33,7 +33,7 @@ int sepk_materialize_forces(sepdev *d)
 	if (!d->f_zero_pending)
 		return 0;
 	const long cnt = 3 * d->npad;
-	k_zero_forces<<<sepk_blocks(cnt), SEPK_BLOCK, 0, d->st>>>(d->s[0].f, cnt);
+	k_zero_forces<<<sepk_blocks(cnt), SEPK_BLOCK, 0, d->st>>>(d->s[0].f, cnt, d->s[0].err);
 	SEPK_LAUNCH_CHECK();
 	d->f_zero_pending = 0;
 	return 0;
@@ -64,12 +64,13 @@ extern "C" int sepdev_zero_forces(sepdev *d)
 // so a pair adds to the stress only when it crosses the box (k != 0), and
 // each atom adds 2 F_i (x) x_i once, here, from its summed force.  The six
 // components are {xx, xy, xz, yy, yz, zz} of f (x) r; the finisher halves
-// the full-list sums.
+// the full-list sums.  frozen: the device error word is set, and f is left
+// as it is.
 template <bool OVERWRITE>
 __device__ __forceinline__ void sepk_pair_epilogue(
 	long t, int sub, long n, long npad, bool act, double fx, double fy, double fz,
 	double px, double py, double pz, double (&acc)[7], double *f, double *partial,
-	long row, long rows)
+	long row, long rows, bool frozen)
 {
 #pragma unroll
 	for (int o = 1; o < SEPK_SUB; o <<= 1) {
@@ -77,7 +78,7 @@ __device__ __forceinline__ void sepk_pair_epilogue(
 		fy += __shfl_xor(fy, o, SEPK_WAVE);
 		fz += __shfl_xor(fz, o, SEPK_WAVE);
 	}
-	if (t < n && sub == 0) {
+	if (t < n && sub == 0 && !frozen) {
 		if (OVERWRITE) {
 			f[t] = fx;
 			f[npad + t] = fy;
@@ -174,6 +175,28 @@ __device__ __forceinline__ double sepk_recip(double r2)
 	return fma(y, e, y);
 }
 
+// The cutoff test of the mixed LJ loops.  Each quantized coordinate is
+// within half a quantization step of the FP64 one, so the quantized r^2 is
+// within c.band of the r^2 the plain kernel forms; a pair that close to the
+// cutoff is decided on the FP64 records, as the plain kernel decides it.
+// (The force of sep_lj and sep_lj_shift jumps by F(rc) = -0.039 at rc = 2.5:
+// a pair taken by one form and dropped by the other would put that whole
+// jump into the force error of both atoms.)  About one pair in 10^8 lies in
+// the band.
+template <bool FAST>
+__device__ __forceinline__ bool sepk_inside_mixed(const sepk_state &s, long t, unsigned j,
+						  double r2, const sepk_lj &c)
+{
+	if (!(fabs(r2 - c.cf2) < c.band))
+		return r2 < c.cf2;
+	const sepk_pos pi = s.pos[t], pj = s.pos[j];
+	double kx, ky, kz;
+	const double dx = sepk_image<FAST>(pi.x - pj.x, c.L0, c.iL0, kx);
+	const double dy = sepk_image<FAST>(pi.y - pj.y, c.L1, c.iL1, ky);
+	const double dz = sepk_image<FAST>(pi.z - pj.z, c.L2, c.iL2, kz);
+	return dx * dx + dy * dy + dz * dz < c.cf2;
+}
+
 // The pair loop of atom t, lane `sub`: the lane's share of the atom's force
 // and of the block's energy and crossing stress, and the position the
 // stress identity pairs the summed force with (px, py, pz).
@@ -222,7 +245,7 @@ __device__ __forceinline__ void sepk_lj_pairs(
 					const double dy = (double)iy * c.qd1;
 					const double dz = (double)iz * c.qd2;
 					const double r2 = dx * dx + dy * dy + dz * dz;
-					if (!(r2 < c.cf2))
+					if (!sepk_inside_mixed<FAST>(s, t, (unsigned)j[u], r2, c))
 						continue;
 					const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
 					const float q3 = q * q * q;
@@ -309,6 +332,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 	const long lb = sepk_xcd_block();
 	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
 	const int sub = threadIdx.x % SEPK_SUB;
+	const bool frozen = *s.err != 0;
 	double fx = 0.0, fy = 0.0, fz = 0.0;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	bool act = t < n;
@@ -321,7 +345,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj(
 	sepk_lj_pairs<TYPED, MIXED, FAST>(s, nbr, cnt, npad, c, t, sub, act, ti, fx, fy, fz, acc,
 					   px, py, pz);
 	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, px, py, pz, acc, s.f,
-				      partial, lb, rows);
+				      partial, lb, rows, frozen);
 }
 
 typedef void (*sepk_lj_kernel)(sepk_state, const int *, const int *, long, long,
@@ -444,6 +468,11 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 	c.qd0 = a->L[0] / SEPK_QSCALE;
 	c.qd1 = a->L[1] / SEPK_QSCALE;
 	c.qd2 = a->L[2] / SEPK_QSCALE;
+	{
+		// |dx~ - dx| <= one step per axis, so |r2~ - r2| <= q (2 sqrt(3) rc + 3 q)
+		const double q = fmax(c.qd0, fmax(c.qd1, c.qd2));
+		c.band = q * (4.0 * sqrt(a->cf2) + 4.0 * q) + 1e-12 * a->cf2;
+	}
 	c.Af = (float)c.A;
 	c.nBf = (float)c.nB;
 	c.Cf = (float)c.C;
@@ -593,6 +622,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 	const long lb = sepk_xcd_block();
 	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
 	const int sub = threadIdx.x % SEPK_SUB;
+	const bool frozen = *s.err != 0;
 	double fx = 0.0, fy = 0.0, fz = 0.0;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	bool act = t < n;
@@ -612,7 +642,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 		const double py = act ? (double)(unsigned)qi.y * c.qd1 : 0.0;
 		const double pz = act ? (double)(unsigned)qi.z * c.qd2 : 0.0;
 		sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, px, py, pz, acc,
-					      s.f, partial, lb, rows);
+					      s.f, partial, lb, rows, frozen);
 		return;
 	}
 	if (act) {
@@ -649,7 +679,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_coulomb(
 		}
 	}
 	sepk_pair_epilogue<OVERWRITE>(t, sub, n, npad, act, fx, fy, fz, pi.x, pi.y, pi.z, acc,
-				      s.f, partial, lb, rows);
+				      s.f, partial, lb, rows, frozen);
 }
 
 // A pair call (LJ, typed or not) and the shifted-force Coulomb call right
@@ -668,6 +698,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 	const long lb = sepk_xcd_block();
 	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
 	const int sub = threadIdx.x % SEPK_SUB;
+	const bool frozen = *s.err != 0;
 	double fl[3] = {0.0, 0.0, 0.0}, fc[3] = {0.0, 0.0, 0.0};
 	double al[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	double ac[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -764,7 +795,8 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 				// the LJ term as in sepk_lj_pairs
 				const bool tl = !TYPED || (ti == c.t0 && tj[u] == c.t1) ||
 						(ti == c.t1 && tj[u] == c.t0);
-				if (act_l && tl && qj[u].w != SEPK_QPHANTOM && r2 < c.cf2) {
+				if (act_l && tl && qj[u].w != SEPK_QPHANTOM &&
+				    sepk_inside_mixed<FAST>(s, t, (unsigned)j[u], r2, c)) {
 					const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
 					const float q3 = q * q * q;
 					const double ft = (double)(fmaf(c.Af, q3, c.nBf) * q3 * q);
@@ -805,7 +837,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 			fl[k] += __shfl_xor(fl[k], o, SEPK_WAVE);
 			fc[k] += __shfl_xor(fc[k], o, SEPK_WAVE);
 		}
-	if (t < n && sub == 0) {
+	if (t < n && sub == 0 && !frozen) {
 		for (int k = 0; k < 3; k++) {
 			double v = OVERWRITE ? fl[k] : s.f[k * npad + t];
 			if (!OVERWRITE && act_l)
@@ -960,6 +992,8 @@ extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot
 __global__ __launch_bounds__(SEPK_BLOCK) void k_nh_sum(sepk_state s, long n, long npad,
 							double *partial, long rows)
 {
+	if (*s.err)
+		return; // the finisher applies nothing either
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	double acc[1] = {0.0};
 	if (t < n) {
@@ -975,7 +1009,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_nh_sum(sepk_state s, long n, lon
 __global__ void k_nh_apply(sepk_state s, long n, long npad, const double *alpha)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n)
+	if (t >= n || *s.err)
 		return;
 	const double al = *alpha;
 	const double m = s.m[t];
@@ -1109,14 +1143,22 @@ __device__ __forceinline__ void sepk_lf_atom(const sepk_state &s, const sepk_lfo
 // sepk_lf_atom spread over lanes 0..2 of the atom's SEPK_SUB lanes, one
 // axis each; lane 0 gathers the other axes' values by shuffles and forms
 // the sums in sepk_lf_atom's order (the same bits).  Every lane of the wave
-// calls it.
+// calls it.  frozen (the device error word is set): the atom stays where it
+// is, its position records copied to o unchanged.
 __device__ __forceinline__ void sepk_lf_atom_lanes(const sepk_state &s, const sepk_lfout &o,
 						   long t, long npad, int sub, bool act, double F0,
 						   double F1, double F2, double dt, double L0,
 						   double L1, double L2, int nh, double al,
 						   int pending, double qs0, double qs1, double qs2,
-						   double (&acc)[9])
+						   double (&acc)[9], bool frozen)
 {
+	if (frozen) {
+		if (act && sub == 0) {
+			o.pos[t] = s.pos[t];
+			o.qpos[t] = s.qpos[t];
+		}
+		return;
+	}
 	double x = 0.0, v = 0.0, vh = 0.0, dr = 0.0, m = 0.0, q = 0.0;
 	if (act && sub < 3) {
 		const int k = sub;
@@ -1189,6 +1231,8 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 	const sepdev_sys *sys, int nh, const double *alpha, double Td, double dt_tau2,
 	double *partial, long rows, double qs0, double qs1, double qs2)
 {
+	if (sys->error)
+		return; // positions are updated in place: nothing to copy
 	const long lb = sepk_xcd_block();
 	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
 	const int sub = threadIdx.x % SEPK_SUB;
@@ -1234,6 +1278,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) __attribute__((amdgpu_waves_per_eu(MIXE
 	}
 	double lf[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	const int pending = sys->xn_pending;
+	const bool frozen = sys->error != 0;
 	double al = 0.0;
 	if (nh)
 		al = sepk_nh_alpha(*alpha, sys->sumv2, (double)n, Td, dt_tau2);
@@ -1248,7 +1293,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) __attribute__((amdgpu_waves_per_eu(MIXE
 		acc[6] = fma(gz, pz, acc[6]);
 	}
 	sepk_lf_atom_lanes(s, o, t, npad, sub, act, fx, fy, fz, dt, c.L0, c.L1, c.L2, nh, al,
-			   pending, qs0, qs1, qs2, lf);
+			   pending, qs0, qs1, qs2, lf, frozen);
 	sepk_block_partial<7>(acc, 0u, ppart, lb, rows);
 	sepk_block_partial<9>(lf, 1u << 7, lpart, lb, rows);
 }

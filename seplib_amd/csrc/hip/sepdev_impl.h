@@ -48,6 +48,9 @@ struct sepk_state {
 	int *mol;
 	int *orig;
 	int *partners; // 40 blocks of npad, allocated with the topology
+	// the device error word (sepdev_sys::error): a step's kernels leave the
+	// state unchanged while it is set
+	const int *err;
 };
 
 // Constants of one LJ pair-force launch (sep_force_pairs / sep_force_lj).
@@ -55,6 +58,8 @@ struct sepk_lj {
 	double cf2, A, nB, C, nD, sig2, ushift;
 	double L0, L1, L2, iL0, iL1, iL2;
 	double qd0, qd1, qd2; // L / 2^32: one step of the quantized positions
+	// the quantized r^2 is within band of the FP64 one (mixed kernels)
+	double band;
 	float Af, nBf, Cf, nDf, sig2f, ushiftf;
 	signed char t0, t1;
 };
@@ -127,6 +132,7 @@ struct sepdev {
 	// ((k/SEPK_SUB)*npad + t)*SEPK_SUB + k%SEPK_SUB, nbr_count[t] entries,
 	// at most cap = lcap*SEPK_SUB
 	int cap, lcap;
+	int cap_limit; // entries a build may write per atom (0: cap)
 	int *nbr, *nbr_count;
 	int list_built;
 	sepdev_grid grid; // grid of the last build

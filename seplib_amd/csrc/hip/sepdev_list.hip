@@ -70,7 +70,7 @@ __device__ __forceinline__ long sepk_cell_of(const sepk_pos &p, const sepdev_gri
 __global__ void k_bin(const sepk_pos *pos, long n, sepdev_grid g, int *cellid, int *slot,
 		      int *count, sepdev_sys *sys, int conditional)
 {
-	if (conditional && sys->neighb_flag == 0)
+	if ((conditional && sys->neighb_flag == 0) || sys->error)
 		return;
 	// the build's atomic maximum starts here
 	if (blockIdx.x == 0 && threadIdx.x == 0)
@@ -121,12 +121,19 @@ __global__ void k_bin(const sepk_pos *pos, long n, sepdev_grid g, int *cellid, i
 // Exclusive scan of count[0..ncell) into start[0..ncell], zeroing count for
 // the next rebuild.  One workgroup; each thread scans SEPK_SCAN_ITEMS
 // consecutive cells of a pass in registers (one memory round trip per pass).
+// It also fixes whether the rest of the rebuild runs: sys->frozen = an error
+// is set (by an earlier step, or by this binning), read by the scatter,
+// permutation and build kernels, which must all run or all skip (the build
+// may itself raise an error while it runs).  The counts are zeroed either
+// way, so a later rebuild starts from zero.
 #define SEPK_SCAN_ITEMS 16
 __global__ __launch_bounds__(1024) void k_scan(int *count, int *start, long ncell,
 					       sepdev_sys *sys, int conditional)
 {
 	if (conditional && sys->neighb_flag == 0)
 		return;
+	if (threadIdx.x == 0)
+		sys->frozen = sys->error != 0;
 	__shared__ int wsum[16];
 	__shared__ int carry_s;
 	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -190,7 +197,7 @@ __global__ __launch_bounds__(1024) void k_scan(int *count, int *start, long ncel
 __global__ void k_scatter(const int *cellid, const int *slot, const int *start, int *order,
 			  long n, sepdev_sys *sys, int conditional)
 {
-	if (conditional && sys->neighb_flag == 0)
+	if ((conditional && sys->neighb_flag == 0) || sys->frozen)
 		return;
 	for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
 	     t += (long)gridDim.x * blockDim.x)
@@ -275,7 +282,7 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 			       long npad, sepdev_sys *sys, int conditional,
 			       int has_partners)
 {
-	if (conditional && sys->neighb_flag == 0)
+	if ((conditional && sys->neighb_flag == 0) || sys->frozen)
 		return;
 	const int pending = sys->xn_pending;
 	const int lane = threadIdx.x & 63;
@@ -656,7 +663,7 @@ __global__ __launch_bounds__(SEPK_BT) __attribute__((amdgpu_waves_per_eu(EXCL ==
 	long n, long npad, int cap, int *nbr, int *cnt, sepdev_sys *sys, int conditional,
 	int has_partners)
 {
-	if (conditional && sys->neighb_flag == 0)
+	if ((conditional && sys->neighb_flag == 0) || sys->frozen)
 		return;
 	__shared__ sepk_build_lds L;
 	const bool shift = g.shift_ok && sys->unclean == 0;
@@ -721,6 +728,28 @@ extern "C" int sepdev_reserve_list(sepdev *d, int cap)
 	return 0;
 }
 
+extern "C" int sepdev_list_limit(sepdev *d, int cap, int *applied)
+{
+	if (cap < 0)
+		return sepk_fail(SEP_GPU_EARG, "list limit %d", cap);
+	// whole batches: a truncated list then ends on a batch boundary, and
+	// the pair kernels read no entry the build did not write
+	if (cap > 0) {
+		cap = cap / SEPK_PAD * SEPK_PAD;
+		if (cap < SEPK_PAD)
+			cap = SEPK_PAD;
+	}
+	d->cap_limit = cap;
+	if (cap > 0 && (!d->nbr || cap > d->cap)) {
+		int rc = sepdev_reserve_list(d, cap);
+		if (rc)
+			return rc;
+	}
+	if (applied)
+		*applied = cap;
+	return 0;
+}
+
 extern "C" int sepdev_reset_max_dist(sepdev *d)
 {
 	// owed to the next leapfrog, which then starts its maximum at zero
@@ -743,7 +772,6 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 		return sepk_fail(SEP_GPU_EARG, "build_list: %ld cells", ncell);
 	if (g.opt == 2 && !d->s[0].partners)
 		return sepk_fail(SEP_GPU_ESTATE, "SEP_EXCL_BONDED needs bonded partners");
-	d->fork_ready = 0; // a rebuild permutes the atoms
 	// a recorded pair call reads the current list: launch it before the
 	// list can be resized or rebuilt
 	int rc = sepk_flush_pair(d);
@@ -752,6 +780,9 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 	rc = sepk_flush_nh(d);
 	if (rc)
 		return rc;
+	// a rebuild permutes the atoms: a fork point recorded before it (also
+	// by the flushed pair call above) must not serve the terms after it
+	d->fork_ready = 0;
 
 	// size buffers on the host side of any capture
 	if (ncell > d->ncell_cap) {
@@ -821,8 +852,9 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 	if (bb > SEPK_BGRID)
 		bb = SEPK_BGRID;
 	auto kb = g.opt == 2 ? k_build<2> : g.opt == 3 ? k_build<3> : k_build<0>;
+	const int capb = d->cap_limit > 0 && d->cap_limit < d->cap ? d->cap_limit : d->cap;
 	kb<<<(unsigned)bb, SEPK_BT, 0, d->st>>>(spare, cur, d->cell_start, g, ncell, (int)nslice,
-						      d->n, d->npad, d->cap, d->nbr, d->nbr_count,
+						      d->n, d->npad, capb, d->nbr, d->nbr_count,
 						      d->sys, conditional, hp);
 	SEPK_LAUNCH_CHECK();
 	// clears the rebuild flag and the owed xn reset

@@ -183,7 +183,9 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, 
 		}
 	}
 	__syncthreads();
-	if (threadIdx.x == 0)
+	// after a device error the step changed nothing, and neither does this
+	const bool frozen = ssys.error != 0;
+	if (threadIdx.x == 0 && !frozen)
 		for (int u = 0; u < f.ntask; u++) {
 			const sepk_task &k = f.t[u];
 			// a conditional build that did not run leaves nothing to finish
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, 
 			f.t[u].ret[word] = sret[u][word];
 	}
 	// the per-step record: a leapfrog's return slot and the new tnow
-	for (int u = 0; u < f.ntask; u++) {
+	for (int u = 0; u < f.ntask && !frozen; u++) {
 		const sepk_task &k = f.t[u];
 		if (k.mode != SEPK_FIN_LF || !k.ring || rmap[u] < 0)
 			continue;
@@ -365,7 +367,7 @@ void sepk_task_ret(sepdev *d, sepk_task &t, int slot)
 __global__ void k_nh_apply2(sepk_state s, long n, long npad, const double *alpha)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (t >= n)
+	if (t >= n || *s.err)
 		return;
 	const double al = *alpha;
 	const double m = s.m[t];

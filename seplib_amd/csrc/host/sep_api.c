@@ -145,6 +145,9 @@ void sep_load(seppart *ptr, int nneighb, size_t npart, const char *file)
 	seph_ctx *c = seph_find(ptr);
 	if (got && c && c->resident) {
 		sepdev_sys ds;
+		/* the file's bond, angle and dihedral partners replace the
+		 * device's at the next call that excludes bonded pairs */
+		c->partners_on_device = 0;
 		int rc = seph_put_atoms(c);
 		if (!rc)
 			rc = seph_dev_fail(sepdev_get_sys(c->dev, &ds));

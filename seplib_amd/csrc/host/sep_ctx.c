@@ -644,6 +644,25 @@ static int write_half_lists(seph_ctx *c)
 	return rc;
 }
 
+/* After a build that overflowed the list (SEP_GPU_ENEIGHB, longest list
+ * max_full): grow the list and clear the error, or, where the longest list
+ * exceeds the bound set by sep_gpu_list_limit, report the overflow. */
+int seph_grow_list(seph_ctx *c, sepdev_sys *ds)
+{
+	if (c->list_limit > 0 && ds->max_full > c->list_limit) {
+		ds->error = 0;
+		sepdev_put_sys(c->dev, ds);
+		return seph_fail(SEP_GPU_ENEIGHB,
+				 "Too many neighbours: a list of %d entries exceeds the limit of %d",
+				 ds->max_full, c->list_limit);
+	}
+	ds->error = 0;
+	int rc = seph_dev_fail(sepdev_put_sys(c->dev, ds));
+	if (!rc)
+		rc = seph_dev_fail(sepdev_reserve_list(c->dev, ds->max_full + ds->max_full / 4 + 32));
+	return rc;
+}
+
 int seph_build(seph_ctx *c, sepsys *sys, const sepdev_grid *g, int conditional)
 {
 	(void)sys;
@@ -655,10 +674,7 @@ int seph_build(seph_ctx *c, sepsys *sys, const sepdev_grid *g, int conditional)
 	if ((rc = seph_dev_fail(sepdev_get_sys(c->dev, &ds))))
 		return rc;
 	if (ds.error == SEP_GPU_ENEIGHB) {
-		ds.error = 0;
-		if ((rc = seph_dev_fail(sepdev_put_sys(c->dev, &ds))))
-			return rc;
-		if ((rc = seph_dev_fail(sepdev_reserve_list(c->dev, ds.max_full + ds.max_full / 4 + 32))))
+		if ((rc = seph_grow_list(c, &ds)))
 			return rc;
 		if ((rc = seph_dev_fail(sepdev_build_list(c->dev, g, 0))))
 			return rc;
@@ -670,15 +686,22 @@ int seph_build(seph_ctx *c, sepsys *sys, const sepdev_grid *g, int conditional)
 
 static int grid_checked(const seph_ctx *c, const sepdev_grid *g)
 {
-	return c->checked_valid && c->checked_n[0] == g->n[0] && c->checked_n[1] == g->n[1] &&
-	       c->checked_n[2] == g->n[2] && c->checked_opt == g->opt &&
-	       c->checked_rlist2 == g->rlist2;
+	for (int i = 0; i < c->nchecked && i < SEPH_CHECKED; i++) {
+		const seph_grid_key *k = &c->checked[i];
+		if (k->n[0] == g->n[0] && k->n[1] == g->n[1] && k->n[2] == g->n[2] &&
+		    k->opt == g->opt && k->rlist2 == g->rlist2)
+			return 1;
+	}
+	return 0;
 }
 
 int seph_build_resident(seph_ctx *c, const sepdev_grid *g, int conditional)
 {
 	int rc = seph_dev_fail(sepdev_build_list(c->dev, g, conditional));
-	if (rc || grid_checked(c, g))
+	/* a capture cannot hold the host check below (a stream
+	 * synchronisation); an overflow there freezes the run, and sep_gpu_run
+	 * reports it after the replay */
+	if (rc || grid_checked(c, g) || sepdev_capturing(c->dev))
 		return rc;
 	/* first build of this grid: the initial capacity comes from the mean
 	 * density, which grids of fewer than 3 cells per axis (whose stencil
@@ -688,19 +711,18 @@ int seph_build_resident(seph_ctx *c, const sepdev_grid *g, int conditional)
 	if ((rc = seph_dev_fail(sepdev_get_sys(c->dev, &ds))))
 		return rc;
 	if (ds.error == SEP_GPU_ENEIGHB) {
-		ds.error = 0;
-		if ((rc = seph_dev_fail(sepdev_put_sys(c->dev, &ds))) ||
-		    (rc = seph_dev_fail(sepdev_reserve_list(c->dev, ds.max_full + ds.max_full / 4 + 32))) ||
+		if ((rc = seph_grow_list(c, &ds)) ||
 		    (rc = seph_dev_fail(sepdev_build_list(c->dev, g, 0))))
 			return rc;
 	}
 	if ((rc = seph_check_device_error(c)))
 		return rc;
-	for (int k = 0; k < 3; k++)
-		c->checked_n[k] = g->n[k];
-	c->checked_opt = g->opt;
-	c->checked_rlist2 = g->rlist2;
-	c->checked_valid = 1;
+	seph_grid_key *k = &c->checked[c->nchecked % SEPH_CHECKED];
+	for (int a = 0; a < 3; a++)
+		k->n[a] = g->n[a];
+	k->opt = g->opt;
+	k->rlist2 = g->rlist2;
+	c->nchecked++;
 	return 0;
 }
 

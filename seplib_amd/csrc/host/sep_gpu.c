@@ -187,6 +187,30 @@ static int run_ops(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *o
 	return sep_gpu_last_error();
 }
 
+/* The device error word after a run: a failed step and the ones after it
+ * left the state alone; report the failure, growing an overflowed list
+ * (within any bound) so that the next run can continue. */
+static int run_check(seph_ctx *c)
+{
+	sepdev_sys ds;
+	int rc = seph_dev_fail(sepdev_get_sys(c->dev, &ds));
+	if (rc || !ds.error)
+		return rc;
+	if (ds.error == SEP_GPU_ENEIGHB) {
+		const int max_full = ds.max_full;
+		if ((rc = seph_grow_list(c, &ds)))
+			return rc;
+		return seph_fail(SEP_GPU_ENEIGHB,
+				 "neighbour list overflow in a resident run (longest list %d): "
+				 "the system stays at the step before it; the list was grown",
+				 max_full);
+	}
+	return seph_check_device_error(c);
+}
+
+static int run_steps(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
+		     int nops, long nsteps, int flags);
+
 int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
 		int nops, long nsteps, int flags)
 {
@@ -198,6 +222,29 @@ int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
 	if (nsteps <= 0)
 		return 0;
 	sep_gpu_clear_error();
+	int rc = run_steps(atoms, sys, ret, ops, nops, nsteps, flags);
+	const int rc2 = run_check(c);
+	return rc ? rc : rc2;
+}
+
+int sep_gpu_list_limit(seppart *atoms, sepsys *sys, int entries)
+{
+	if (entries < 0)
+		return seph_fail(SEP_GPU_EARG, "list limit %d", entries);
+	seph_ctx *c = seph_ctx_for(atoms, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	int applied = 0;
+	int rc = seph_dev_fail(sepdev_list_limit(c->dev, entries, &applied));
+	if (!rc)
+		c->list_limit = applied;
+	return rc;
+}
+
+static int run_steps(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
+		     int nops, long nsteps, int flags)
+{
+	seph_ctx *c = seph_ctx_for(atoms, sys);
 	/* the first step runs eagerly: it binds slots and sizes every buffer */
 	int rc = run_one_step(atoms, sys, ret, ops, nops, flags);
 	if (rc || nsteps == 1)

@@ -26,6 +26,12 @@ typedef struct {
 	int slot;
 } seph_bind;
 
+#define SEPH_CHECKED 4
+typedef struct {
+	int n[3], opt;
+	double rlist2;
+} seph_grid_key;
+
 typedef struct {
 	seppart *atoms;
 	long n;
@@ -48,9 +54,13 @@ typedef struct {
 	 * while resident, where types change only through sep_gpu_upload */
 	char uniform_type;
 	int uniform_type_valid;
-	/* grid whose list capacity was confirmed on the host (resident mode) */
-	int checked_n[3], checked_opt, checked_valid;
-	double checked_rlist2;
+	/* grids whose list capacity was confirmed on the host (resident mode);
+	 * one step may alternate between grids, e.g. brute-force pair and
+	 * Coulomb calls at different cutoffs */
+	seph_grid_key checked[SEPH_CHECKED];
+	int nchecked;
+	/* sep_gpu_list_limit: entries per atom a build may write (0: none) */
+	int list_limit;
 	/* host staging */
 	double *v3;
 	int *i3;
@@ -92,6 +102,8 @@ int seph_put_atoms(seph_ctx *c);
  * the fields the hot path does not write are left as they are. */
 void seph_ret_unpack(sepret *r, const double *w);
 int seph_check_device_error(seph_ctx *c);
+/* after a list overflow: grow the list (clearing the error) or report it */
+int seph_grow_list(seph_ctx *c, sepdev_sys *ds);
 
 /* return/alpha slots */
 int seph_ret_slot(seph_ctx *c, sepret *ret);   /* resident: bound slot */

@@ -42,8 +42,13 @@ typedef struct {
 	int error;           /* sticky device error code (SEP_GPU_*)   */
 	int max_full;        /* largest full-list length of the last build */
 	int unclean;         /* the last binning saw an atom outside [0, n*l) */
+	int frozen;          /* error was set when this rebuild's binning ended */
 	double sumv2;        /* last sum m v^2 (Nose-Hoover)           */
 } sepdev_sys;
+/* While error is nonzero every kernel of a step leaves the state alone (a
+ * fused step copies its positions through unchanged) and the finisher
+ * applies nothing, so a resident run that fails stays at the step before
+ * the failure; the host reads and clears error. */
 
 /* ---- cell grid + stencil of one neighbour-list build */
 #define SEPDEV_MAX_OFF 28
@@ -114,6 +119,11 @@ int sepdev_scalar_put(sepdev *d, int slot, double v);
 int sepdev_build_list(sepdev *d, const sepdev_grid *g, int conditional);
 /* grow the per-atom full-list capacity (drops the current list) */
 int sepdev_reserve_list(sepdev *d, int cap);
+/* upper bound on the entries a build writes per atom (full list), rounded
+ * down to a multiple of SEPDEV_LANES * 4; 0: the allocation alone.  The
+ * list is grown to the bound if it is smaller; a longer list raises
+ * SEP_GPU_ENEIGHB.  Returns the bound in effect in *applied. */
+int sepdev_list_limit(sepdev *d, int cap, int *applied);
 int sepdev_list_capacity(sepdev *d);
 /* sys->max_dist2 = 0 on the device (resident sep_reset_force) */
 int sepdev_reset_max_dist(sepdev *d);
@@ -198,6 +208,8 @@ int sepdev_timing_get(sepdev *d, const char *name, double *ms, long *count);
  * launches (replaces f with the pair forces; queues no reduction) */
 int sepdev_time_pair(sepdev *d, int reps, double *ms_per_launch);
 int sepdev_graph_begin(sepdev *d);
+/* 1 between sepdev_graph_begin and sepdev_graph_end */
+int sepdev_capturing(sepdev *d);
 int sepdev_graph_end(sepdev *d);
 int sepdev_graph_launch(sepdev *d, long times);
 int sepdev_graph_drop(sepdev *d);

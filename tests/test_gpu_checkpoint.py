@@ -79,3 +79,43 @@ def test_load_with_smaller_neighbour_arrays(product, tmp_path):
         assert len(half) > 0
     finally:
         s.close()
+
+
+def test_resident_load_replaces_bonded_partners(product, tmp_path):
+    """sep_load reads each atom's bond, angle and dihedral partners from the
+    file: a resident system that excluded bonded pairs before the load
+    excludes by the loaded partners after it (here none), as a synchronised
+    system loaded from the same file does."""
+    import os
+    from tests.golden import runs as R
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "g6_excl.npz"),
+                allow_pickle=False)
+    w = R.g6_workload(g, "butane")
+    plain = {k: v for k, v in w.items() if k not in ("bonds", "angles", "dihedrals")}
+    path = str(tmp_path / "nobonds.bin").encode()
+    kw = dict(cf=2.5, dt=0.001)
+    lib = product.lib
+    src = SepSystem.from_workload(product, plain, **kw)
+    b = SepSystem.from_workload(product, w, **kw)
+    c = SepSystem.from_workload(product, plain, **kw)
+    try:
+        lib.sep_save(src.atoms, src.n, path)
+        b.gpu_resident(True)
+        b.reset_retval()
+        b.reset_force()
+        b.force_pairs("CC", 2.5, "sep_lj_shift", S.SEP_EXCL_BONDED)  # partners uploaded
+        lib.sep_load(b.atoms, S.SEP_NEIGHB, b.n, path)
+        for s in (b, c):
+            s.reset_retval()
+            s.reset_force()
+            s.force_pairs("CC", 2.5, "sep_lj_shift", S.SEP_EXCL_BONDED)
+        b.gpu_sync()
+        fb, fc = np.array(b.view["f"]), np.array(c.view["f"])
+        assert force_error(fb, fc) <= FP64_TOL
+        assert rel(b.ret.epot, c.ret.epot) <= FP64_TOL
+        # without exclusions the bonded neighbours dominate the forces
+        assert np.abs(fc).max() > 1e3
+    finally:
+        src.close()
+        b.close()
+        c.close()

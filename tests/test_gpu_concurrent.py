@@ -184,10 +184,13 @@ def test_water_pair_coulomb_walk_bit_identical(product, water, precision):
     _assert_same(out[0], out[1])
 
 
-def test_water_brute_pair_then_coulomb_order(product, water):
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_water_brute_pair_then_coulomb_order(product, water, graph):
     """SEP_BRUTE: each pair call builds its own list, so a recorded pair call
     has to run before the Coulomb call's build replaces the list.  A
-    sep_gpu_run step equals the same calls made one at a time."""
+    sep_gpu_run step equals the same calls made one at a time.  The two
+    calls build at different cutoffs (two grids in one step), which a graph
+    capture must hold without a host check between them."""
     kw = dict(cf=WAT.RC_COUL, dt=WAT.DT, nneighb=WAT.NNEIGHB, update=S.SEP_BRUTE)
     out = []
     for run in (True, False):
@@ -195,7 +198,7 @@ def test_water_brute_pair_then_coulomb_order(product, water):
         try:
             s.gpu_resident(True)
             if run:
-                s.gpu_run(WAT.step_ops(s), 5, graph=False)
+                s.gpu_run(WAT.step_ops(s), 5, graph=graph)
             else:
                 for _ in range(5):
                     s.reset_retval()
@@ -211,3 +214,37 @@ def test_water_brute_pair_then_coulomb_order(product, water):
             s.close()
     assert np.abs(out[0]["f"]).max() > 0
     _assert_same(out[0], out[1])
+
+
+def _water_brute_equal_cutoffs(s):
+    """LJ and Coulomb at the same cutoff, then the bonded terms: the
+    Coulomb call's brute-force build reuses the pair call's grid, and the
+    pair call it flushes records a fork point before that build."""
+    return [s.op_pairs("OO", WAT.RC_COUL, "sep_lj_shift", S.SEP_EXCL_SAME_MOL),
+            s.op_coulomb(WAT.RC_COUL, S.SEP_EXCL_SAME_MOL),
+            s.op_bonded(S.OP_BOND, 0, [WAT.LBOND, WAT.KBOND]),
+            s.op_bonded(S.OP_ANGLE, 0, [WAT.THETA0, WAT.KANGLE]),
+            s.op_nosehoover(WAT.TEMP, WAT.TAU),
+            s.op_leapfrog()]
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_water_brute_equal_cutoffs_two_streams(product, water, graph):
+    """SEP_BRUTE water with the pair and Coulomb calls at one cutoff and the
+    bonded calls after them on the second stream: the bonded kernels must
+    wait for the Coulomb call's rebuild, which permutes the atoms, so the
+    two-stream run equals the one-stream run bit for bit."""
+    kw = dict(cf=WAT.RC_COUL, dt=WAT.DT, nneighb=WAT.NNEIGHB)
+
+    def run(concurrent):
+        s = SepSystem.from_workload(product, water, update=S.SEP_BRUTE, **kw)
+        try:
+            s.gpu_resident(True)
+            s.gpu_run(_water_brute_equal_cutoffs(s), 6, graph=graph, concurrent=concurrent)
+            return _final(s)
+        finally:
+            s.close()
+
+    one, two = run(False), run(True)
+    assert np.abs(one["f"]).max() > 0
+    _assert_same(one, two)

@@ -95,3 +95,47 @@ def test_record_matches_synchronised_steps(product):
         s.check(lib.sep_gpu_record_steps(s.atoms, 0))
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("name", ["g1_lj_nve.npz", "g4_lj_nvt.npz"])
+def test_record_matches_reference_golden(product, name):
+    """The record of a resident graph run against the compiled reference's
+    per-step sepret values (golden G1: NVE, epot, ekin, pot_P, kin_P and
+    tnow; G4: NVT, epot and ekin) over the first 60 steps, before chaotic
+    growth of summation-order differences could reach the bar of 1e-10
+    relative to each column's scale."""
+    import os
+    from tests.golden import runs as R
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", name), allow_pickle=False)
+    n = 60
+    s = SepSystem.from_workload(product, R.workload_from(g), **KW)
+    lib = s._lib()
+    try:
+        s.gpu_resident(True)
+        ops = [s.op_pairs("AA", 2.5)]
+        if name.startswith("g4"):
+            ops.append(s.op_nosehoover(0.728, 10.0))
+        ops.append(s.op_leapfrog())
+        s.check(lib.sep_gpu_record_steps(s.atoms, 64))
+        s.gpu_run(ops, n, graph=True)
+        rets, tnow, lost = _read(s, 64)
+        assert len(rets) == n and lost == 0
+        epot = np.array([r.epot for r in rets])
+        ekin = np.array([r.ekin for r in rets])
+
+        def close(a, b):
+            return np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
+
+        if name.startswith("g1"):
+            sc = g["scal"][:n]
+            assert close(epot, sc[:, 0]) and close(ekin, sc[:, 1])
+            potP = np.array([S.mat(r.pot_P).ravel() for r in rets])
+            kinP = np.array([S.mat(r.kin_P).ravel() for r in rets])
+            assert close(potP, sc[:, 3:12]) and close(kinP, sc[:, 12:21])
+            assert np.array_equal(tnow, sc[:, 21])
+        else:
+            ref = g["alpha_ekin_epot"][:n]
+            assert close(ekin, ref[:, 1]) and close(epot, ref[:, 2])
+        s.check(lib.sep_gpu_record_steps(s.atoms, 0))
+    finally:
+        s.close()
