@@ -136,6 +136,21 @@ int sep_gpu_lj_adjoint(seppart *atoms, sepsys *sys, int ntypes, const char *type
 		       const double *params, const double *dLdF, double *dLdx,
 		       double *dLdtheta);
 
+/* sep_gpu_lj_adjoint without any device or host allocation, for callers that
+ * cannot allocate (a fitting loop in a process that forbids allocation after
+ * its set-up, code that must not reach hipMalloc).  No counterpart in the
+ * reference.  Same arguments and results, bit for bit.  Its device buffers
+ * (g, dL/dx, the parameter-gradient rows and sums: about 50 bytes per atom)
+ * are a fixed 32 MiB array that each GPU the library runs on holds until the
+ * process ends, so it serves systems of up to about 670000 atoms and refuses
+ * larger ones with SEP_GPU_EARG.  The atoms must already have device state
+ * (any force call on them creates it; otherwise SEP_GPU_ESTATE).  All calls
+ * on one GPU share the array: calls that overlap on one GPU (from several
+ * host threads) race. */
+int sep_gpu_lj_adjoint_noalloc(seppart *atoms, sepsys *sys, int ntypes, const char *types,
+			       const double *params, const double *dLdF, double *dLdx,
+			       double *dLdtheta);
+
 /* ---- per-step record --------------------------------------------------- */
 /* No counterpart in the reference, whose samplers read sepret on the host
  * after every step (the stress autocorrelation of source/sepsampler.c:

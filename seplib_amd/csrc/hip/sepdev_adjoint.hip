@@ -309,6 +309,19 @@ extern "C" int sepdev_lj_table(sepdev *d, const sepdev_ljtab_args *a, int ret_sl
 	return sepk_queue_task(d, f);
 }
 
+// dldtheta[(i * nt + k) * 2 + {0, 1}] = dL/d eps, dL/d sigma of pair ik from
+// the pair sums, written for both orders of a pair
+static void unpack_theta(const sepdev_ljtab_args *a, const double *full, double *dldtheta)
+{
+	for (int i = 0; i < a->ntypes; i++)
+		for (int k = 0; k < a->ntypes; k++) {
+			const int lo = i < k ? i : k, hi = i < k ? k : i;
+			const int idx = lo * a->ntypes - lo * (lo - 1) / 2 + (hi - lo);
+			dldtheta[(i * a->ntypes + k) * 2] = full[2 * idx];
+			dldtheta[(i * a->ntypes + k) * 2 + 1] = full[2 * idx + 1];
+		}
+}
+
 extern "C" int sepdev_lj_adjoint(sepdev *d, const sepdev_ljtab_args *a, const double *g3n,
 				 double *dldx3n, double *dldtheta)
 {
@@ -345,14 +358,89 @@ extern "C" int sepdev_lj_adjoint(sepdev *d, const sepdev_ljtab_args *a, const do
 	if ((rc = sepk_get_vec_from(d, dldx3n, d->adj_x)))
 		return rc;
 	SEPK_CHECK(hipStreamSynchronize(d->st));
-	// dldtheta[(i * nt + k) * 2 + {0, 1}] = dL/d eps, dL/d sigma of pair ik,
-	// written for both orders of a pair
-	for (int i = 0; i < a->ntypes; i++)
-		for (int k = 0; k < a->ntypes; k++) {
-			const int lo = i < k ? i : k, hi = i < k ? k : i;
-			const int idx = lo * a->ntypes - lo * (lo - 1) / 2 + (hi - lo);
-			dldtheta[(i * a->ntypes + k) * 2] = full[2 * idx];
-			dldtheta[(i * a->ntypes + k) * 2 + 1] = full[2 * idx + 1];
-		}
+	unpack_theta(a, full, dldtheta);
+	return 0;
+}
+
+// ---- the adjoint without allocation ----------------------------------------
+//
+// The device buffers of sepdev_lj_adjoint live in this array instead of
+// memory allocated on first use.  Each GPU the library runs on holds its own
+// copy (SEPDEV_SCRATCH_BYTES) until the process ends.  Host code never uses
+// the array's name, only the device address hipGetSymbolAddress returns for
+// it on each device (scratch_dev).  Every call on a GPU uses the same array:
+// nothing in it is read before the call has written it, and calls that
+// overlap on one GPU race.
+static __device__ __align__(256) double scratch[SEPDEV_SCRATCH_BYTES / sizeof(double)];
+
+#define SEPK_MAX_DEVICES 64
+static double *scratch_dev[SEPK_MAX_DEVICES];
+
+// The device address of scratch on the device of d, looked up once per
+// device with that device current.
+static int scratch_for(sepdev *d, double **out)
+{
+	if (d->device < 0 || d->device >= SEPK_MAX_DEVICES)
+		return sepk_fail(SEP_GPU_EARG, "device %d beyond the scratch table", d->device);
+	if (!scratch_dev[d->device]) {
+		int cur = 0;
+		SEPK_CHECK(hipGetDevice(&cur));
+		if (cur != d->device)
+			SEPK_CHECK(hipSetDevice(d->device));
+		void *p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (cur != d->device)
+			(void)hipSetDevice(cur);
+		if (e != hipSuccess || !p)
+			return sepk_fail(SEP_GPU_EDEVICE, "hipGetSymbolAddress(scratch): %s",
+					 hipGetErrorString(e));
+		scratch_dev[d->device] = (double *)p;
+	}
+	*out = scratch_dev[d->device];
+	return 0;
+}
+
+extern "C" int sepdev_lj_adjoint_noalloc(sepdev *d, const sepdev_ljtab_args *a,
+					 const double *g3n, double *dldx3n, double *dldtheta)
+{
+	if (!d->list_built)
+		return sepk_fail(SEP_GPU_ESTATE, "adjoint before a neighbour list was built");
+	if (d->capturing)
+		return sepk_fail(SEP_GPU_ESTATE, "adjoint during a graph capture");
+	sepk_ljtab c;
+	int rc = fill_table(a, c);
+	if (rc)
+		return rc;
+	// layout (doubles): g (3 npad), dL/dx (3 npad), the pair sums (2 NPAIR,
+	// padded to 32), the partial rows (one per block, 2 NPAIR columns)
+	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
+	const size_t vec = (size_t)3 * d->npad, sums = 32;
+	const size_t need = 2 * vec + sums + (size_t)nb * 2 * SEPK_NPAIR;
+	const size_t have = SEPDEV_SCRATCH_BYTES / sizeof(double);
+	if (need > have)
+		return sepk_fail(SEP_GPU_EARG,
+				 "the no-allocation adjoint needs %zu bytes for %ld atoms; its scratch "
+				 "holds %ld bytes (about %ld atoms)",
+				 need * sizeof(double), d->n, (long)SEPDEV_SCRATCH_BYTES,
+				 (long)(have / (6.0 + 2.0 * SEPK_NPAIR * SEPK_SUB / SEPK_BLOCK)) - 1024);
+	double *base = nullptr;
+	if ((rc = scratch_for(d, &base)) || (rc = sepk_flush_nh(d)) || (rc = sepk_flush_tasks(d)))
+		return rc;
+	double *g = base, *dldx = base + vec, *out = base + 2 * vec, *part = out + sums;
+	if ((rc = sepk_put_vec_to(d, g, g3n)))
+		return rc;
+	k_lj_adjoint<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->nbr, d->nbr_count, d->n, d->npad,
+						   c, g, dldx, part, (long)nb);
+	SEPK_LAUNCH_CHECK();
+	k_colsum<<<1, 256, 0, d->st>>>(part, (long)nb, 2 * SEPK_NPAIR, (int)nb, out);
+	SEPK_LAUNCH_CHECK();
+	// the sums go out through the pinned staging buffer after dL/dx
+	if ((rc = sepk_get_vec_from(d, dldx3n, dldx)))
+		return rc;
+	double full[2 * SEPK_NPAIR];
+	SEPK_CHECK(hipMemcpyAsync(d->hstage, out, sizeof full, hipMemcpyDeviceToHost, d->st));
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	memcpy(full, d->hstage, sizeof full);
+	unpack_theta(a, full, dldtheta);
 	return 0;
 }

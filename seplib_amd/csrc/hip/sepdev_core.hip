@@ -117,6 +117,8 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 	}
 	sepdev *d = (sepdev *)calloc(1, sizeof(sepdev));
 	d->n = npart;
+	if (hipGetDevice(&d->device) != hipSuccess)
+		d->device = 0;
 	// at least one slot past the last atom: the phantom that pads lists
 	d->npad = (npart + 1 + 63) / 64 * 64;
 	d->qpos_stale = 1;

@@ -238,6 +238,8 @@ struct sepdev {
 	hipGraph_t graph;
 	hipGraphExec_t exec;
 	int capturing;
+
+	int device; // the HIP device current at sepdev_create
 };
 
 // error recording (sepdev.hip)

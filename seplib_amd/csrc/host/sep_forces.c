@@ -518,6 +518,34 @@ int sep_gpu_lj_adjoint(seppart *ptr, sepsys *sys, int ntypes, const char *types,
 	return seph_dev_fail(sepdev_lj_adjoint(c->dev, &ta, dLdF, dLdx, dLdtheta));
 }
 
+/* The same adjoint without any allocation: the device context must exist
+ * (a force call made it), and the device buffers are the library's fixed
+ * scratch array instead of memory allocated on the first call. */
+int sep_gpu_lj_adjoint_noalloc(seppart *ptr, sepsys *sys, int ntypes, const char *types,
+			       const double *params, const double *dLdF, double *dLdx,
+			       double *dLdtheta)
+{
+	seph_ctx *c = seph_find(ptr);
+	if (!c || !c->dev)
+		return seph_fail(SEP_GPU_ESTATE,
+				 "sep_gpu_lj_adjoint_noalloc: no device state for these atoms "
+				 "(a force call creates it)");
+	if (sys->npart != c->n)
+		return seph_fail(SEP_GPU_EARG, "sys->npart (%d) differs from the %ld atoms",
+				 sys->npart, c->n);
+	int rc;
+	if (!c->resident && ((rc = seph_put_x(c)) || (rc = seph_put_scalars(c))))
+		return rc;
+	sepdev_ljtab_args ta;
+	memset(&ta, 0, sizeof ta);
+	ta.ntypes = ntypes;
+	ta.types = types;
+	ta.params = params;
+	for (int k = 0; k < 3; k++)
+		ta.L[k] = sys->length[k];
+	return seph_dev_fail(sepdev_lj_adjoint_noalloc(c->dev, &ta, dLdF, dLdx, dLdtheta));
+}
+
 /* ---- OpenMP Model II: source/sepomp.c:16-328 ---------------------------- */
 
 /* A Model II term runs through the synchronised path of its sep_*

@@ -167,6 +167,13 @@ int sepdev_lj_table(sepdev *d, const sepdev_ljtab_args *a, int ret_slot);
  * dldtheta[(i * ntypes + k) * 2 + {0,1}] = dL/d eps, dL/d sigma */
 int sepdev_lj_adjoint(sepdev *d, const sepdev_ljtab_args *a, const double *g3n,
 		      double *dldx3n, double *dldtheta);
+/* The same with its device buffers (g, dL/dx, the parameter-gradient rows
+ * and sums) in a fixed device array of SEPDEV_SCRATCH_BYTES per GPU instead
+ * of memory allocated on first use: no allocation at all.  Calls that
+ * overlap on one GPU share the array and race. */
+#define SEPDEV_SCRATCH_BYTES (32L << 20)
+int sepdev_lj_adjoint_noalloc(sepdev *d, const sepdev_ljtab_args *a, const double *g3n,
+			      double *dldx3n, double *dldtheta);
 
 /* bonded terms; lists hold ORIGINAL atom indices as in sepmolinfo */
 int sepdev_put_topology(sepdev *d, const unsigned *blist, int nb,

@@ -225,6 +225,8 @@ def load_product(path=PRODUCT_LIB):
     f("sep_gpu_force_lj_table", C.c_int, P_ATOMS, C.c_int, C.c_char_p, P_D, P_SYS, P_RET,
       C.c_uint)
     f("sep_gpu_lj_adjoint", C.c_int, P_ATOMS, P_SYS, C.c_int, C.c_char_p, P_D, P_D, P_D, P_D)
+    f("sep_gpu_lj_adjoint_noalloc", C.c_int, P_ATOMS, P_SYS, C.c_int, C.c_char_p, P_D, P_D,
+      P_D, P_D)
     f("sep_gpu_synchronize", C.c_int, P_ATOMS)
     f("sep_gpu_list_limit", C.c_int, P_ATOMS, P_SYS, C.c_int)
     f("sep_gpu_record_steps", C.c_int, P_ATOMS, C.c_long)
