@@ -197,6 +197,70 @@ __device__ __forceinline__ bool sepk_inside_mixed(const sepk_state &s, long t, u
 	return dx * dx + dy * dy + dz * dz < c.cf2;
 }
 
+// One pair of the mixed LJ loop: the FP32 force and energy terms from the
+// quantized squared distance r2 (dx, dy, dz in FP64), summed in FP64, and
+// the crossing stress when the integer difference wrapped.
+__device__ __forceinline__ void sepk_lj_mixed_add(
+	const sepk_lj &c, unsigned ux, unsigned vx, unsigned uy, unsigned vy, unsigned uz,
+	unsigned vz, int ix, int iy, int iz, double dx, double dy, double dz, double r2,
+	double &fx, double &fy, double &fz, double (&acc)[7])
+{
+	const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
+	const float q3 = q * q * q;
+	const double ft = (double)(fmaf(c.Af, q3, c.nBf) * q3 * q);
+	const double e = (double)fmaf(fmaf(c.Cf, q3, c.nDf), q3, c.ushiftf);
+	fx = fma(ft, dx, fx);
+	fy = fma(ft, dy, fy);
+	fz = fma(ft, dz, fz);
+	acc[0] += e;
+	const bool bx = (ux < vx) != (ix < 0), by = (uy < vy) != (iy < 0),
+		   bz = (uz < vz) != (iz < 0);
+	if (bx | by | bz) {
+		// k = +1 when the unwrapped difference is >= 2^31 steps (wrapped
+		// negative), -1 when it is below -2^31 (wrapped non-negative)
+		const double kx = bx ? (ix < 0 ? 1.0 : -1.0) : 0.0;
+		const double ky = by ? (iy < 0 ? 1.0 : -1.0) : 0.0;
+		const double kz = bz ? (iz < 0 ? 1.0 : -1.0) : 0.0;
+		sepk_pair_cross(ft, dx, dy, dz, kx, ky, kz, c.L0, c.L1, c.L2, acc);
+	}
+}
+
+// The pairs of the mixed loop that lay within c.band of the cutoff (the
+// loop left them out and flagged the lane): decided on the FP64 records as
+// the plain kernel decides them (sepk_inside_mixed), and added when inside.
+// A lane gets here for about one pair in 10^8.
+template <bool TYPED, bool FAST>
+__device__ __forceinline__ void sepk_lj_mixed_band(
+	const sepk_state &s, const int *__restrict__ nbr, const int *__restrict__ cnt, long npad,
+	const sepk_lj &c, long t, int sub, signed char ti, int4 qi, double &fx, double &fy,
+	double &fz, double (&acc)[7])
+{
+	const int n = cnt[t];
+	for (int e = sub; e < n; e += SEPK_SUB) {
+		const unsigned j = (unsigned)nbr[sepk_entry(e, t, npad)];
+		const int4 qj = s.qpos[j];
+		if (TYPED) {
+			const signed char tj = s.type[j];
+			if (!((ti == c.t0 && tj == c.t1) || (ti == c.t1 && tj == c.t0)))
+				continue;
+		}
+		if (qj.w == SEPK_QPHANTOM)
+			continue;
+		const unsigned ux = (unsigned)qi.x, vx = (unsigned)qj.x;
+		const unsigned uy = (unsigned)qi.y, vy = (unsigned)qj.y;
+		const unsigned uz = (unsigned)qi.z, vz = (unsigned)qj.z;
+		const int ix = (int)(ux - vx), iy = (int)(uy - vy), iz = (int)(uz - vz);
+		const double dx = (double)ix * c.qd0;
+		const double dy = (double)iy * c.qd1;
+		const double dz = (double)iz * c.qd2;
+		const double r2 = dx * dx + dy * dy + dz * dz;
+		if (r2 < c.cf2lo || !(r2 < c.cf2hi) || !sepk_inside_mixed<FAST>(s, t, j, r2, c))
+			continue;
+		sepk_lj_mixed_add(c, ux, vx, uy, vy, uz, vz, ix, iy, iz, dx, dy, dz, r2, fx, fy, fz,
+				  acc);
+	}
+}
+
 // The pair loop of atom t, lane `sub`: the lane's share of the atom's force
 // and of the block's energy and crossing stress, and the position the
 // stress identity pairs the summed force with (px, py, pz).
@@ -215,6 +279,7 @@ __device__ __forceinline__ void sepk_lj_pairs(
 		int4 qi = make_int4(0, 0, 0, SEPK_QPHANTOM);
 		if (act) {
 			qi = s.qpos[t];
+			bool band = false;
 			const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 			const size_t stride = (size_t)npad * SEPK_SUB;
 			const int4 *lp = sepk_lane_list(nbr, t, sub);
@@ -245,30 +310,18 @@ __device__ __forceinline__ void sepk_lj_pairs(
 					const double dy = (double)iy * c.qd1;
 					const double dz = (double)iz * c.qd2;
 					const double r2 = dx * dx + dy * dy + dz * dz;
-					if (!sepk_inside_mixed<FAST>(s, t, (unsigned)j[u], r2, c))
+					if (!(r2 < c.cf2lo)) {
+						// near the cutoff: decided after the loop
+						band |= r2 < c.cf2hi;
 						continue;
-					const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
-					const float q3 = q * q * q;
-					const double ft = (double)(fmaf(c.Af, q3, c.nBf) * q3 * q);
-					const double e = (double)fmaf(fmaf(c.Cf, q3, c.nDf), q3, c.ushiftf);
-					fx = fma(ft, dx, fx);
-					fy = fma(ft, dy, fy);
-					fz = fma(ft, dz, fz);
-					acc[0] += e;
-					const bool bx = (ux < vx) != (ix < 0), by = (uy < vy) != (iy < 0),
-						   bz = (uz < vz) != (iz < 0);
-					if (bx | by | bz) {
-						// k = +1 when the unwrapped difference is >= 2^31
-						// steps (wrapped negative), -1 when it is below
-						// -2^31 (wrapped non-negative)
-						const double kx = bx ? (ix < 0 ? 1.0 : -1.0) : 0.0;
-						const double ky = by ? (iy < 0 ? 1.0 : -1.0) : 0.0;
-						const double kz = bz ? (iz < 0 ? 1.0 : -1.0) : 0.0;
-						sepk_pair_cross(ft, dx, dy, dz, kx, ky, kz, c.L0, c.L1, c.L2,
-								acc);
 					}
+					sepk_lj_mixed_add(c, ux, vx, uy, vy, uz, vz, ix, iy, iz, dx, dy, dz,
+							  r2, fx, fy, fz, acc);
 				}
 			}
+			if (band)
+				sepk_lj_mixed_band<TYPED, FAST>(s, nbr, cnt, npad, c, t, sub, ti, qi, fx,
+								fy, fz, acc);
 		}
 		px = act ? (double)(unsigned)qi.x * c.qd0 : 0.0;
 		py = act ? (double)(unsigned)qi.y * c.qd1 : 0.0;
@@ -472,6 +525,8 @@ extern "C" int sepdev_lj(sepdev *d, const sepdev_lj_args *a, int ret_slot)
 		// |dx~ - dx| <= one step per axis, so |r2~ - r2| <= q (2 sqrt(3) rc + 3 q)
 		const double q = fmax(c.qd0, fmax(c.qd1, c.qd2));
 		c.band = q * (4.0 * sqrt(a->cf2) + 4.0 * q) + 1e-12 * a->cf2;
+		c.cf2lo = a->cf2 - c.band;
+		c.cf2hi = a->cf2 + c.band;
 	}
 	c.Af = (float)c.A;
 	c.nBf = (float)c.nB;
@@ -766,6 +821,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 		const size_t stride = (size_t)npad * SEPK_SUB;
 		const int4 *lp = sepk_lane_list(nbr, t, sub);
+		bool band = false;
 		for (int b = 0; b < nbat; b++, lp += stride) {
 			const int4 jb = *lp;
 			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
@@ -792,11 +848,14 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 				const double kx = bx ? (ix < 0 ? 1.0 : -1.0) : 0.0;
 				const double ky = by ? (iy < 0 ? 1.0 : -1.0) : 0.0;
 				const double kz = bz ? (iz < 0 ? 1.0 : -1.0) : 0.0;
-				// the LJ term as in sepk_lj_pairs
+				// the LJ term as in sepk_lj_pairs, pairs near the cutoff
+				// left to sepk_lj_mixed_band after the loop
 				const bool tl = !TYPED || (ti == c.t0 && tj[u] == c.t1) ||
 						(ti == c.t1 && tj[u] == c.t0);
-				if (act_l && tl && qj[u].w != SEPK_QPHANTOM &&
-				    sepk_inside_mixed<FAST>(s, t, (unsigned)j[u], r2, c)) {
+				const bool lj = act_l && tl && qj[u].w != SEPK_QPHANTOM;
+				if (lj && !(r2 < c.cf2lo))
+					band |= r2 < c.cf2hi;
+				if (lj && r2 < c.cf2lo) {
 					const float q = c.sig2f * __builtin_amdgcn_rcpf((float)r2);
 					const float q3 = q * q * q;
 					const double ft = (double)(fmaf(c.Af, q3, c.nBf) * q3 * q);
@@ -828,6 +887,9 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 				}
 			}
 		}
+		if (band)
+			sepk_lj_mixed_band<TYPED, FAST>(s, nbr, cnt, npad, c, t, sub, ti, qi, fl[0],
+							fl[1], fl[2], al);
 	}
 	// the two epilogues of sepk_pair_epilogue, the second adding to the first
 #pragma unroll
@@ -1143,22 +1205,14 @@ __device__ __forceinline__ void sepk_lf_atom(const sepk_state &s, const sepk_lfo
 // sepk_lf_atom spread over lanes 0..2 of the atom's SEPK_SUB lanes, one
 // axis each; lane 0 gathers the other axes' values by shuffles and forms
 // the sums in sepk_lf_atom's order (the same bits).  Every lane of the wave
-// calls it.  frozen (the device error word is set): the atom stays where it
-// is, its position records copied to o unchanged.
+// calls it.
 __device__ __forceinline__ void sepk_lf_atom_lanes(const sepk_state &s, const sepk_lfout &o,
 						   long t, long npad, int sub, bool act, double F0,
 						   double F1, double F2, double dt, double L0,
 						   double L1, double L2, int nh, double al,
 						   int pending, double qs0, double qs1, double qs2,
-						   double (&acc)[9], bool frozen)
+						   double (&acc)[9])
 {
-	if (frozen) {
-		if (act && sub == 0) {
-			o.pos[t] = s.pos[t];
-			o.qpos[t] = s.qpos[t];
-		}
-		return;
-	}
 	double x = 0.0, v = 0.0, vh = 0.0, dr = 0.0, m = 0.0, q = 0.0;
 	if (act && sub < 3) {
 		const int k = sub;
@@ -1264,9 +1318,18 @@ __global__ __launch_bounds__(SEPK_BLOCK) __attribute__((amdgpu_waves_per_eu(MIXE
 	const long lb = sepk_xcd_block();
 	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
 	const int sub = threadIdx.x % SEPK_SUB;
+	const bool act = t < n;
+	if (sys->error) {
+		// a step after a device error changes nothing: the positions go
+		// through to the other buffer unchanged (the whole grid returns)
+		if (act && sub == 0) {
+			o.pos[t] = s.pos[t];
+			o.qpos[t] = s.qpos[t];
+		}
+		return;
+	}
 	double fx = 0.0, fy = 0.0, fz = 0.0;
 	double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-	const bool act = t < n;
 	double px = 0.0, py = 0.0, pz = 0.0;
 	sepk_lj_pairs<false, MIXED, FAST>(s, nbr, cnt, npad, c, t, sub, act, 0, fx, fy, fz, acc,
 					   px, py, pz);
@@ -1278,7 +1341,6 @@ __global__ __launch_bounds__(SEPK_BLOCK) __attribute__((amdgpu_waves_per_eu(MIXE
 	}
 	double lf[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 	const int pending = sys->xn_pending;
-	const bool frozen = sys->error != 0;
 	double al = 0.0;
 	if (nh)
 		al = sepk_nh_alpha(*alpha, sys->sumv2, (double)n, Td, dt_tau2);
@@ -1293,7 +1355,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) __attribute__((amdgpu_waves_per_eu(MIXE
 		acc[6] = fma(gz, pz, acc[6]);
 	}
 	sepk_lf_atom_lanes(s, o, t, npad, sub, act, fx, fy, fz, dt, c.L0, c.L1, c.L2, nh, al,
-			   pending, qs0, qs1, qs2, lf, frozen);
+			   pending, qs0, qs1, qs2, lf);
 	sepk_block_partial<7>(acc, 0u, ppart, lb, rows);
 	sepk_block_partial<9>(lf, 1u << 7, lpart, lb, rows);
 }

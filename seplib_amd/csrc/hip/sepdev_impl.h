@@ -58,8 +58,9 @@ struct sepk_lj {
 	double cf2, A, nB, C, nD, sig2, ushift;
 	double L0, L1, L2, iL0, iL1, iL2;
 	double qd0, qd1, qd2; // L / 2^32: one step of the quantized positions
-	// the quantized r^2 is within band of the FP64 one (mixed kernels)
-	double band;
+	// the quantized r^2 is within band of the FP64 one (mixed kernels);
+	// cf2lo = cf2 - band, cf2hi = cf2 + band
+	double band, cf2lo, cf2hi;
 	float Af, nBf, Cf, nDf, sig2f, ushiftf;
 	signed char t0, t1;
 };
