@@ -117,6 +117,25 @@ int sep_gpu_run(seppart *atoms, sepsys *sys, sepret *ret,
  * use until the next build.  No counterpart in the reference. */
 int sep_gpu_list_limit(seppart *atoms, sepsys *sys, int entries);
 
+/* ---- device checkpoint ------------------------------------------------- */
+/* Writes the device state of a resident system to `file`: positions (also in
+ * their quantized form), velocities, forces, accelerations, xn,
+ * cross_neighb and crossings, masses, types, molecules and bonded partners,
+ * all in the device's own atom order together with that order; the current
+ * neighbour list; the sepsys counters (tnow, neighb_flag, nupdate_neighb,
+ * max_dist2) and the thermostat's running sum; and, if given, the sepret
+ * and Nose-Hoover alpha the run uses.  No host copy of the atoms is made.
+ * sep_gpu_restore reads it into a resident system of the same size (which
+ * may be another system, in another process) and the run continues exactly
+ * as the one that wrote it, bit for bit; it also sets the sepsys counters,
+ * *ret and *alpha on the host.  The reference's own checkpoint is sep_save,
+ * the raw AoS records (source/sepmisc.c:320-330); this one has no
+ * counterpart there. */
+int sep_gpu_checkpoint(seppart *atoms, sepsys *sys, const sepret *ret, const double *alpha,
+		       const char *file);
+int sep_gpu_restore(seppart *atoms, sepsys *sys, sepret *ret, double *alpha,
+		    const char *file);
+
 /* ---- multi-type LJ and force matching --------------------------------- */
 /* The pair forces of sep_force_lj (include/sepprfrc.h:69, source/
  * sepprfrc.c:743-923) for every pair of the ntypes labels in `types` in one

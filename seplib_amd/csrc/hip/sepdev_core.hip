@@ -792,3 +792,161 @@ extern "C" int sepdev_graph_drop(sepdev *d)
 	}
 	return 0;
 }
+
+// ---- device checkpoint ---------------------------------------------------
+//
+// The state a resident run carries from one step to the next, as it stands
+// between steps: every per-atom array in device order (with the order
+// itself), the neighbour list, the system scalars and the host-side facts
+// the next step's kernels depend on (list geometry, whether sum m v^2 of
+// the current velocities is known, the box of the quantized positions).
+// Read back into a device of the same size, the run continues bit for bit.
+
+#define SEPK_CKPT_MAGIC "SEPGPUC1"
+
+struct sepk_ckpt_head {
+	char magic[8];
+	long n, npad;
+	int lcap, cap, cap_limit, list_built;
+	int has_partners, sumv2_valid, qpos_stale, pad_;
+	double qL[3];
+	sepdev_grid grid;
+	sepdev_sys sys;
+};
+
+static int ckpt_io(FILE *f, void *host, size_t bytes, int writing)
+{
+	const size_t done = writing ? fwrite(host, 1, bytes, f) : fread(host, 1, bytes, f);
+	if (done != bytes)
+		return sepk_fail(SEP_GPU_EARG, "checkpoint: short %s (%zu of %zu bytes)",
+				 writing ? "write" : "read", done, bytes);
+	return 0;
+}
+
+// One device array through a host buffer, in either direction.
+static int ckpt_array(sepdev *d, FILE *f, void *dev, size_t bytes, void *buf, int writing)
+{
+	int rc;
+	if (writing) {
+		SEPK_CHECK(hipMemcpy(buf, dev, bytes, hipMemcpyDeviceToHost));
+		return ckpt_io(f, buf, bytes, 1);
+	}
+	if ((rc = ckpt_io(f, buf, bytes, 0)))
+		return rc;
+	SEPK_CHECK(hipMemcpy(dev, buf, bytes, hipMemcpyHostToDevice));
+	return 0;
+}
+
+static int ckpt_arrays(sepdev *d, FILE *f, const sepk_ckpt_head &h, int writing)
+{
+	const size_t np = (size_t)d->npad;
+	const size_t list_ints = (size_t)h.lcap * SEPK_SUB * np;
+	size_t most = np * 40 * sizeof(int);
+	if (h.list_built && list_ints * sizeof(int) > most)
+		most = list_ints * sizeof(int);
+	void *buf = malloc(most);
+	if (!buf)
+		return sepk_fail(SEP_GPU_EHOST, "checkpoint: out of host memory");
+	sepk_state &s = d->s[0];
+	struct {
+		void *p;
+		size_t bytes;
+	} parts[] = {
+		{s.pos, np * sizeof(sepk_pos)}, {s.qpos, np * sizeof(int4)},
+		{s.v, 3 * np * sizeof(double)}, {s.f, 3 * np * sizeof(double)},
+		{s.a, 3 * np * sizeof(double)}, {s.xn, 3 * np * sizeof(double)},
+		{s.cn, 3 * np * sizeof(int)},   {s.cr, 3 * np * sizeof(int)},
+		{s.m, np * sizeof(double)},     {s.type, np},
+		{s.mol, np * sizeof(int)},      {s.orig, np * sizeof(int)},
+		{d->iperm, np * sizeof(int)},   {d->nbr_count, np * sizeof(int)},
+	};
+	int rc = 0;
+	for (auto &p : parts)
+		if ((rc = ckpt_array(d, f, p.p, p.bytes, buf, writing)))
+			break;
+	if (!rc && h.has_partners)
+		rc = ckpt_array(d, f, s.partners, np * 40 * sizeof(int), buf, writing);
+	if (!rc && h.list_built)
+		rc = ckpt_array(d, f, d->nbr, list_ints * sizeof(int), buf, writing);
+	free(buf);
+	return rc;
+}
+
+extern "C" int sepdev_checkpoint_write(sepdev *d, void *file)
+{
+	FILE *f = (FILE *)file;
+	if (d->capturing)
+		return sepk_fail(SEP_GPU_ESTATE, "checkpoint during a graph capture");
+	// the step's deferred work is done: the recorded calls, the finisher
+	int rc = sepk_flush_pair(d);
+	if (rc || (rc = sepk_flush_nh(d)) || (rc = sepk_materialize_forces(d)) ||
+	    (rc = sepk_flush_tasks(d)))
+		return rc;
+	sepk_ckpt_head h;
+	memset(&h, 0, sizeof h);
+	memcpy(h.magic, SEPK_CKPT_MAGIC, 8);
+	h.n = d->n;
+	h.npad = d->npad;
+	h.list_built = d->list_built && d->nbr;
+	h.lcap = h.list_built ? d->lcap : 0;
+	h.cap = h.list_built ? d->cap : 0;
+	h.cap_limit = d->cap_limit;
+	h.has_partners = d->s[0].partners != nullptr;
+	h.sumv2_valid = d->sumv2_valid;
+	h.qpos_stale = d->qpos_stale;
+	for (int k = 0; k < 3; k++)
+		h.qL[k] = d->qL[k];
+	h.grid = d->grid;
+	if ((rc = sepdev_get_sys(d, &h.sys)))
+		return rc;
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	if ((rc = ckpt_io(f, &h, sizeof h, 1)))
+		return rc;
+	return ckpt_arrays(d, f, h, 1);
+}
+
+extern "C" int sepdev_checkpoint_read(sepdev *d, void *file)
+{
+	FILE *f = (FILE *)file;
+	if (d->capturing)
+		return sepk_fail(SEP_GPU_ESTATE, "restore during a graph capture");
+	sepk_ckpt_head h;
+	int rc = ckpt_io(f, &h, sizeof h, 0);
+	if (rc)
+		return rc;
+	if (memcmp(h.magic, SEPK_CKPT_MAGIC, 8) != 0)
+		return sepk_fail(SEP_GPU_EARG, "not a device checkpoint of this library");
+	if (h.n != d->n || h.npad != d->npad)
+		return sepk_fail(SEP_GPU_EARG, "checkpoint of %ld atoms, system of %ld", h.n, d->n);
+	if ((rc = sepk_flush_pair(d)) || (rc = sepk_flush_nh(d)) || (rc = sepk_flush_tasks(d)))
+		return rc;
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	if (h.list_built && (rc = sepdev_reserve_list(d, h.cap)))
+		return rc;
+	if (h.has_partners && !d->s[0].partners) {
+		for (int b = 0; b < 2; b++)
+			if ((rc = dalloc(&d->s[b].partners, (size_t)40 * d->npad)))
+				return rc;
+	}
+	if ((rc = ckpt_arrays(d, f, h, 0)))
+		return rc;
+	if ((rc = sepdev_put_sys(d, &h.sys)))
+		return rc;
+	SEPK_CHECK(hipStreamSynchronize(d->st));
+	// the host side of the device state
+	d->list_built = h.list_built;
+	if (h.list_built) {
+		d->cap = h.cap;
+		d->grid = h.grid;
+	}
+	d->cap_limit = h.cap_limit;
+	d->sumv2_valid = h.sumv2_valid;
+	d->qpos_stale = h.qpos_stale;
+	for (int k = 0; k < 3; k++)
+		d->qL[k] = h.qL[k];
+	d->f_zero_pending = 0;
+	d->max_reset_pending = 0;
+	d->nh_pending = 0;
+	d->fork_ready = 0;
+	return 0;
+}

@@ -531,7 +531,7 @@ int sep_gpu_lj_adjoint_noalloc(seppart *ptr, sepsys *sys, int ntypes, const char
 				 "sep_gpu_lj_adjoint_noalloc: no device state for these atoms "
 				 "(a force call creates it)");
 	if (sys->npart != c->n)
-		return seph_fail(SEP_GPU_EARG, "sys->npart (%d) differs from the %ld atoms",
+		return seph_fail(SEP_GPU_EARG, "sys->npart (%ld) differs from the %ld atoms",
 				 sys->npart, c->n);
 	int rc;
 	if (!c->resident && ((rc = seph_put_x(c)) || (rc = seph_put_scalars(c))))

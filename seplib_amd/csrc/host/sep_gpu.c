@@ -277,6 +277,96 @@ static int run_steps(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op 
 	return rc;
 }
 
+/* ---- device checkpoint ------------------------------------------------ */
+
+#define SEPH_CKPT_TAG "SEPGPUH1"
+
+int sep_gpu_checkpoint(seppart *atoms, sepsys *sys, const sepret *ret, const double *alpha,
+		       const char *file)
+{
+	seph_ctx *c = seph_ctx_for(atoms, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	if (!c->resident)
+		return seph_fail(SEP_GPU_ESTATE, "sep_gpu_checkpoint needs a resident system");
+	/* the values the run left in its sepret and thermostat variable */
+	sepret r;
+	memset(&r, 0, sizeof r);
+	double al = 0.0;
+	int rc = 0;
+	for (int b = 0; !rc && b < c->nrets; b++)
+		if (c->rets[b].host == ret) {
+			double w[SEPDEV_RET_WORDS];
+			rc = seph_dev_fail(sepdev_ret_get(c->dev, c->rets[b].slot, w));
+			seph_ret_unpack(&r, w);
+		}
+	if (!rc && ret && c->nrets == 0)
+		r = *ret;
+	int have_alpha = 0;
+	for (int b = 0; !rc && b < c->nalphas; b++)
+		if (c->alphas[b].host == alpha) {
+			rc = seph_dev_fail(sepdev_scalar_get(c->dev, c->alphas[b].slot, &al));
+			have_alpha = 1;
+		}
+	if (!rc && alpha && !have_alpha)
+		al = *alpha;
+	if (rc)
+		return rc;
+	FILE *f = fopen(file, "wb");
+	if (!f)
+		return seph_fail(SEP_GPU_EARG, "sep_gpu_checkpoint: cannot open %s", file);
+	int has[2] = {ret != NULL, alpha != NULL};
+	if (fwrite(SEPH_CKPT_TAG, 1, 8, f) != 8 || fwrite(has, sizeof has, 1, f) != 1 ||
+	    fwrite(&r, sizeof r, 1, f) != 1 || fwrite(&al, sizeof al, 1, f) != 1)
+		rc = seph_fail(SEP_GPU_EHOST, "sep_gpu_checkpoint: write failed");
+	if (!rc)
+		rc = seph_dev_fail(sepdev_checkpoint_write(c->dev, f));
+	if (fclose(f) != 0 && !rc)
+		rc = seph_fail(SEP_GPU_EHOST, "sep_gpu_checkpoint: write failed");
+	return rc;
+}
+
+int sep_gpu_restore(seppart *atoms, sepsys *sys, sepret *ret, double *alpha, const char *file)
+{
+	seph_ctx *c = seph_ctx_for(atoms, sys);
+	if (!c)
+		return sep_gpu_last_error();
+	if (!c->resident)
+		return seph_fail(SEP_GPU_ESTATE, "sep_gpu_restore needs a resident system");
+	FILE *f = fopen(file, "rb");
+	if (!f)
+		return seph_fail(SEP_GPU_EARG, "sep_gpu_restore: cannot open %s", file);
+	char tag[8];
+	int has[2];
+	sepret r;
+	double al;
+	int rc = 0;
+	if (fread(tag, 1, 8, f) != 8 || memcmp(tag, SEPH_CKPT_TAG, 8) != 0 ||
+	    fread(has, sizeof has, 1, f) != 1 || fread(&r, sizeof r, 1, f) != 1 ||
+	    fread(&al, sizeof al, 1, f) != 1)
+		rc = seph_fail(SEP_GPU_EARG, "sep_gpu_restore: %s is not a device checkpoint", file);
+	if (!rc)
+		rc = seph_dev_fail(sepdev_checkpoint_read(c->dev, f));
+	fclose(f);
+	if (rc)
+		return rc;
+	sepdev_sys ds;
+	if ((rc = seph_dev_fail(sepdev_get_sys(c->dev, &ds))))
+		return rc;
+	sys->tnow = ds.tnow;
+	sys->neighb_flag = ds.neighb_flag;
+	sys->nupdate_neighb = ds.nupdate_neighb;
+	sys->max_dist2 = ds.max_dist2;
+	if (ret && has[0])
+		*ret = r;
+	if (alpha && has[1])
+		*alpha = al;
+	/* bound values are taken from the host again at their next use */
+	c->nrets = 0;
+	c->nalphas = 0;
+	return 0;
+}
+
 /* ---- diagnostics ------------------------------------------------------ */
 
 int sep_gpu_neighbour_pairs(seppart *atoms, sepsys *sys, int *pairs, long cap,

@@ -206,6 +206,13 @@ int sepdev_concurrent_terms(sepdev *d, int on);
  * walk of a pair call followed by a Coulomb call. */
 int sepdev_fuse_pair_coulomb(sepdev *d, int on);
 
+/* ---- device checkpoint: the device state between steps (per-atom arrays
+ * in device order, the order, the neighbour list, the system scalars and
+ * the host-side facts the next step depends on) to or from an open FILE;
+ * reading needs a device of the same size and replaces its state */
+int sepdev_checkpoint_write(sepdev *d, void *file);
+int sepdev_checkpoint_read(sepdev *d, void *file);
+
 /* ---- misc */
 int sepdev_synchronize(sepdev *d);
 int sepdev_timing(sepdev *d, int on);

@@ -229,6 +229,8 @@ def load_product(path=PRODUCT_LIB):
       P_D, P_D)
     f("sep_gpu_synchronize", C.c_int, P_ATOMS)
     f("sep_gpu_list_limit", C.c_int, P_ATOMS, P_SYS, C.c_int)
+    f("sep_gpu_checkpoint", C.c_int, P_ATOMS, P_SYS, P_RET, P_D, C.c_char_p)
+    f("sep_gpu_restore", C.c_int, P_ATOMS, P_SYS, P_RET, P_D, C.c_char_p)
     f("sep_gpu_record_steps", C.c_int, P_ATOMS, C.c_long)
     f("sep_gpu_read_steps", C.c_int, P_ATOMS, C.c_void_p, P_D, C.c_long,
       C.POINTER(C.c_long), C.POINTER(C.c_long))

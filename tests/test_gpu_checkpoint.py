@@ -119,3 +119,71 @@ def test_resident_load_replaces_bonded_partners(product, tmp_path):
         src.close()
         b.close()
         c.close()
+
+
+def _snap(s):
+    s.gpu_sync()
+    st = s.state()
+    r = s.ret
+    st.update(epot=r.epot, ekin=r.ekin, potP=S.mat(r.pot_P), kinP=S.mat(r.kin_P))
+    return st
+
+
+@pytest.mark.parametrize("precision", [S.SEP_GPU_FP64, S.SEP_GPU_MIXED], ids=["fp64", "mixed"])
+def test_device_checkpoint_resumes_bit_for_bit(product, tmp_path, precision):
+    """sep_gpu_checkpoint after 37 graph steps, then 53 more: a second
+    system (its device state first taken elsewhere by 5 steps of its own)
+    restored from the file and run 53 steps ends bit for bit where the first
+    one does -- positions, velocities, forces, xn, crossings, the sepret,
+    the thermostat alpha and the sepsys counters.  Writing the checkpoint
+    leaves the host copy of the atoms alone (no AoS round trip)."""
+    import ctypes as C
+    w = W.lj_fcc(10, 0.8442, 1.0, seed=8)
+    path = str(tmp_path / "run.ckpt").encode()
+    lib = product.lib
+    kw = dict(cf=2.5, dt=0.005, skin=0.3)
+    a = SepSystem.from_workload(product, w, **kw)
+    b = SepSystem.from_workload(product, w, **kw)
+
+    def ops(s):
+        return [s.op_pairs("AA", 2.5), s.op_nosehoover(1.0, 10.0), s.op_leapfrog()]
+    try:
+        for s in (a, b):
+            s.gpu_precision(precision)
+            s.gpu_resident(True)
+        a.gpu_run(ops(a), 37, graph=True)
+        x_host = np.array(a.view["x"])
+        assert lib.sep_gpu_checkpoint(a.atoms, a.sysp, a.retp, C.byref(a.alpha), path) == 0
+        assert np.array_equal(np.array(a.view["x"]), x_host)
+        a.gpu_sync()
+        at_ckpt = (a.sys.tnow, a.sys.nupdate_neighb, a.alpha.value, a.ret.epot)
+        a.gpu_run(ops(a), 53, graph=True)
+        end_a = _snap(a)
+        b.gpu_run(ops(b), 5, graph=False)
+        assert lib.sep_gpu_restore(b.atoms, b.sysp, b.retp, C.byref(b.alpha), path) == 0
+        assert (b.sys.tnow, b.sys.nupdate_neighb, b.alpha.value, b.ret.epot) == at_ckpt
+        b.gpu_run(ops(b), 53, graph=True)
+        end_b = _snap(b)
+        assert end_a["nupdate_neighb"] > at_ckpt[1]  # rebuilds after the restore
+        for k in ("x", "v", "f", "a", "xn", "cross_neighb", "crossings", "potP", "kinP"):
+            assert np.array_equal(end_a[k], end_b[k]), k
+        for k in ("epot", "ekin", "alpha", "max_dist2", "nupdate_neighb", "neighb_flag", "tnow"):
+            assert end_a[k] == end_b[k], k
+        # a file that is not a checkpoint, and a system of another size
+        bad = str(tmp_path / "bad.ckpt").encode()
+        with open(bad, "wb") as f:
+            f.write(b"not a checkpoint")
+        lib.sep_gpu_clear_error()
+        assert lib.sep_gpu_restore(b.atoms, b.sysp, None, None, bad) == S.SEP_GPU_EARG
+        small = SepSystem.from_workload(product, W.lj_fcc(6, 0.8442, 1.0, seed=8), **kw)
+        try:
+            small.gpu_resident(True)
+            lib.sep_gpu_clear_error()
+            assert lib.sep_gpu_restore(small.atoms, small.sysp, None, None, path) == \
+                S.SEP_GPU_EARG
+        finally:
+            small.close()
+        lib.sep_gpu_clear_error()
+    finally:
+        a.close()
+        b.close()
