@@ -225,6 +225,31 @@ __device__ __forceinline__ void sepk_lj_mixed_add(
 	}
 }
 
+// One entry of a mixed LJ walk (untyped): the pair of records qi, qj,
+// added when inside c.cf2lo; one nearer the cutoff than that sets band and
+// is left to sepk_lj_mixed_band.
+__device__ __forceinline__ void sepk_lj_mixed_entry(const sepk_lj &c, const int4 &qi,
+						    const int4 &qj, double &fx, double &fy,
+						    double &fz, double (&acc)[7], bool &band)
+{
+	if (qj.w == SEPK_QPHANTOM)
+		return;
+	const unsigned ux = (unsigned)qi.x, vx = (unsigned)qj.x;
+	const unsigned uy = (unsigned)qi.y, vy = (unsigned)qj.y;
+	const unsigned uz = (unsigned)qi.z, vz = (unsigned)qj.z;
+	const int ix = (int)(ux - vx), iy = (int)(uy - vy), iz = (int)(uz - vz);
+	const double dx = (double)ix * c.qd0;
+	const double dy = (double)iy * c.qd1;
+	const double dz = (double)iz * c.qd2;
+	const double r2 = dx * dx + dy * dy + dz * dz;
+	if (!(r2 < c.cf2lo)) {
+		// near the cutoff: decided after the walk
+		band |= r2 < c.cf2hi;
+		return;
+	}
+	sepk_lj_mixed_add(c, ux, vx, uy, vy, uz, vz, ix, iy, iz, dx, dy, dz, r2, fx, fy, fz, acc);
+}
+
 // The pairs of the mixed loop that lay within c.band of the cutoff (the
 // loop left them out and flagged the lane): decided on the FP64 records as
 // the plain kernel decides them (sepk_inside_mixed), and added when inside.
@@ -299,24 +324,7 @@ __device__ __forceinline__ void sepk_lj_pairs(
 					if (TYPED && !((ti == c.t0 && tj[u] == c.t1) ||
 						       (ti == c.t1 && tj[u] == c.t0)))
 						continue;
-					if (qj[u].w == SEPK_QPHANTOM)
-						continue;
-					const unsigned ux = (unsigned)qi.x, vx = (unsigned)qj[u].x;
-					const unsigned uy = (unsigned)qi.y, vy = (unsigned)qj[u].y;
-					const unsigned uz = (unsigned)qi.z, vz = (unsigned)qj[u].z;
-					const int ix = (int)(ux - vx), iy = (int)(uy - vy),
-						  iz = (int)(uz - vz);
-					const double dx = (double)ix * c.qd0;
-					const double dy = (double)iy * c.qd1;
-					const double dz = (double)iz * c.qd2;
-					const double r2 = dx * dx + dy * dy + dz * dz;
-					if (!(r2 < c.cf2lo)) {
-						// near the cutoff: decided after the loop
-						band |= r2 < c.cf2hi;
-						continue;
-					}
-					sepk_lj_mixed_add(c, ux, vx, uy, vy, uz, vz, ix, iy, iz, dx, dy, dz,
-							  r2, fx, fy, fz, acc);
+					sepk_lj_mixed_entry(c, qi, qj[u], fx, fy, fz, acc, band);
 				}
 			}
 			if (band)
@@ -1051,21 +1059,24 @@ extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot
 // leapfrog computed it) the call is only recorded and the next leapfrog
 // applies f -= alpha m v itself; otherwise: sum kernel, finisher, apply.
 
+// Sum of m v^2 over the atoms, with the leapfrog's blocks (SEPK_SUB lanes
+// per atom, lane 0 working, XCD-remapped block index) and its order of
+// terms, so that it is the same bits as the sum a leapfrog leaves in
+// sys->sumv2 for the same velocities.
 __global__ __launch_bounds__(SEPK_BLOCK) void k_nh_sum(sepk_state s, long n, long npad,
 							double *partial, long rows)
 {
 	if (*s.err)
 		return; // the finisher applies nothing either
-	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	const long lb = sepk_xcd_block();
+	const long t = lb * (SEPK_BLOCK / SEPK_SUB) + threadIdx.x / SEPK_SUB;
 	double acc[1] = {0.0};
-	if (t < n) {
+	if (t < n && threadIdx.x % SEPK_SUB == 0) {
 		const double m = s.m[t];
-		for (int k = 0; k < 3; k++) {
-			const double v = s.v[k * npad + t];
-			acc[0] += v * v * m;
-		}
+		const double vx = s.v[t], vy = s.v[npad + t], vz = s.v[2 * npad + t];
+		acc[0] = (vx * vx * m + vy * vy * m) + vz * vz * m;
 	}
-	sepk_block_partial<1>(acc, 0u, partial, blockIdx.x, rows);
+	sepk_block_partial<1>(acc, 0u, partial, lb, rows);
 }
 
 __global__ void k_nh_apply(sepk_state s, long n, long npad, const double *alpha)
@@ -1096,7 +1107,7 @@ extern "C" int sepdev_nosehoover(sepdev *d, double Td, double tau, double dt,
 	}
 	if ((rc = sepk_materialize_forces(d)))
 		return rc;
-	const unsigned nb = sepk_blocks(d->n);
+	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);
 	if ((rc = sepk_flush_tasks(d)))
 		return rc;
 	k_nh_sum<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad,
@@ -1115,7 +1126,8 @@ extern "C" int sepdev_nosehoover(sepdev *d, double Td, double tau, double dt,
 	if ((rc = sepk_queue_task(d, f)) || (rc = sepk_flush_tasks(d)))
 		return rc;
 	d->sumv2_valid = 1;
-	k_nh_apply<<<nb, SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad, d->scal + alpha_slot);
+	k_nh_apply<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad,
+								 d->scal + alpha_slot);
 	SEPK_LAUNCH_CHECK();
 	return 0;
 }
