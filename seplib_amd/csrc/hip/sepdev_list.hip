@@ -5,6 +5,9 @@
 //                  per-cell counts and slots by wave-aggregated atomics
 //   k_scan         one workgroup: exclusive scan of the counts -> cell_start
 //   k_scatter      atom -> start of its cell + its slot
+//                  (k_scan_scatter: both in one launch, every workgroup
+//                  scanning the counts itself, for grids of up to 16352
+//                  cells)
 //   k_sort_permute one wave per cell: order the segment by original atom
 //                  index (so the device order and every list are independent
 //                  of atomic timing) and move each member's state to its new
@@ -193,6 +196,80 @@ __global__ __launch_bounds__(1024) void k_scan(int *count, int *start, long ncel
 		start[ncell] = carry;
 }
 
+// The scan and the scatter in one launch, for grids of at most
+// SEPK_PLACE_CELLS cells: every workgroup scans the cell counts into LDS
+// (one pass of SEPK_SCAN_ITEMS cells a thread, as k_scan), the first also
+// writes them to start[], and each then places its share of the atoms.  No
+// workgroup waits for another.  It fixes sys->frozen as k_scan does; the
+// counts, which every workgroup reads, are zeroed by k_sort_permute.
+#define SEPK_PLACE_CELLS (1024 * SEPK_SCAN_ITEMS - 32) // 64 KB of LDS with the wave sums
+__global__ __launch_bounds__(1024) void k_scan_scatter(const int *count, int *start,
+						       const int *cellid, const int *slot, int *order,
+						       long ncell, long n, sepdev_sys *sys,
+						       int conditional)
+{
+	if (conditional && sys->neighb_flag == 0)
+		return;
+	const bool frozen = sys->error != 0;
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		sys->frozen = frozen;
+	if (frozen)
+		return; // the permutation and the build skip as well
+	__shared__ int sstart[SEPK_PLACE_CELLS + 1];
+	__shared__ int wsum[16];
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const long c0 = (long)threadIdx.x * SEPK_SCAN_ITEMS;
+	int v[SEPK_SCAN_ITEMS];
+	if (c0 + SEPK_SCAN_ITEMS <= ncell) {
+		// 16-byte aligned (count comes from hipMalloc, c0 % 16 == 0)
+		const int4 *q = (const int4 *)(count + c0);
+#pragma unroll
+		for (int k = 0; k < SEPK_SCAN_ITEMS / 4; k++) {
+			const int4 w = q[k];
+			v[4 * k] = w.x;
+			v[4 * k + 1] = w.y;
+			v[4 * k + 2] = w.z;
+			v[4 * k + 3] = w.w;
+		}
+	} else {
+#pragma unroll
+		for (int k = 0; k < SEPK_SCAN_ITEMS; k++)
+			v[k] = c0 + k < ncell ? count[c0 + k] : 0;
+	}
+	int tot = 0;
+#pragma unroll
+	for (int k = 0; k < SEPK_SCAN_ITEMS; k++)
+		tot += v[k];
+	int x = tot; // inclusive wave scan of the thread totals
+	for (int o = 1; o < 64; o <<= 1) {
+		const int y = __shfl_up(x, o, 64);
+		if (lane >= o)
+			x += y;
+	}
+	if (lane == 63)
+		wsum[wid] = x;
+	__syncthreads();
+	int off = 0;
+	for (int w = 0; w < wid; w++)
+		off += wsum[w];
+	off += x - tot;
+#pragma unroll
+	for (int k = 0; k < SEPK_SCAN_ITEMS; k++) {
+		if (c0 + k < ncell)
+			sstart[c0 + k] = off;
+		off += v[k];
+		if (c0 + k == ncell - 1)
+			sstart[ncell] = off;
+	}
+	__syncthreads();
+	if (blockIdx.x == 0)
+		for (long c = threadIdx.x; c <= ncell; c += blockDim.x)
+			start[c] = sstart[c];
+	for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+	     t += (long)gridDim.x * blockDim.x)
+		order[sstart[cellid[t]] + slot[t]] = (int)t;
+}
+
 // Atom t -> its place in the cell-sorted order (slot from k_bin).
 __global__ void k_scatter(const int *cellid, const int *slot, const int *start, int *order,
 			  long n, sepdev_sys *sys, int conditional)
@@ -280,9 +357,16 @@ __device__ __forceinline__ void sepk_move_atom(const sepk_state &src, long o,
 __global__ void k_sort_permute(const int *order, const int *start, sepk_state src,
 			       sepk_state dst, int *iperm, long ncell,
 			       long npad, sepdev_sys *sys, int conditional,
-			       int has_partners)
+			       int has_partners, int *count)
 {
-	if ((conditional && sys->neighb_flag == 0) || sys->frozen)
+	if (conditional && sys->neighb_flag == 0)
+		return;
+	// the counts start from zero at the next rebuild, whether this one runs
+	// on or stops here
+	for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+	     c += (long)gridDim.x * blockDim.x)
+		count[c] = 0;
+	if (sys->frozen)
 		return;
 	const int pending = sys->xn_pending;
 	const int lane = threadIdx.x & 63;
@@ -825,13 +909,24 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 	k_bin<<<nb, SEPK_BLOCK, 0, d->st>>>(cur.pos, d->n, g, d->cellid_old, d->cell_slot,
 					    d->cell_count, d->sys,
 					    conditional);
-	k_scan<<<1, 1024, 0, d->st>>>(d->cell_count, d->cell_start, ncell, d->sys,
-				      conditional);
-	k_scatter<<<nb, SEPK_BLOCK, 0, d->st>>>(d->cellid_old, d->cell_slot, d->cell_start,
-						d->order_tmp, d->n, d->sys, conditional);
+	if (ncell <= SEPK_PLACE_CELLS) {
+		// one launch: each workgroup scans the counts itself
+		long pb = (d->n + 1023) / 1024;
+		if (pb > 256)
+			pb = 256;
+		k_scan_scatter<<<(unsigned)pb, 1024, 0, d->st>>>(
+			d->cell_count, d->cell_start, d->cellid_old, d->cell_slot, d->order_tmp, ncell,
+			d->n, d->sys, conditional);
+	} else {
+		k_scan<<<1, 1024, 0, d->st>>>(d->cell_count, d->cell_start, ncell, d->sys,
+					      conditional);
+		k_scatter<<<nb, SEPK_BLOCK, 0, d->st>>>(d->cellid_old, d->cell_slot,
+							d->cell_start, d->order_tmp, d->n, d->sys,
+							conditional);
+	}
 	k_sort_permute<<<(unsigned)sb, SEPK_BLOCK, 0, d->st>>>(
 		d->order_tmp, d->cell_start, cur, spare, d->iperm, ncell, d->npad,
-		d->sys, conditional, hp);
+		d->sys, conditional, hp, d->cell_count);
 	SEPK_LAUNCH_CHECK();
 	// Home atoms of a cell split over nslice workgroups when the cells are
 	// few and full (small boxes, dense molecular systems): at least 16 home
