@@ -28,7 +28,14 @@ launches on the final state of the timed run, with the integrator's outputs
 sent to scratch so that every launch sees the same state.
 ``traffic`` is the HBM bytes per launch of that kernel from the rocprofv3
 FETCH_SIZE / WRITE_SIZE counter summary committed under profiles/, scaled by
-the counters' reading for a plain device copy of known size.
+the counters' reading for a plain device copy of known size.  ``valu`` gives
+the same kernel's FP64 and FP32 arithmetic rates against the chip's vector
+peaks, from algorithmic operation counts: per full-list entry 8 FP64
+operations (three scalings of the integer differences, r^2); per entry
+inside the cutoff (counted on the final state with a periodic k-d tree) 7
+more in FP64 (three force FMAs, the energy sum) and 12 in FP32 (the
+reciprocal and the LJ terms); per atom 45 FP64 for the thermostat term, the
+leapfrog and its sums.
 
 cpu_baseline: the reference's own C (oracle/_ref/libsepref_fast.so, built
 from its sources with the shipped -Ofast flags) with OpenMP on the host's
@@ -55,6 +62,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6   # MI355X vector FP64
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32 (packed)
 N_SIDE = 40
 RHO = 0.8442
 TEMP = 1.0
@@ -223,6 +232,29 @@ def precision_study(api, w, nvt_steps, nve_steps, every=100):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lj_lj256k_traffic.json")
 
 
+def pairs_inside(x, L, rc):
+    """Ordered pairs (i, j != i) closer than rc under the minimum image."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+    x = np.mod(np.asarray(x, dtype=float), L)
+    t = cKDTree(x, boxsize=L)
+    return int(t.count_neighbors(t, rc)) - len(x)
+
+
+def valu_rates(n, nbar, inside, kernel_ms):
+    """FP64 / FP32 operation rates of the fused mixed step kernel."""
+    entries = n * nbar
+    fp64 = 8.0 * entries + 7.0 * inside + 45.0 * n
+    fp32 = 12.0 * inside
+    sec = kernel_ms * 1e-3
+    r64, r32 = fp64 / sec / 1e12, fp32 / sec / 1e12
+    return dict(fp64_tflops=round(r64, 3), fp64_peak=FP64_PEAK_TFLOPS,
+                fp64_frac=round(r64 / FP64_PEAK_TFLOPS, 4),
+                fp32_tflops=round(r32, 3), fp32_peak=FP32_PEAK_TFLOPS,
+                fp32_frac=round(r32 / FP32_PEAK_TFLOPS, 4),
+                fp64_ops=round(fp64), fp32_ops=round(fp32), pairs_inside=inside)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary,
     or None."""
@@ -284,6 +316,7 @@ def main():
             "epot/N %.6f" % (name, secs, n * args.steps / secs, kms * 1e3, nbar,
                              s.sys.nupdate_neighb, s.ret.epot / n))
     ferr_max, ferr_rms, eerr = final_state_force_error(api, res["mixed"]["sys"])
+    inside = pairs_inside(res["mixed"]["sys"].view["x"], w["L"][0], RC)
     log("mixed vs fp64 forces on the final state: max %.3e rms %.3e (relative to RMS|F|), "
         "epot %.3e" % (ferr_max, ferr_rms, eerr))
     for r in res.values():
@@ -311,6 +344,7 @@ def main():
                     kernel_us=round(m["kms"] * 1e3, 3),
                     algorithmic_bytes=round(bytes_per_atom * n),
                     bytes_per_atom=round(bytes_per_atom, 2))
+    valu = valu_rates(n, m["nbar"], inside, m["kms"])
     study = None
     if not args.no_study:
         k = args.study_steps
@@ -357,6 +391,7 @@ def main():
         "fp64_pair_kernel_us": round(res["fp64"]["kms"] * 1e3, 3),
         "rebuilds": m["rebuilds"],
         "roofline": roofline,
+        "valu": valu,
         "cpu_baseline": cpu,
     }
     if study is not None:
