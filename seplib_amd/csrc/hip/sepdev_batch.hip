@@ -1185,8 +1185,11 @@ static int put_sys(sepdev_batch *b)
 
 static void drop_graph(sepdev_batch *b)
 {
-	if (b->exec)
+	// replays of the old graph may still be queued on the stream
+	if (b->exec) {
+		(void)hipStreamSynchronize(b->st);
 		(void)hipGraphExecDestroy(b->exec);
+	}
 	if (b->graph)
 		(void)hipGraphDestroy(b->graph);
 	b->exec = nullptr;

@@ -762,8 +762,9 @@ extern "C" int sepdev_graph_end(sepdev *d)
 	int rc = sepk_flush_tasks(d);
 	d->capturing = 0;
 	if (rc) {
-		hipGraph_t g;
-		(void)hipStreamEndCapture(d->st, &g);
+		hipGraph_t g = nullptr;
+		if (hipStreamEndCapture(d->st, &g) == hipSuccess && g)
+			(void)hipGraphDestroy(g);
 		return rc;
 	}
 	SEPK_CHECK(hipStreamEndCapture(d->st, &d->graph));
@@ -783,6 +784,9 @@ extern "C" int sepdev_graph_launch(sepdev *d, long times)
 extern "C" int sepdev_graph_drop(sepdev *d)
 {
 	if (d->exec) {
+		// the replays of the last run may still be queued on the stream: the
+		// executable graph goes only after they are done
+		(void)hipStreamSynchronize(d->st);
 		(void)hipGraphExecDestroy(d->exec);
 		d->exec = nullptr;
 	}
