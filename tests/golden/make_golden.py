@@ -8,7 +8,7 @@ with ``make -C oracle``).  Run from the repository root:
 Fixtures (numpy .npz, no pickles; inputs and outputs together, so the tests
 never read the reference tree):
 
-  g1_lj_nve.npz    config 1: LJ N=1000 SC 10^3, rho 0.8442, T 0.728, rc 2.5,
+  g1_lj_nve.npz    LJ N=1000 SC 10^3, rho 0.8442, T 0.728, rc 2.5,
                    skin 0.5, dt 0.005, sep_force_pairs + sep_leapfrog for
                    1000 steps.  Per step: epot, ekin, max_dist2, pot_P,
                    kin_P, tnow; neighb_flag, nupdate_neighb; half-list size
@@ -60,7 +60,7 @@ G1_FSTEPS = (0, 1, 500, 999)
 
 
 def g1(ref, orc):
-    w = R.lj_cfg1()
+    w = R.lj_sc1000()
     scal, flags, hashes, npairs, forces = R.run_nve(ref, orc.lib, w, 1000, G1_FSTEPS)
     np.savez_compressed(os.path.join(HERE, "g1_lj_nve.npz"), x=w["x"], v=w["v"], m=w["m"],
                         L=w["L"], scal=scal, flags=flags, hashes=hashes, npairs=npairs,
@@ -68,7 +68,7 @@ def g1(ref, orc):
 
 
 def g4(ref):
-    w = R.lj_cfg1()
+    w = R.lj_sc1000()
     out = R.run_nvt(ref, w, 200)
     np.savez_compressed(os.path.join(HERE, "g4_lj_nvt.npz"), x=w["x"], v=w["v"], m=w["m"],
                         L=w["L"], alpha_ekin_epot=out)

@@ -114,9 +114,9 @@ def term_run(api, w, cf, update, calls, skin=None, prepare=None):
     return np.array(forces), np.array(rets)
 
 
-# ---- config 1 (prgs/prg0.c): LJ NVE ---------------------------------------
+# ---- LJ NVE, 1000 atoms on a simple-cubic lattice (prgs/prg0.c) -------------
 
-def lj_cfg1():
+def lj_sc1000():
     return W.lj_sc(10, 0.8442, 0.728, seed=1)
 
 

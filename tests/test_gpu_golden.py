@@ -4,7 +4,7 @@ the compiled reference (tests/golden/make_golden.py).
 Bars (plain FP64 path): forces max|dF_i| / RMS|F| <= 1e-12 per force term;
 energies and the virial tensors <= 1e-12 relative (tensors relative to their
 largest component); pair sets bit-exact; rebuild flags and counters exact.
-Trajectories (config 1) are compared over their first 60 steps, before the
+Trajectories (LJ N=1000 NVE) are compared over their first 60 steps, before the
 chaotic growth of summation-order differences (~1e-16 per step) can reach
 the bar; the state-synchronised tests in test_gpu_lj.py cover long runs.
 """

@@ -80,8 +80,8 @@ def _sync_from(ga, orc):
 
 
 @pytest.mark.parametrize("nh", [False, True], ids=["nve", "nvt"])
-def test_lj_cfg1_state_synchronised(product, oracle, nh):
-    """Config 1 (SC 10^3, rho 0.8442, T 0.728, rc 2.5, skin 0.5, dt 0.005)."""
+def test_lj_sc1000_state_synchronised(product, oracle, nh):
+    """LJ N=1000 (SC 10^3, rho 0.8442, T 0.728, rc 2.5, skin 0.5, dt 0.005)."""
     w = W.lj_sc(10, 0.8442, 0.728, seed=1)
     ga = SepSystem.from_workload(product, w, cf=2.5, dt=0.005, skin=0.5)
     orc = SepSystem.from_workload(oracle, w, cf=2.5, dt=0.005, skin=0.5)
