@@ -425,17 +425,17 @@ __device__ __forceinline__ void sepk_copy_home(const sepk_state &src, const sepk
 // One workgroup per home cell at a time (cells split into one contiguous
 // range per XCD, walked with a stride of the range's workgroup count).
 // The stencil cells' segments are staged in LDS as SoA chunks of
-// SEPK_BCHUNK candidates (position, device index, stencil offset, molecule)
-// and the cell's members ("home" atoms, SEPK_BHOME at a time) beside them.
-// Each wave owns every fourth home atom: it takes 64 staged candidates into
-// registers and tests them against each of its atoms, whose position is an
-// LDS broadcast; the wave ballot of each test (one bit per candidate) is
-// kept in LDS.  An emit pass then writes the lists: each thread takes one
-// 64-candidate word of one home atom, finds the word's first entry number
-// from the atom's running length and the popcounts of its earlier words,
-// and walks the set bits.  Lists therefore come out in stencil-offset
-// order and, within a cell, in device order.  Entry k of atom t goes to
-// slot sepk_entry(k, t, npad).
+// SEPK_BCHUNK candidates (position, device index, stencil offset, molecule,
+// quantized position) and the cell's members ("home" atoms, SEPK_BHOME at a
+// time) beside them.  Each wave owns every fourth home atom: a round takes
+// 64 staged candidates into registers (one a lane) and tests them against
+// each of the wave's atoms, whose position is an LDS broadcast; the lanes
+// that pass store their candidate to the atom's list at once, each at the
+// atom's length so far plus its rank among the passing lanes (mbcnt), and
+// the length (lane kk of a register for the wave's kk-th atom) grows by
+// their number.  Lists therefore come out in stencil-offset order and,
+// within a cell, in device order.  Entry k of atom t goes to slot
+// sepk_entry(k, t, npad).
 //
 // SHIFT: with at least 5 cells on every axis and every atom inside its own
 // cell, |x_i - x_j| < 2l <= 0.4 L for a target cell reached without
@@ -444,150 +444,216 @@ __device__ __forceinline__ void sepk_copy_home(const sepk_state &src, const sepk
 // is the constant shift of the target cell; d + shift rounds exactly like
 // its d - L / d + L, and d + 0 is d.  Otherwise the wrap is evaluated per
 // candidate.
+//
+// Quantized test (SHIFT and a cubic box): every coordinate is also held as
+// floor(x * 65536 / L) mod 2^16, x and y in one word and z in the low half
+// of another.  A packed 16-bit subtraction gives the minimum-image
+// difference in units of u = L / 65536, each component within one unit of
+// the exact one (|x_i - x_j| < 0.4 L keeps it away from the wrap), and two
+// saturating packed dot products its square r2q.  With R = r_list / u the
+// distance is below r_list - 0.26 u when r2q < floor((R - 2)^2) and above
+// r_list + 0.26 u when r2q > ceil((R + 2)^2) (sqrt(3) < 1.74), far outside
+// the rounding of the FP64 r2; only candidates between the two bounds (a
+// shell about 4 u thick: a few in ten thousand) take the FP64 test, the
+// reference's own, so the accepted set is the FP64 test's exactly.
 #define SEPK_BT 256      // threads of a build workgroup (4 waves)
 #define SEPK_BCHUNK 384  // candidates staged at a time
-#define SEPK_BWORDS (SEPK_BCHUNK / 64)
 #define SEPK_BHOME 64    // home atoms staged at a time
 #define SEPK_BGRID 1792  // build workgroups at most (a multiple of 8)
 static_assert(SEPK_BCHUNK % 64 == 0, "candidates are tested 64 at a time");
+static_assert(SEPK_BHOME <= 64 * (SEPK_BT / 64), "a wave's home atoms fit its lanes");
 
 struct sepk_build_lds {
 	double cx[SEPK_BCHUNK], cy[SEPK_BCHUNK], cz[SEPK_BCHUNK];
+	uint2 cq[SEPK_BCHUNK];
 	int cj[SEPK_BCHUNK], cm[SEPK_BCHUNK];
 	unsigned char co[SEPK_BCHUNK];
 	double hx[SEPK_BHOME], hy[SEPK_BHOME], hz[SEPK_BHOME];
+	uint2 hq[SEPK_BHOME];
 	int horig[SEPK_BHOME], hmol[SEPK_BHOME];
 	int segjs[SEPDEV_MAX_OFF], segpre[SEPDEV_MAX_OFF + 1];
 	int segrule[SEPDEV_MAX_OFF]; // rule when the target is the home cell, else -1
 	double segsh[SEPDEV_MAX_OFF][3];
-	// accepted candidates of word w of the chunk for home atom i
-	unsigned long long vote[SEPK_BHOME][SEPK_BWORDS];
-	int hcnt[2][SEPK_BHOME]; // list lengths before / after the chunk
 	int hs, h, anyshift;
 };
 
-// Tests the 64 staged candidates [q0, q0 + 64) of a chunk of kn against
-// the wave's home atoms i = wid, wid + 4, ... < hn and stores each ballot
-// in L.vote[i][q0 / 64].
-// SH: some stencil target of the cell lies across the box (shift adds).
+// Bounds of the quantized test; on = the test applies (SHIFT, cubic box).
+struct sepk_qtest {
+	double scale; // 65536 / L
+	int lo2, hi2;
+	bool on;
+};
+
+__device__ __forceinline__ sepk_qtest sepk_qtest_of(const sepdev_grid &g, bool shift)
+{
+	sepk_qtest q;
+	q.scale = 65536.0 / g.L[0];
+	const double R = sqrt(g.rlist2) * q.scale;
+	// SHIFT has L >= 5 r_list, so R <= 13108 and both bounds fit 31 bits
+	q.on = shift && g.L[0] == g.L[1] && g.L[1] == g.L[2] && R < 30000.0;
+	q.lo2 = R > 2.0 ? (int)floor((R - 2.0) * (R - 2.0)) : 0;
+	q.hi2 = q.on ? (int)ceil((R + 2.0) * (R + 2.0)) : 0;
+	return q;
+}
+
+// x and y in the low and high halves of .x, z in the low half of .y
+__device__ __forceinline__ uint2 sepk_quantize(const sepk_pos &p, double scale)
+{
+	const unsigned qx = (unsigned)(int)floor(p.x * scale) & 0xffffu;
+	const unsigned qy = (unsigned)(int)floor(p.y * scale) & 0xffffu;
+	const unsigned qz = (unsigned)(int)floor(p.z * scale) & 0xffffu;
+	return make_uint2(qx | qy << 16, qz);
+}
+
+typedef short sepk_short2 __attribute__((ext_vector_type(2)));
+
+// Squared quantized minimum-image distance, saturated at 2^31 - 1.
+__device__ __forceinline__ int sepk_r2q(uint2 a, uint2 b)
+{
+	const sepk_short2 dxy =
+		__builtin_bit_cast(sepk_short2, a.x) - __builtin_bit_cast(sepk_short2, b.x);
+	const sepk_short2 dz =
+		__builtin_bit_cast(sepk_short2, a.y) - __builtin_bit_cast(sepk_short2, b.y);
+	return __builtin_amdgcn_sdot2(dz, dz, __builtin_amdgcn_sdot2(dxy, dxy, 0, true), true);
+}
+
+// Lane `lane` of v (a wave-uniform lane number) becomes val, whatever the
+// execution mask.
+__device__ __forceinline__ void sepk_writelane(int &v, int val, int lane)
+{
+	asm volatile("v_writelane_b32 %0, %1, %2" : "+v"(v) : "s"(val), "{m0}"(lane));
+}
+
+// One round: the 64 staged candidates [q0, q0 + 64) of a chunk of kn
+// against the wave's home atoms i = wid, wid + 4, ... < hn, storing the
+// accepted ones to their lists.  Lane kk of cnt holds the list length of
+// the wave's kk-th home atom; entries at or beyond cap are counted but not
+// stored.
+// QNT: the quantized test (SHIFT holds).  SH: some stencil target of the
+// cell lies across the box (shift adds; the quantized test needs none).
 // EXCL: 0 = SEP_ALL, 2 = SEP_EXCL_BONDED, 3 = SEP_EXCL_SAME_MOL.
-template <bool SHIFT, bool SH, int EXCL>
-__device__ __forceinline__ void sepk_build_test(
-	sepk_build_lds &L, const sepk_state &s, const sepdev_grid &g, int q0, int kn,
-	long tbase, int hn, long npad)
+template <bool SHIFT, bool SH, int EXCL, bool QNT>
+__device__ __forceinline__ void sepk_build_round(
+	sepk_build_lds &L, const sepk_state &s, const sepdev_grid &g, const sepk_qtest &qt,
+	int q0, int kn, long tbase, int hn, long npad, int cap, int *nbr, int &cnt)
 {
 	const int lane = threadIdx.x & 63;
 	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
 	const int q = min(q0 + lane, kn - 1);
-	const bool valid = q0 + lane < kn;
-	const double px = L.cx[q], py = L.cy[q], pz = L.cz[q];
+	const int left = kn - q0;
+	const unsigned long long vmask = left >= 64 ? ~0ull : (1ull << left) - 1ull;
+	// lists of fewer than 2^20 padded atoms: 32-bit slot offsets with a
+	// full-rate 24-bit multiply (0: sepk_entry)
+	const unsigned npad16 = npad < (1l << 20) ? (unsigned)npad * SEPK_PAD : 0u;
 	const int j = L.cj[q];
 	const int o = L.co[q];
 	const int rule = L.segrule[o];
 	const int mj = EXCL == 3 ? L.cm[q] : -1;
+	// lanes whose candidate is in the home cell (SHIFT: through offset 0,
+	// whose rule is NE)
+	const unsigned long long home = __ballot(rule >= 0);
+	// original index of a home-cell candidate, for the GT / LT rules
+	const int oj = !SHIFT && rule >= SEPDEV_RULE_GT ? s.orig[j] : 0;
+	uint2 cq = make_uint2(0u, 0u);
+	double px = 0.0, py = 0.0, pz = 0.0;
+	if (QNT) {
+		cq = L.cq[q];
+	} else {
+		px = L.cx[q];
+		py = L.cy[q];
+		pz = L.cz[q];
+	}
 	double sx = 0.0, sy = 0.0, sz = 0.0;
-	if (SH) {
+	if (SH && !QNT) {
 		sx = L.segsh[o][0];
 		sy = L.segsh[o][1];
 		sz = L.segsh[o][2];
 	}
-	// lane kk collects the ballot of the wave's kk-th home atom
-	unsigned vlo = 0u, vhi = 0u;
 	int kk = 0;
 	for (int i = wid; i < hn; i += SEPK_BT / 64, kk++) {
 		const long t = tbase + i;
 		const int t32 = (int)t;
-		bool take = valid;
+		unsigned long long take;
+		if (QNT) {
+			const int r2q = sepk_r2q(L.hq[i], cq);
+			take = __ballot(r2q < qt.lo2) & vmask;
+			const unsigned long long near = __ballot(r2q <= qt.hi2) & vmask & ~take;
+			if (near) {
+				// the reference's FP64 test, with the target cell's shift
+				const double dx = L.hx[i] - L.cx[q] + L.segsh[o][0];
+				const double dy = L.hy[i] - L.cy[q] + L.segsh[o][1];
+				const double dz = L.hz[i] - L.cz[q] + L.segsh[o][2];
+				const double r2 = dx * dx + dy * dy + dz * dz;
+				take |= __ballot(r2 < g.rlist2) & near;
+			}
+		} else {
+			double dx, dy, dz;
+			if (SHIFT) {
+				dx = L.hx[i] - px;
+				dy = L.hy[i] - py;
+				dz = L.hz[i] - pz;
+				if (SH) {
+					dx += sx;
+					dy += sy;
+					dz += sz;
+				}
+			} else {
+				dx = sepk_wrap(L.hx[i] - px, g.L[0]);
+				dy = sepk_wrap(L.hy[i] - py, g.L[1]);
+				dz = sepk_wrap(L.hz[i] - pz, g.L[2]);
+			}
+			const double r2 = dx * dx + dy * dy + dz * dz;
+			take = __ballot(r2 < g.rlist2) & vmask;
+		}
 		if (SHIFT) {
 			// the 27-cell stencil reaches the home cell only through
-			// offset 0, whose rule is NE
-			take = take && !(rule >= 0 && j == t32);
-		} else if (rule == SEPDEV_RULE_NE) {
-			take = take && j != t32;
-		} else if (rule >= 0 && take) {
-			const int oj = s.orig[j];
+			// offset 0: drop the atom itself
+			take &= ~(__ballot(j == t32) & home);
+		} else if (take & home) {
 			const int oi = L.horig[i];
-			take = rule == SEPDEV_RULE_GT ? oj > oi : oj < oi;
+			const bool drop = rule == SEPDEV_RULE_NE ? j == t32
+					  : rule == SEPDEV_RULE_GT ? oj <= oi
+								   : oj >= oi;
+			take &= ~(__ballot(drop) & home);
 		}
 		if (EXCL == 3 /* SEP_EXCL_SAME_MOL */) {
 			const int mi = L.hmol[i];
-			take = take && (mi == -1 || mi != mj);
+			if (mi != -1)
+				take &= ~__ballot(mj == mi);
 		}
-		double dx, dy, dz;
-		if (SHIFT) {
-			dx = L.hx[i] - px;
-			dy = L.hy[i] - py;
-			dz = L.hz[i] - pz;
-			if (SH) {
-				dx += sx;
-				dy += sy;
-				dz += sz;
-			}
-		} else {
-			dx = sepk_wrap(L.hx[i] - px, g.L[0]);
-			dy = sepk_wrap(L.hy[i] - py, g.L[1]);
-			dz = sepk_wrap(L.hz[i] - pz, g.L[2]);
-		}
-		const double r2 = dx * dx + dy * dy + dz * dz;
-		take = take & (r2 < g.rlist2);
 		if (EXCL == 2 /* SEP_EXCL_BONDED */ && take) {
-			const int oi = L.horig[i];
-			const int oj = s.orig[j];
-			int sh = sepk_share(s.partners, npad, t, j, 0, 10, oi, oj);
-			if (g.bonded_all)
-				sh += sepk_share(s.partners, npad, t, j, 10, 10, oi, oj) +
-				      sepk_share(s.partners, npad, t, j, 20, 20, oi, oj);
-			// rule 2 is the Coulomb brute-force test of
-			// source/sepcoulomb.c:44, which excludes only when exactly
-			// one kind of partnership holds
-			take = g.bonded_all == 2 ? sh != 1 : sh == 0;
+			bool ok = false;
+			if (__builtin_amdgcn_inverse_ballot_w64(take)) {
+				const int oi = L.horig[i];
+				const int oj2 = s.orig[j];
+				int sh = sepk_share(s.partners, npad, t, j, 0, 10, oi, oj2);
+				if (g.bonded_all)
+					sh += sepk_share(s.partners, npad, t, j, 10, 10, oi, oj2) +
+					      sepk_share(s.partners, npad, t, j, 20, 20, oi, oj2);
+				// rule 2 is the Coulomb brute-force test of
+				// source/sepcoulomb.c:44, which excludes only when
+				// exactly one kind of partnership holds
+				ok = g.bonded_all == 2 ? sh != 1 : sh == 0;
+			}
+			take = __ballot(ok);
 		}
-		const unsigned long long vote = __ballot(take);
-		if (lane == kk) {
-			vlo = (unsigned)vote;
-			vhi = (unsigned)(vote >> 32);
+		if (take) {
+			const int e0 = __builtin_amdgcn_readlane(cnt, kk);
+			const int e = (int)__builtin_amdgcn_mbcnt_hi(
+				(unsigned)(take >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)take, (unsigned)e0));
+			if (__builtin_amdgcn_inverse_ballot_w64(take) && e < cap) {
+				if (npad16 != 0u) {
+					// the atom's first slot as a scalar base, a 32-bit offset
+					const unsigned off = __umul24((unsigned)e / SEPK_PAD, npad16) +
+							     ((unsigned)e % SEPK_SUB) * SEPK_BATCH +
+							     ((unsigned)e / SEPK_SUB) % SEPK_BATCH;
+					(nbr + (size_t)t * SEPK_PAD)[off] = j;
+				} else {
+					nbr[sepk_entry(e, t, npad)] = j;
+				}
+			}
+			sepk_writelane(cnt, e0 + __popcll(take), kk);
 		}
-	}
-	const int i = wid + (SEPK_BT / 64) * lane;
-	if (lane < kk && i < hn)
-		L.vote[i][q0 / 64] = ((unsigned long long)vhi << 32) | vlo;
-}
-
-// Slot of entry e of atom t (sepk_entry); lists of fewer than 2^20 padded
-// atoms take 32-bit arithmetic with a full-rate 24-bit multiply.
-__device__ __forceinline__ int *sepk_slot_ptr(int *nbr, unsigned e, long t, long npad)
-{
-	if (npad < (1l << 20)) {
-		const unsigned npad16 = (unsigned)npad * SEPK_PAD;
-		const unsigned off = __umul24(e / SEPK_PAD, npad16) + (unsigned)t * SEPK_PAD +
-				     (e % SEPK_SUB) * SEPK_BATCH + (e / SEPK_SUB) % SEPK_BATCH;
-		return nbr + off;
-	}
-	return nbr + sepk_entry(e, t, npad);
-}
-
-// Writes the accepted candidates of a chunk of kn to the lists of the hn
-// home atoms from tbase: thread x takes the words (home atom, word) x,
-// x + SEPK_BT, ...  The lengths before the chunk are in L.hcnt[par]; the
-// last word of each atom leaves the length after it in L.hcnt[par ^ 1].
-// Entries at or beyond cap are counted but not stored.
-__device__ __forceinline__ void sepk_build_emit(sepk_build_lds &L, int kn, int hn, int par,
-						long tbase, long npad, int cap, int *nbr)
-{
-	const int nw = (kn + 63) / 64;
-	for (int it = threadIdx.x; it < hn * nw; it += SEPK_BT) {
-		const int i = it / nw, w = it - i * nw;
-		int e = L.hcnt[par][i];
-		for (int u = 0; u < w; u++)
-			e += __popcll(L.vote[i][u]);
-		unsigned long long m = L.vote[i][w];
-		if (w == nw - 1)
-			L.hcnt[par ^ 1][i] = e + __popcll(m);
-		const long t = tbase + i;
-		const int *cj = L.cj + w * 64;
-		for (; m; m &= m - 1, e++)
-			if (e < cap)
-				*sepk_slot_ptr(nbr, (unsigned)e, t, npad) = cj[__builtin_ctzll(m)];
 	}
 }
 
@@ -596,8 +662,8 @@ __device__ __forceinline__ void sepk_build_emit(sepk_build_lds &L, int kn, int h
 template <bool SHIFT, int EXCL>
 __device__ __forceinline__ void sepk_build_cell(
 	sepk_build_lds &L, const sepk_state &s, long c, int slice, int nslice,
-	const int *start, const sepdev_grid &g, long npad, int cap, int phantom, int *nbr,
-	int *cnt, sepdev_sys *sys, int &mymax)
+	const int *start, const sepdev_grid &g, const sepk_qtest &qt, long npad, int cap,
+	int phantom, int *nbr, int *cnt, sepdev_sys *sys, int &mymax)
 {
 	const int lane = threadIdx.x & 63;
 	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
@@ -649,12 +715,11 @@ __device__ __forceinline__ void sepk_build_cell(
 	// h < 2^24 atoms a cell and nslice <= 64: 32-bit products
 	const int hbeg = nslice == 1 ? 0 : (int)((unsigned)(h * slice) / (unsigned)nslice);
 	const int hend = nslice == 1 ? h : (int)((unsigned)(h * (slice + 1)) / (unsigned)nslice);
+	const bool qnt = SHIFT && qt.on;
 	for (int hb = hbeg; hb < hend; hb += SEPK_BHOME) {
 		const int hn = min(hend - hb, SEPK_BHOME);
-		if (threadIdx.x < SEPK_BHOME)
-			L.hcnt[0][threadIdx.x] = 0;
-		int par = 0;
-		for (int k0 = 0; k0 < T; k0 += SEPK_BCHUNK, par ^= 1) {
+		int len = 0; // lane kk: list length of the wave's kk-th home atom
+		for (int k0 = 0; k0 < T; k0 += SEPK_BCHUNK) {
 			const int kn = min(T - k0, SEPK_BCHUNK);
 			if (k0 > 0)
 				__syncthreads(); // the previous chunk is no longer read
@@ -692,6 +757,8 @@ __device__ __forceinline__ void sepk_build_cell(
 					L.cx[q] = pq[r].x;
 					L.cy[q] = pq[r].y;
 					L.cz[q] = pq[r].z;
+					if (qnt)
+						L.cq[q] = sepk_quantize(pq[r], qt.scale);
 					L.cj[q] = jq[r];
 					L.co[q] = (unsigned char)oq[r];
 					if (EXCL == 3)
@@ -705,24 +772,29 @@ __device__ __forceinline__ void sepk_build_cell(
 					L.hx[i] = p.x;
 					L.hy[i] = p.y;
 					L.hz[i] = p.z;
+					if (qnt)
+						L.hq[i] = sepk_quantize(p, qt.scale);
 					L.horig[i] = s.orig[t];
 					L.hmol[i] = EXCL == 3 ? s.mol[t] : -1;
 				}
 			__syncthreads();
+			const long tb = hs + hb;
 			for (int q0 = 0; q0 < kn; q0 += 64) {
-				if (SHIFT && anyshift)
-					sepk_build_test<SHIFT, true, EXCL>(L, s, g, q0, kn, hs + hb, hn, npad);
+				if (qnt)
+					sepk_build_round<true, false, EXCL, true>(L, s, g, qt, q0, kn, tb, hn,
+										  npad, cap, nbr, len);
+				else if (SHIFT && anyshift)
+					sepk_build_round<SHIFT, true, EXCL, false>(L, s, g, qt, q0, kn, tb, hn,
+										   npad, cap, nbr, len);
 				else
-					sepk_build_test<SHIFT, false, EXCL>(L, s, g, q0, kn, hs + hb, hn, npad);
+					sepk_build_round<SHIFT, false, EXCL, false>(L, s, g, qt, q0, kn, tb, hn,
+										    npad, cap, nbr, len);
 			}
-			__syncthreads();
-			sepk_build_emit(L, kn, hn, par, hs + hb, npad, cap, nbr);
 		}
-		__syncthreads();
-		// the list is padded with the phantom slot to a multiple of SEPK_PAD
-		if (threadIdx.x < hn) {
-			const int i = threadIdx.x;
-			const int len = L.hcnt[par][i];
+		// lane kk of each wave: its home atom's count, length bound and
+		// padding with the phantom slot to a multiple of SEPK_PAD
+		const int i = wid + (SEPK_BT / 64) * lane;
+		if (lane < SEPK_BHOME / (SEPK_BT / 64) && i < hn) {
 			const long t = hs + hb + i;
 			cnt[t] = len < cap ? len : cap;
 			if (len > cap)
@@ -751,6 +823,7 @@ __global__ __launch_bounds__(SEPK_BT) __attribute__((amdgpu_waves_per_eu(EXCL ==
 		return;
 	__shared__ sepk_build_lds L;
 	const bool shift = g.shift_ok && sys->unclean == 0;
+	const sepk_qtest qt = sepk_qtest_of(g, shift);
 	const long per = gridDim.x / SEPK_NXCD;
 	const long xcd = blockIdx.x % SEPK_NXCD, k = blockIdx.x / SEPK_NXCD;
 	const long nitem = ncell * nslice;
@@ -762,11 +835,11 @@ __global__ __launch_bounds__(SEPK_BT) __attribute__((amdgpu_waves_per_eu(EXCL ==
 		const long c = nslice == 1 ? it : (long)((unsigned)it / (unsigned)nslice);
 		const int sl = (int)(it - c * nslice);
 		if (shift)
-			sepk_build_cell<true, EXCL>(L, s, c, sl, nslice, start, g, npad, cap, (int)n,
-						    nbr, cnt, sys, mymax);
+			sepk_build_cell<true, EXCL>(L, s, c, sl, nslice, start, g, qt, npad, cap,
+						    (int)n, nbr, cnt, sys, mymax);
 		else
-			sepk_build_cell<false, EXCL>(L, s, c, sl, nslice, start, g, npad, cap, (int)n,
-						     nbr, cnt, sys, mymax);
+			sepk_build_cell<false, EXCL>(L, s, c, sl, nslice, start, g, qt, npad, cap,
+						     (int)n, nbr, cnt, sys, mymax);
 	}
 	// one wave per cell, SEPK_BT / 64 cells at a time
 	const long cper = (ncell + SEPK_NXCD - 1) / SEPK_NXCD;
