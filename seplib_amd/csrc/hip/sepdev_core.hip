@@ -151,7 +151,6 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 	}
 	rc |= dalloc(&d->partials, d->partials_words);
 	rc |= dalloc(&d->fin_l2, (size_t)SEPK_MAXTASK * SEPK_MAXNV * SEPK_FIN_BLOCKS);
-	rc |= dalloc(&d->fin_count, 1);
 	rc |= dalloc(&d->sys, 1);
 	d->ret_cap = 16;
 	rc |= dalloc(&d->ret, (size_t)d->ret_cap * SEPDEV_RET_WORDS);
@@ -174,7 +173,6 @@ extern "C" sepdev *sepdev_create(long npart, int *err)
 	// build reads no list entries
 	(void)hipMemsetAsync(d->nbr_count, 0, d->npad * sizeof(int), d->st);
 	(void)hipMemsetAsync(d->sys, 0, sizeof(sepdev_sys), d->st);
-	(void)hipMemsetAsync(d->fin_count, 0, sizeof(unsigned), d->st);
 	(void)hipMemsetAsync(d->ret, 0, (size_t)d->ret_cap * SEPDEV_RET_WORDS * sizeof(double), d->st);
 	for (int b = 0; b < 2; b++) {
 		k_init_identity<<<sepk_blocks(d->npad), SEPK_BLOCK, 0, d->st>>>(d->s[b], d->iperm, d->n, d->npad);
@@ -200,7 +198,7 @@ extern "C" void sepdev_destroy(sepdev *d)
 	free_state(&d->s[1]);
 	void *ptrs[] = {d->iperm, d->cell_count, d->cell_start,
 			d->cellid_old, d->order_tmp, d->cell_slot,
-			d->nbr, d->nbr_count, d->partials, d->fin_l2, d->fin_count, d->ring, d->ring_n, d->ret,
+			d->nbr, d->nbr_count, d->partials, d->fin_l2, d->ring, d->ring_n, d->ret,
 			d->scal, d->sys, d->dstage, d->blist, d->alist, d->dlist,
 			d->bcontrib, d->acontrib, d->dcontrib, d->bcsr_off,
 			d->bcsr_idx, d->acsr_off, d->acsr_idx, d->dcsr_off,

@@ -107,7 +107,6 @@ struct sepk_fin {
 	long region_words, region_rows;
 	sepdev_sys *sys;
 	double *l2;       // SEPK_MAXTASK x SEPK_MAXNV x SEPK_FIN_BLOCKS block results
-	unsigned *count;  // arrivals of the current launch; 0 between launches
 };
 
 struct sepk_timer {
@@ -138,7 +137,7 @@ struct sepdev {
 	int list_built;
 	sepdev_grid grid; // grid of the last build
 
-	// reductions: one row of partial results per block, read by k_finish;
+	// reductions: one row of partial results per block, read by k_fin_rows;
 	// SEPK_NREGION regions of region_words = SEPK_MAXNV * region_rows
 	// doubles, each column-major
 	double *partials;
@@ -146,7 +145,6 @@ struct sepdev {
 	sepk_task pending[SEPK_MAXTASK];
 	int npending;
 	double *fin_l2;
-	unsigned *fin_count;
 	double *ring;                // per-step record (SEPDEV_RING_WORDS each)
 	long ring_cap;
 	unsigned long long *ring_n;  // steps recorded since the last read
@@ -341,8 +339,9 @@ __device__ __forceinline__ int4 sepk_quant(const sepk_pos &p, double s0, double 
 // rows, partial[i*rows + row]:
 // columns whose bit is set in maxmask take the maximum, the others the sum,
 // in a fixed order (wave butterfly, then the block's waves in index order).
-// A one-block finisher kernel (k_finish) combines the rows afterwards; the
-// kernel boundary makes every row visible to it, so no fences are needed.
+// The finisher kernels (k_fin_rows, k_fin_apply) combine the rows
+// afterwards; the kernel boundary makes every row visible to them, so no
+// fences are needed.
 template <int NV>
 __device__ __forceinline__ void sepk_block_partial(double (&v)[NV], unsigned maxmask,
 						   double *partial, long row, long rows)

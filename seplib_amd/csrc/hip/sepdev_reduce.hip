@@ -3,7 +3,7 @@
 // bookkeeping of a step, and the deferred-work helpers around it.
 //
 // Kernels leave one row per block in a region of the partials buffer and
-// queue a task; queued tasks run together in one launch of k_finish at the
+// queue a task; queued tasks run together in the finisher's launches at the
 // next flush (each leapfrog, each host read, the end of a graph capture).
 // Every reduction is fixed-order, so repeated runs give identical bits.
 #include <string.h>
@@ -76,14 +76,18 @@ __device__ static void fin_apply(const sepk_task &k, const double *tot, double *
 	}
 }
 
-// SEPK_FIN_BLOCKS workgroups of one wave per queued task (at least two).
-// Wave w of every workgroup takes task w: a contiguous chunk of its rows, one lane per row
-// (SEPK_MAXNV columns loaded at once with clamped addresses, all tasks'
-// loads in flight together), reduced in a fixed order -- lane-strided rows,
-// then a wave butterfly -- to one value per column.  The last workgroup to
-// arrive combines the SEPK_FIN_BLOCKS values of each column in block order
-// (wave w again for task w) and applies the tasks in queue order.  Neither
-// step depends on timing, so repeated runs give identical bits.
+// Two launches.  k_fin_rows: SEPK_FIN_BLOCKS workgroups of one wave per
+// queued task (at least two); wave w of every workgroup takes task w: a
+// contiguous chunk of its rows, one lane per row (SEPK_MAXNV columns loaded
+// at once with clamped addresses, all tasks' loads in flight together),
+// reduced in a fixed order -- lane-strided rows, then a wave butterfly --
+// to one value per column.  k_fin_apply, one workgroup, combines the
+// SEPK_FIN_BLOCKS values of each column in block order (wave w again for
+// task w) and applies the tasks in queue order.  The kernel boundary makes
+// the first launch's values visible to the second, where a last-workgroup
+// handshake in one launch would need a device-scope fence in every
+// workgroup (a write-back of its XCD's L2).  Neither step depends on
+// timing, so repeated runs give identical bits.
 __device__ static double fin_comb(bool mx, double a, double b)
 {
 	return mx ? fmax(a, b) : a + b;
@@ -92,13 +96,8 @@ __device__ static double fin_comb(bool mx, double a, double b)
 static_assert(FIN_THREADS == 64 * SEPK_MAXTASK, "one finisher wave per task");
 static_assert(sizeof(sepdev_sys) <= 64 * sizeof(int), "the second wave copies the scalars");
 
-__global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, sepk_fin f)
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_rows(const double *partials, sepk_fin f)
 {
-	__shared__ double tot[SEPK_MAXTASK][SEPK_MAXNV];
-	__shared__ double sret[SEPK_MAXTASK][SEPDEV_RET_WORDS];
-	__shared__ double salpha[SEPK_MAXTASK];
-	__shared__ sepdev_sys ssys;
-	__shared__ int last;
 	const int lane = threadIdx.x & 63;
 	const int ti = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // this wave's task
 	const int G = gridDim.x;
@@ -131,20 +130,20 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, 
 				f.l2[((size_t)ti * SEPK_MAXNV + i) * G + blockIdx.x] = x;
 		}
 	}
-	// publish this block's values (every wave that stored fences its own
-	// stores), then count the block in
-	__threadfence();
-	__syncthreads();
-	if (threadIdx.x == 0)
-		last = atomicAdd(f.count, 1u) == (unsigned)(G - 1);
-	__syncthreads();
-	if (!last)
-		return;
-	__threadfence();
+}
+
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_apply(sepk_fin f, int G)
+{
+	__shared__ double tot[SEPK_MAXTASK][SEPK_MAXNV];
+	__shared__ double sret[SEPK_MAXTASK][SEPDEV_RET_WORDS];
+	__shared__ double salpha[SEPK_MAXTASK];
+	__shared__ sepdev_sys ssys;
+	const int lane = threadIdx.x & 63;
+	const int ti = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // this wave's task
 	// The tasks read and update the system scalars, their return slots and
 	// thermostat variables: one parallel load of each into LDS (issued with
-	// the level-2 loads), the tasks applied there in queue order, one
-	// parallel store back.
+	// the loads of the row results), the tasks applied there in queue order,
+	// one parallel store back.
 	int rmap[SEPK_MAXTASK], amap[SEPK_MAXTASK]; // first task sharing the pointer
 	for (int u = 0; u < f.ntask; u++) {
 		rmap[u] = amap[u] = -1;
@@ -219,8 +218,6 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finish(const double *partials, 
 		*f.t[threadIdx.x].alpha = salpha[threadIdx.x];
 	if (threadIdx.x >= 64 && threadIdx.x < 64 + nsys)
 		((int *)f.sys)[threadIdx.x - 64] = ((const int *)&ssys)[threadIdx.x - 64];
-	if (threadIdx.x == 0)
-		atomicExch(f.count, 0u);
 }
 
 double *sepk_region(sepdev *d, int region)
@@ -245,12 +242,12 @@ int sepk_flush_tasks(sepdev *d)
 	f.region_rows = d->region_rows;
 	f.sys = d->sys;
 	f.l2 = d->fin_l2;
-	f.count = d->fin_count;
 	d->npending = 0;
 	// one wave per task, at least two (the system scalars are copied by the
 	// second wave)
 	const unsigned threads = 64u * (unsigned)(f.ntask > 2 ? f.ntask : 2);
-	k_finish<<<SEPK_FIN_BLOCKS, threads, 0, d->st>>>(d->partials, f);
+	k_fin_rows<<<SEPK_FIN_BLOCKS, threads, 0, d->st>>>(d->partials, f);
+	k_fin_apply<<<1, threads, 0, d->st>>>(f, SEPK_FIN_BLOCKS);
 	SEPK_LAUNCH_CHECK();
 	return 0;
 }
