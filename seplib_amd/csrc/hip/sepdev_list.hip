@@ -350,10 +350,10 @@ __device__ __forceinline__ void sepk_move_atom(const sepk_state &src, long o,
 	dst.type[t] = ty;
 }
 
-// One wave per cell (grid-stride over cells): rank each member by its
-// original index -- the keys of the cell are held one per lane and compared
-// through wave shuffles -- and move its state to start + rank in the spare
-// copy.
+// One half-wave per cell (grid-stride over cells; the two halves of a wave
+// take two cells): rank each member by its original index -- the keys of
+// the cell are held one per lane and compared through shuffles within the
+// half -- and move its state to start + rank in the spare copy.
 __global__ void k_sort_permute(const int *order, const int *start, sepk_state src,
 			       sepk_state dst, int *iperm, long ncell,
 			       long npad, sepdev_sys *sys, int conditional,
@@ -369,22 +369,23 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 	if (sys->frozen)
 		return;
 	const int pending = sys->xn_pending;
-	const int lane = threadIdx.x & 63;
-	const long waves = (long)gridDim.x * (blockDim.x / 64);
-	for (long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) / 64; c < ncell; c += waves) {
+	constexpr int HW = 32; // lanes of a half-wave
+	const int hl = threadIdx.x % HW;
+	const long halves = (long)gridDim.x * (blockDim.x / HW);
+	for (long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) / HW; c < ncell; c += halves) {
 		const int s = start[c], e = start[c + 1];
-		for (int i0 = s; i0 < e; i0 += 64) {
-			const int i = i0 + lane;
+		for (int i0 = s; i0 < e; i0 += HW) {
+			const int i = i0 + hl;
 			const long o = i < e ? order[i] : 0;
 			const int key = i < e ? src.orig[o] : 0;
 			int rank = 0;
-			for (int b = s; b < e; b += 64) {
-				// keys of members b .. b+63, one per lane
+			for (int b = s; b < e; b += HW) {
+				// keys of members b .. b+31, one per lane of the half
 				const int kj = b == i0 ? key
-						       : (b + lane < e ? src.orig[order[b + lane]] : 0);
-				const int m = min(64, e - b);
+						       : (b + hl < e ? src.orig[order[b + hl]] : 0);
+				const int m = min(HW, e - b);
 				for (int jj = 0; jj < m; jj++)
-					rank += __shfl(kj, jj, 64) < key;
+					rank += __shfl(kj, jj, HW) < key;
 			}
 			if (i >= e)
 				continue;
@@ -976,7 +977,7 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 	unsigned nb = sepk_blocks(d->n);
 	if (nb > SEPK_GRID_CAP)
 		nb = SEPK_GRID_CAP;
-	long sb = (ncell * 64 + SEPK_BLOCK - 1) / SEPK_BLOCK;
+	long sb = (ncell * 32 + SEPK_BLOCK - 1) / SEPK_BLOCK; // a half-wave per cell
 	if (sb > SEPK_GRID_CAP)
 		sb = SEPK_GRID_CAP;
 	k_bin<<<nb, SEPK_BLOCK, 0, d->st>>>(cur.pos, d->n, g, d->cellid_old, d->cell_slot,
