@@ -8,13 +8,14 @@
 //                  (k_scan_scatter: both in one launch, every workgroup
 //                  scanning the counts itself, for grids of up to 16352
 //                  cells)
-//   k_sort_permute one wave per cell: order the segment by original atom
+//   k_sort_permute a half-wave per cell: order the segment by original atom
 //                  index (so the device order and every list are independent
 //                  of atomic timing) and move each member's state to its new
 //                  cell-sorted index in the spare copy
-//   k_build        one workgroup per cell: copy its members' state home,
-//                  stage the stencil cells' segments in LDS and append the
-//                  candidates within the list cutoff, one wave per atom
+//   k_build        one workgroup per cell: stage the stencil cells' segments
+//                  in LDS and append the candidates within the list cutoff
+//                  (each wave every fourth home atom, 64 candidates a round),
+//                  then copy the cells' permuted state home
 // and a BUILD task for the step's finisher.  With conditional = 1 every
 // kernel returns at once unless the device rebuild flag is set, so the
 // sequence can sit in a captured graph.
