@@ -8,7 +8,8 @@
 //                  (k_scan_scatter: both in one launch, every workgroup
 //                  scanning the counts itself, for grids of up to 16352
 //                  cells)
-//   k_sort_permute a half-wave per cell: order the segment by original atom
+//   k_sort_permute a half-wave per cell (a wave for cells of more than 32
+//                  atoms on average): order the segment by original atom
 //                  index (so the device order and every list are independent
 //                  of atomic timing) and move each member's state to its new
 //                  cell-sorted index in the spare copy
@@ -351,10 +352,12 @@ __device__ __forceinline__ void sepk_move_atom(const sepk_state &src, long o,
 	dst.type[t] = ty;
 }
 
-// One half-wave per cell (grid-stride over cells; the two halves of a wave
-// take two cells): rank each member by its original index -- the keys of
-// the cell are held one per lane and compared through shuffles within the
-// half -- and move its state to start + rank in the spare copy.
+// HW lanes per cell (grid-stride over cells; HW = 32: the two halves of a
+// wave take two cells, for cells of up to about 32 atoms; HW = 64: a wave
+// per cell): rank each member by its original index -- the keys of the
+// cell are held one per lane and compared through shuffles within the
+// group -- and move its state to start + rank in the spare copy.
+template <int HW>
 __global__ void k_sort_permute(const int *order, const int *start, sepk_state src,
 			       sepk_state dst, int *iperm, long ncell,
 			       long npad, sepdev_sys *sys, int conditional,
@@ -370,7 +373,7 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 	if (sys->frozen)
 		return;
 	const int pending = sys->xn_pending;
-	constexpr int HW = 32; // lanes of a half-wave
+	static_assert(HW == 32 || HW == 64, "half or whole waves");
 	const int hl = threadIdx.x % HW;
 	const long halves = (long)gridDim.x * (blockDim.x / HW);
 	for (long c = ((long)blockIdx.x * blockDim.x + threadIdx.x) / HW; c < ncell; c += halves) {
@@ -381,7 +384,7 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 			const int key = i < e ? src.orig[o] : 0;
 			int rank = 0;
 			for (int b = s; b < e; b += HW) {
-				// keys of members b .. b+31, one per lane of the half
+				// keys of members b .. b + HW - 1, one per lane of the group
 				const int kj = b == i0 ? key
 						       : (b + hl < e ? src.orig[order[b + hl]] : 0);
 				const int m = min(HW, e - b);
@@ -978,7 +981,9 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 	unsigned nb = sepk_blocks(d->n);
 	if (nb > SEPK_GRID_CAP)
 		nb = SEPK_GRID_CAP;
-	long sb = (ncell * 32 + SEPK_BLOCK - 1) / SEPK_BLOCK; // a half-wave per cell
+	// half a wave per cell when the cells hold 32 atoms or fewer on average
+	const bool half = d->n <= 32 * ncell;
+	long sb = (ncell * (half ? 32 : 64) + SEPK_BLOCK - 1) / SEPK_BLOCK;
 	if (sb > SEPK_GRID_CAP)
 		sb = SEPK_GRID_CAP;
 	k_bin<<<nb, SEPK_BLOCK, 0, d->st>>>(cur.pos, d->n, g, d->cellid_old, d->cell_slot,
@@ -999,7 +1004,7 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 							d->cell_start, d->order_tmp, d->n, d->sys,
 							conditional);
 	}
-	k_sort_permute<<<(unsigned)sb, SEPK_BLOCK, 0, d->st>>>(
+	(half ? k_sort_permute<32> : k_sort_permute<64>)<<<(unsigned)sb, SEPK_BLOCK, 0, d->st>>>(
 		d->order_tmp, d->cell_start, cur, spare, d->iperm, ncell, d->npad,
 		d->sys, conditional, hp, d->cell_count);
 	SEPK_LAUNCH_CHECK();
