@@ -107,6 +107,9 @@ struct sepk_fin {
 	long region_words, region_rows;
 	sepdev_sys *sys;
 	double *l2;       // SEPK_MAXTASK x SEPK_MAXNV x SEPK_FIN_BLOCKS block results
+	// per task: the first task with the same return slot / thermostat
+	// variable (-1: none), so that shared ones are loaded and stored once
+	signed char rmap[SEPK_MAXTASK], amap[SEPK_MAXTASK];
 };
 
 struct sepk_timer {
@@ -348,17 +351,22 @@ __device__ __forceinline__ void sepk_block_partial(double (&v)[NV], unsigned max
 {
 	__shared__ double sh[SEPK_BLOCK / SEPK_WAVE][NV];
 	const int lane = threadIdx.x & (SEPK_WAVE - 1), wid = threadIdx.x / SEPK_WAVE;
+	// butterfly step by step over all columns at once, so that each step's
+	// shuffles are in flight together (one wait a step, not one a column)
 #pragma unroll
-	for (int i = 0; i < NV; i++) {
-		double x = v[i];
-		const bool mx = (maxmask >> i) & 1u;
+	for (int o = SEPK_WAVE / 2; o > 0; o >>= 1) {
+		double y[NV];
 #pragma unroll
-		for (int o = SEPK_WAVE / 2; o > 0; o >>= 1) {
-			const double y = __shfl_xor(x, o, SEPK_WAVE);
-			x = mx ? fmax(x, y) : x + y;
-		}
-		if (lane == 0)
-			sh[wid][i] = x;
+		for (int i = 0; i < NV; i++)
+			y[i] = __shfl_xor(v[i], o, SEPK_WAVE);
+#pragma unroll
+		for (int i = 0; i < NV; i++)
+			v[i] = (maxmask >> i) & 1u ? fmax(v[i], y[i]) : v[i] + y[i];
+	}
+	if (lane == 0) {
+#pragma unroll
+		for (int i = 0; i < NV; i++)
+			sh[wid][i] = v[i];
 	}
 	__syncthreads();
 	if (threadIdx.x < NV) {

@@ -120,15 +120,21 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_rows(const double *partials
 			for (int i = 0; i < SEPK_MAXNV; i++)
 				a[i] = fin_comb((k.maxmask >> i) & 1u, a[i], v[i]);
 		}
+		// butterfly step by step over all columns at once (one wait a step)
 #pragma unroll
-		for (int i = 0; i < SEPK_MAXNV; i++) {
-			const bool mx = (k.maxmask >> i) & 1u;
-			double x = a[i];
-			for (int o = 32; o > 0; o >>= 1)
-				x = fin_comb(mx, x, __shfl_xor(x, o, 64));
-			if (lane == 0 && i < k.nv)
-				f.l2[((size_t)ti * SEPK_MAXNV + i) * G + blockIdx.x] = x;
+		for (int o = 32; o > 0; o >>= 1) {
+			double y[SEPK_MAXNV];
+#pragma unroll
+			for (int i = 0; i < SEPK_MAXNV; i++)
+				y[i] = __shfl_xor(a[i], o, 64);
+#pragma unroll
+			for (int i = 0; i < SEPK_MAXNV; i++)
+				a[i] = fin_comb((k.maxmask >> i) & 1u, a[i], y[i]);
 		}
+#pragma unroll
+		for (int i = 0; i < SEPK_MAXNV; i++)
+			if (lane == 0 && i < k.nv)
+				f.l2[((size_t)ti * SEPK_MAXNV + i) * G + blockIdx.x] = a[i];
 	}
 }
 
@@ -144,16 +150,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_apply(sepk_fin f, int G)
 	// thermostat variables: one parallel load of each into LDS (issued with
 	// the loads of the row results), the tasks applied there in queue order,
 	// one parallel store back.
-	int rmap[SEPK_MAXTASK], amap[SEPK_MAXTASK]; // first task sharing the pointer
-	for (int u = 0; u < f.ntask; u++) {
-		rmap[u] = amap[u] = -1;
-		for (int v = u; v >= 0; v--) {
-			if (f.t[u].ret && f.t[v].ret == f.t[u].ret)
-				rmap[u] = v;
-			if (f.t[u].alpha && f.t[v].alpha == f.t[u].alpha)
-				amap[u] = v;
-		}
-	}
+	const signed char *rmap = f.rmap, *amap = f.amap; // first task sharing the pointer
 	const int nsys = (int)(sizeof(sepdev_sys) / sizeof(int));
 	for (int w = threadIdx.x; w < f.ntask * SEPDEV_RET_WORDS; w += blockDim.x) {
 		const int u = w / SEPDEV_RET_WORDS, word = w % SEPDEV_RET_WORDS;
@@ -173,13 +170,19 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_apply(sepk_fin f, int G)
 			x[i] = lane < G ? col[lane] : 0.0;
 		}
 #pragma unroll
-		for (int i = 0; i < SEPK_MAXNV; i++) {
-			const bool mx = (k.maxmask >> i) & 1u;
-			for (int o = 32; o > 0; o >>= 1)
-				x[i] = fin_comb(mx, x[i], __shfl_xor(x[i], o, 64));
+		for (int o = 32; o > 0; o >>= 1) {
+			double y[SEPK_MAXNV];
+#pragma unroll
+			for (int i = 0; i < SEPK_MAXNV; i++)
+				y[i] = __shfl_xor(x[i], o, 64);
+#pragma unroll
+			for (int i = 0; i < SEPK_MAXNV; i++)
+				x[i] = fin_comb((k.maxmask >> i) & 1u, x[i], y[i]);
+		}
+#pragma unroll
+		for (int i = 0; i < SEPK_MAXNV; i++)
 			if (lane == 0)
 				tot[ti][i] = x[i];
-		}
 	}
 	__syncthreads();
 	// after a device error the step changed nothing, and neither does this
@@ -242,6 +245,15 @@ int sepk_flush_tasks(sepdev *d)
 	f.region_rows = d->region_rows;
 	f.sys = d->sys;
 	f.l2 = d->fin_l2;
+	for (int u = 0; u < f.ntask; u++) {
+		f.rmap[u] = f.amap[u] = -1;
+		for (int v = u; v >= 0; v--) {
+			if (f.t[u].ret && f.t[v].ret == f.t[u].ret)
+				f.rmap[u] = (signed char)v;
+			if (f.t[u].alpha && f.t[v].alpha == f.t[u].alpha)
+				f.amap[u] = (signed char)v;
+		}
+	}
 	d->npending = 0;
 	// one wave per task, at least two (the system scalars are copied by the
 	// second wave)
