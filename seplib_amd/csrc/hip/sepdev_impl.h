@@ -110,6 +110,7 @@ struct sepk_fin {
 	// per task: the first task with the same return slot / thermostat
 	// variable (-1: none), so that shared ones are loaded and stored once
 	signed char rmap[SEPK_MAXTASK], amap[SEPK_MAXTASK];
+	int ring_task;    // the leapfrog task writing the per-step record, or -1
 };
 
 struct sepk_timer {

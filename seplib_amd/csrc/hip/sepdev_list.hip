@@ -796,8 +796,7 @@ __device__ __forceinline__ void sepk_build_cell(
 										    npad, cap, nbr, len);
 			}
 		}
-		// lane kk of each wave: its home atom's count, length bound and
-		// padding with the phantom slot to a multiple of SEPK_PAD
+		// lane kk of each wave: its home atom's count and length bound
 		const int i = wid + (SEPK_BT / 64) * lane;
 		if (lane < SEPK_BHOME / (SEPK_BT / 64) && i < hn) {
 			const long t = hs + hb + i;
@@ -805,9 +804,18 @@ __device__ __forceinline__ void sepk_build_cell(
 			if (len > cap)
 				sepk_set_error(sys, SEP_GPU_ENEIGHB);
 			mymax = max(mymax, len);
-			const int pad = min((len + SEPK_PAD - 1) / SEPK_PAD * SEPK_PAD, cap);
-			for (int e = len; e < pad; e++)
-				nbr[sepk_entry(e, t, npad)] = phantom;
+		}
+		// padding with the phantom slot to a multiple of SEPK_PAD, four
+		// atoms a store: lane 16 a + s takes entry s of the last block of
+		// the a-th atom of the group when that entry lies in [len, pad)
+		const int nk = wid < hn ? (hn - wid + (SEPK_BT / 64) - 1) / (SEPK_BT / 64) : 0;
+		for (int g0 = 0; g0 < nk; g0 += 64 / SEPK_PAD) {
+			const int kk = g0 + lane / SEPK_PAD;
+			const int lk = __shfl(len, kk, 64);
+			const int pad = min((lk + SEPK_PAD - 1) / SEPK_PAD * SEPK_PAD, cap);
+			const int e = lk / SEPK_PAD * SEPK_PAD + lane % SEPK_PAD;
+			if (kk < nk && e >= lk && e < pad)
+				nbr[sepk_entry(e, hs + hb + wid + (SEPK_BT / 64) * kk, npad)] = phantom;
 		}
 		if (hb + SEPK_BHOME < hend)
 			__syncthreads(); // the next batch restages the candidates

@@ -14,32 +14,57 @@
 
 // ret, alpha and sys point at the finisher's LDS copies of the task's
 // return slot, thermostat variable and the system scalars.
-__device__ static void fin_apply(const sepk_task &k, const double *tot, double *ret,
-				 double *alpha, sepdev_sys *sys)
+// Row-major entry m of a symmetric 3x3 tensor stored as its six distinct
+// entries (xx, xy, xz, yy, yz, zz): the index of that entry.
+__device__ static int fin_sym(int m)
 {
-	if (ret && k.zero_first)
-		for (int w = 0; w < SEPDEV_RET_WORDS; w++)
-			ret[w] = 0.0;
+	return (int)((0x542431210ull >> (4 * m)) & 15ull);
+}
+
+// One word w of a task's return slot: what the task adds to (or sets in)
+// that word, given the word's value v before the task.
+__device__ static double fin_word(const sepk_task &k, const double *tot, int w, double v)
+{
+	if (k.zero_first)
+		v = 0.0;
+	const int mp = w - SEPDEV_RET_POTP, mk = w - SEPDEV_RET_KINP, mb = w - SEPDEV_RET_BONDP;
 	switch (k.mode) {
 	case SEPK_FIN_PAIR: {
 		// each pair was seen from both ends: halve energy and stress
 		const double e = 0.5 * tot[0];
-		if (k.epot_assign)
-			ret[SEPDEV_RET_EPOT] = e;
-		else
-			ret[SEPDEV_RET_EPOT] += e;
-		if (k.also_ecoul)
-			ret[SEPDEV_RET_ECOUL] += e;
-		sepk_add_sym9(ret + SEPDEV_RET_POTP, tot + 1, 0.5);
+		if (w == SEPDEV_RET_EPOT)
+			v = k.epot_assign ? e : v + e;
+		if (w == SEPDEV_RET_ECOUL && k.also_ecoul)
+			v += e;
+		if (mp >= 0 && mp < 9)
+			v += 0.5 * tot[1 + fin_sym(mp)];
 		break;
 	}
 	case SEPK_FIN_BONDED:
-		ret[SEPDEV_RET_EPOT] += tot[0];
-		if (k.with_stress) {
-			sepk_add_sym9(ret + SEPDEV_RET_POTP, tot + 1, 1.0);
-			sepk_add_sym9(ret + SEPDEV_RET_BONDP, tot + 1, 1.0);
-		}
+		if (w == SEPDEV_RET_EPOT)
+			v += tot[0];
+		if (k.with_stress && mp >= 0 && mp < 9)
+			v += 1.0 * tot[1 + fin_sym(mp)];
+		if (k.with_stress && mb >= 0 && mb < 9)
+			v += 1.0 * tot[1 + fin_sym(mb)];
 		break;
+	case SEPK_FIN_LF:
+		if (w == SEPDEV_RET_EKIN)
+			v += 0.5 * tot[0];
+		if (mk >= 0 && mk < 9)
+			v += 1.0 * tot[1 + fin_sym(mk)];
+		break;
+	default:
+		break;
+	}
+	return v;
+}
+
+// A task's effect on the system scalars and its thermostat variable.
+__device__ static void fin_sys(const sepk_task &k, const double *tot, double *alpha,
+			       sepdev_sys *sys)
+{
+	switch (k.mode) {
 	case SEPK_FIN_NH: {
 		// nblk == 0: the sum of m v^2 is already in sys->sumv2
 		const double sum = k.nblk > 0 ? tot[0] : sys->sumv2;
@@ -50,8 +75,6 @@ __device__ static void fin_apply(const sepk_task &k, const double *tot, double *
 	case SEPK_FIN_LF: {
 		if (k.nh_fused)
 			*alpha = sepk_nh_alpha(*alpha, sys->sumv2, k.npart, k.Td, k.dt_tau2);
-		ret[SEPDEV_RET_EKIN] += 0.5 * tot[0];
-		sepk_add_sym9(ret + SEPDEV_RET_KINP, tot + 1, 1.0);
 		// source/sepintgr.c:61,72-84: running maximum since sep_reset_force
 		double mx = k.reset_max ? 0.0 : sys->max_dist2;
 		if (tot[7] > mx)
@@ -73,6 +96,8 @@ __device__ static void fin_apply(const sepk_task &k, const double *tot, double *
 		sys->xn_pending = 0; // applied by the permutation
 		sys->unclean = 0;    // judged afresh by the next binning
 		break;
+	default:
+		break;
 	}
 }
 
@@ -83,7 +108,9 @@ __device__ static void fin_apply(const sepk_task &k, const double *tot, double *
 // reduced in a fixed order -- lane-strided rows, then a wave butterfly --
 // to one value per column.  k_fin_apply, one workgroup, combines the
 // SEPK_FIN_BLOCKS values of each column in block order (wave w again for
-// task w) and applies the tasks in queue order.  The kernel boundary makes
+// task w) and applies the tasks in queue order: lane w of the first wave
+// takes word w of every return slot, lane 0 of the second wave the system
+// scalars and thermostat variables (no task's words depend on them).  The kernel boundary makes
 // the first launch's values visible to the second, where a last-workgroup
 // handshake in one launch would need a device-scope fence in every
 // workgroup (a write-back of its XCD's L2).  Neither step depends on
@@ -161,6 +188,8 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_apply(sepk_fin f, int G)
 		salpha[threadIdx.x] = *f.t[threadIdx.x].alpha;
 	if (threadIdx.x >= 64 && threadIdx.x < 64 + nsys)
 		((int *)&ssys)[threadIdx.x - 64] = ((const int *)f.sys)[threadIdx.x - 64];
+	const unsigned long long ring_at =
+		f.ring_task >= 0 && ti == 0 ? *f.t[f.ring_task].ring_n : 0ull;
 	if (ti < f.ntask) {
 		const sepk_task &k = f.t[ti];
 		double x[SEPK_MAXNV];
@@ -187,35 +216,35 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_apply(sepk_fin f, int G)
 	__syncthreads();
 	// after a device error the step changed nothing, and neither does this
 	const bool frozen = ssys.error != 0;
-	if (threadIdx.x == 0 && !frozen)
+	if (threadIdx.x < SEPDEV_RET_WORDS && !frozen) {
+		for (int u = 0; u < f.ntask; u++)
+			if (rmap[u] >= 0)
+				sret[rmap[u]][threadIdx.x] =
+					fin_word(f.t[u], tot[u], threadIdx.x, sret[rmap[u]][threadIdx.x]);
+	} else if (threadIdx.x == 64 && !frozen) {
 		for (int u = 0; u < f.ntask; u++) {
 			const sepk_task &k = f.t[u];
 			// a conditional build that did not run leaves nothing to finish
 			if (k.mode == SEPK_FIN_BUILD && k.conditional && ssys.neighb_flag == 0)
 				continue;
-			fin_apply(k, tot[u], rmap[u] >= 0 ? sret[rmap[u]] : nullptr,
-				  amap[u] >= 0 ? &salpha[amap[u]] : nullptr, &ssys);
+			fin_sys(k, tot[u], amap[u] >= 0 ? &salpha[amap[u]] : nullptr, &ssys);
 		}
+	}
 	__syncthreads();
 	for (int w = threadIdx.x; w < f.ntask * SEPDEV_RET_WORDS; w += blockDim.x) {
 		const int u = w / SEPDEV_RET_WORDS, word = w % SEPDEV_RET_WORDS;
 		if (rmap[u] == u)
 			f.t[u].ret[word] = sret[u][word];
 	}
-	// the per-step record: a leapfrog's return slot and the new tnow
-	for (int u = 0; u < f.ntask && !frozen; u++) {
-		const sepk_task &k = f.t[u];
-		if (k.mode != SEPK_FIN_LF || !k.ring || rmap[u] < 0)
-			continue;
-		const unsigned long long at = *k.ring_n;
-		double *rec = k.ring + (size_t)(at % (unsigned long long)k.ring_cap) * SEPDEV_RING_WORDS;
-		if (threadIdx.x < SEPDEV_RET_WORDS)
-			rec[threadIdx.x] = sret[rmap[u]][threadIdx.x];
-		else if (threadIdx.x == SEPDEV_RET_WORDS)
-			rec[SEPDEV_RET_WORDS] = ssys.tnow;
-		__syncthreads(); // every thread has read the count
+	// the per-step record: the leapfrog's return slot and the new tnow; the
+	// count was read at the start by this same wave, which alone stores it
+	if (f.ring_task >= 0 && !frozen && threadIdx.x <= SEPDEV_RET_WORDS) {
+		const sepk_task &k = f.t[f.ring_task];
+		double *rec = k.ring + (size_t)(ring_at % (unsigned long long)k.ring_cap) * SEPDEV_RING_WORDS;
+		rec[threadIdx.x] = threadIdx.x < SEPDEV_RET_WORDS ? sret[rmap[f.ring_task]][threadIdx.x]
+								  : ssys.tnow;
 		if (threadIdx.x == 0)
-			*k.ring_n = at + 1;
+			*k.ring_n = ring_at + 1;
 	}
 	if (threadIdx.x < f.ntask && amap[threadIdx.x] == (int)threadIdx.x)
 		*f.t[threadIdx.x].alpha = salpha[threadIdx.x];
@@ -254,6 +283,12 @@ int sepk_flush_tasks(sepdev *d)
 				f.amap[u] = (signed char)v;
 		}
 	}
+	// the leapfrog's per-step record (a leapfrog task always ends a flush,
+	// so there is at most one)
+	f.ring_task = -1;
+	for (int u = 0; u < f.ntask; u++)
+		if (f.t[u].mode == SEPK_FIN_LF && f.t[u].ring && f.rmap[u] >= 0)
+			f.ring_task = u;
 	d->npending = 0;
 	// one wave per task, at least two (the system scalars are copied by the
 	// second wave)
