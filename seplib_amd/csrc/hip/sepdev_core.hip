@@ -111,8 +111,9 @@ __global__ void k_init_alt(sepk_pos *pos, int4 *qpos, long n, long npad)
 extern "C" sepdev *sepdev_create(long npart, int *err)
 {
 	*err = 0;
-	if (npart <= 0) {
-		*err = sepk_fail(SEP_GPU_EARG, "sepdev_create: npart = %ld", npart);
+	if (npart <= 0 || npart > SEPDEV_MAXN) {
+		*err = sepk_fail(SEP_GPU_EARG, "sepdev_create: npart = %ld (1 to %ld)", npart,
+				 (long)SEPDEV_MAXN);
 		return nullptr;
 	}
 	sepdev *d = (sepdev *)calloc(1, sizeof(sepdev));

@@ -263,7 +263,7 @@ __device__ __forceinline__ void sepk_lj_mixed_band(
 	const int n = cnt[t];
 	for (int e = sub; e < n; e += SEPK_SUB) {
 		const unsigned j = (unsigned)nbr[sepk_entry(e, t, npad)];
-		const int4 qj = s.qpos[j];
+		const int4 qj = sepk_at(s.qpos, j);
 		if (TYPED) {
 			const signed char tj = s.type[j];
 			if (!((ti == c.t0 && tj == c.t1) || (ti == c.t1 && tj == c.t0)))
@@ -307,18 +307,24 @@ __device__ __forceinline__ void sepk_lj_pairs(
 			bool band = false;
 			const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 			const size_t stride = (size_t)npad * SEPK_SUB;
-			const int4 *lp = sepk_lane_list(nbr, t, sub);
-			for (int b = 0; b < nbat; b++, lp += stride) {
-				const int4 jb = *lp;
+			const int4 *lp = sepk_wave_list(nbr, t, sub);
+			const unsigned ln = threadIdx.x & 63;
+			// batch 0 goes out with the count, and batch b + 1 while batch b
+			// is computed (the list allocation holds a spare batch past
+			// the last, so both stay inside it)
+			int4 jb = sepk_at(lp, ln);
+			for (int b = 0; b < nbat; b++) {
 				const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
 				int4 qj[SEPK_BATCH];
 				signed char tj[SEPK_BATCH];
 #pragma unroll
 				for (int u = 0; u < SEPK_BATCH; u++) {
-					qj[u] = s.qpos[(unsigned)j[u]];
+					qj[u] = sepk_at(s.qpos, (unsigned)j[u]);
 					if (TYPED)
 						tj[u] = s.type[(unsigned)j[u]];
 				}
+				lp += stride;
+				jb = sepk_at(lp, ln);
 #pragma unroll
 				for (int u = 0; u < SEPK_BATCH; u++) {
 					if (TYPED && !((ti == c.t0 && tj[u] == c.t1) ||
@@ -341,10 +347,12 @@ __device__ __forceinline__ void sepk_lj_pairs(
 		pi = s.pos[t];
 		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 		const size_t stride = (size_t)npad * SEPK_SUB;
-		const int4 *lp = sepk_lane_list(nbr, t, sub);
+		const int4 *lp = sepk_wave_list(nbr, t, sub);
+		const unsigned ln = threadIdx.x & 63;
 		const char *pb = (const char *)s.pos;
-		for (int b = 0; b < nbat; b++, lp += stride) {
-			const int4 jb = *lp;
+		// batch b + 1 in flight while batch b is computed, as above
+		int4 jb = sepk_at(lp, ln);
+		for (int b = 0; b < nbat; b++) {
 			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
 			double px[SEPK_BATCH], py[SEPK_BATCH], pz[SEPK_BATCH];
 			signed char tj[SEPK_BATCH];
@@ -357,6 +365,8 @@ __device__ __forceinline__ void sepk_lj_pairs(
 				if (TYPED)
 					tj[u] = s.type[(unsigned)j[u]];
 			}
+			lp += stride;
+			jb = sepk_at(lp, ln);
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {
 				if (TYPED && !((ti == c.t0 && tj[u] == c.t1) ||
@@ -633,7 +643,7 @@ __device__ __forceinline__ void sepk_coul_pairs_mixed(
 		int4 qj[SEPK_BATCH];
 #pragma unroll
 		for (int u = 0; u < SEPK_BATCH; u++)
-			qj[u] = s.qpos[(unsigned)j[u]];
+			qj[u] = sepk_at(s.qpos, (unsigned)j[u]);
 #pragma unroll
 		for (int u = 0; u < SEPK_BATCH; u++) {
 			const float qjf = __int_as_float(qj[u].w);
@@ -781,10 +791,13 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 		// the FP64 loops of sepk_lj_pairs and k_coulomb on 32-byte records
 		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 		const size_t stride = (size_t)npad * SEPK_SUB;
-		const int4 *lp = sepk_lane_list(nbr, t, sub);
+		const int4 *lp = sepk_wave_list(nbr, t, sub);
+		const unsigned ln = threadIdx.x & 63;
 		const char *pb = (const char *)s.pos;
-		for (int b = 0; b < nbat; b++, lp += stride) {
-			const int4 jb = *lp;
+		// batch b + 1 in flight while batch b is computed (the list
+		// allocation holds a spare batch past the last)
+		int4 jb = sepk_at(lp, ln);
+		for (int b = 0; b < nbat; b++) {
 			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
 			sepk_pos pj[SEPK_BATCH];
 			signed char tj[SEPK_BATCH];
@@ -794,6 +807,8 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 				if (TYPED)
 					tj[u] = s.type[(unsigned)j[u]];
 			}
+			lp += stride;
+			jb = sepk_at(lp, ln);
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {
 				double kx, ky, kz;
@@ -828,19 +843,22 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 		const float epsf = (float)DBL_EPSILON;
 		const int nbat = (cnt[t] + SEPK_PAD - 1) / SEPK_PAD;
 		const size_t stride = (size_t)npad * SEPK_SUB;
-		const int4 *lp = sepk_lane_list(nbr, t, sub);
+		const int4 *lp = sepk_wave_list(nbr, t, sub);
+		const unsigned ln = threadIdx.x & 63;
 		bool band = false;
-		for (int b = 0; b < nbat; b++, lp += stride) {
-			const int4 jb = *lp;
+		int4 jb = sepk_at(lp, ln); // batch b + 1 in flight, as above
+		for (int b = 0; b < nbat; b++) {
 			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
 			int4 qj[SEPK_BATCH];
 			signed char tj[SEPK_BATCH];
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {
-				qj[u] = s.qpos[(unsigned)j[u]];
+				qj[u] = sepk_at(s.qpos, (unsigned)j[u]);
 				if (TYPED)
 					tj[u] = s.type[(unsigned)j[u]];
 			}
+			lp += stride;
+			jb = sepk_at(lp, ln);
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {
 				const unsigned ux = (unsigned)qi.x, vx = (unsigned)qj[u].x;
@@ -1321,7 +1339,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 // the pair loop and its rows as in k_lj, then the atom's integrator step
 // from its summed force, with the new positions to o.pos / o.qpos.
 template <bool MIXED, bool FAST>
-__global__ __launch_bounds__(SEPK_BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED || FAST ? 7 : 5))) void k_lj_step(
+__global__ __launch_bounds__(SEPK_BLOCK) __attribute__((amdgpu_waves_per_eu(MIXED ? 7 : FAST ? 6 : 5))) void k_lj_step(
 	sepk_state s, const int *__restrict__ nbr, const int *__restrict__ cnt, long n,
 	long npad, sepk_lj c, const sepdev_sys *sys, int nh, const double *alpha, double Td,
 	double dt_tau2, double dt, double qs0, double qs1, double qs2, sepk_lfout o,

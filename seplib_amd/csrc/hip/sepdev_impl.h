@@ -293,10 +293,32 @@ __device__ __forceinline__ size_t sepk_entry(long e, long t, long npad)
 	       (e / SEPK_SUB) % SEPK_BATCH;
 }
 
+// Element j of a per-atom array addressed by a 32-bit byte offset from the
+// wave-uniform base: one VGPR per address where an index would take two.
+// No per-atom array holds more than SEPDEV_MAXN + 64 records of at most 32
+// bytes (sepdev_create), so the offset fits.
+template <class T>
+__device__ __forceinline__ const T &sepk_at(const T *base, unsigned j)
+{
+	return *(const T *)((const char *)base + j * (unsigned)sizeof(T));
+}
+
 // Batch 0 of lane `sub` of atom t; batch b is at [b * npad * SEPK_SUB].
 __device__ __forceinline__ const int4 *sepk_lane_list(const int *nbr, long t, int sub)
 {
 	return (const int4 *)nbr + (size_t)t * SEPK_SUB + sub;
+}
+
+// The same for a list kernel's wave, whose lanes take atoms t and lanes sub
+// with t * SEPK_SUB + sub = (first of the wave) + (lane in the wave): batch 0
+// of the wave's first lane as a wave-uniform base, which lane l reads at
+// element l (sepk_at); batch b is at [b * npad * SEPK_SUB].
+__device__ __forceinline__ const int4 *sepk_wave_list(const int *nbr, long t, int sub)
+{
+	const long w = (long)t * SEPK_SUB + sub - (long)(threadIdx.x & 63);
+	const long wu = (long)(((unsigned long)(unsigned)__builtin_amdgcn_readfirstlane((int)(w >> 32)) << 32) |
+			       (unsigned)__builtin_amdgcn_readfirstlane((int)w));
+	return (const int4 *)nbr + wu;
 }
 
 // Single-image minimum image with strict comparisons against +-L/2, the

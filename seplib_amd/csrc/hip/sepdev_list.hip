@@ -547,9 +547,12 @@ __device__ __forceinline__ void sepk_build_round(
 	const int q = min(q0 + lane, kn - 1);
 	const int left = kn - q0;
 	const unsigned long long vmask = left >= 64 ? ~0ull : (1ull << left) - 1ull;
-	// lists of fewer than 2^20 padded atoms: 32-bit slot offsets with a
-	// full-rate 24-bit multiply (0: sepk_entry)
-	const unsigned npad16 = npad < (1l << 20) ? (unsigned)npad * SEPK_PAD : 0u;
+	// lists of fewer than 2^20 padded atoms and under 4 GiB: 32-bit byte
+	// offsets with a full-rate 24-bit multiply (0: sepk_entry)
+	const unsigned npad16 =
+		npad < (1l << 20) && (long)((cap + SEPK_PAD - 1) / SEPK_PAD) * npad * SEPK_PAD * 4 < (1l << 32)
+			? (unsigned)npad * SEPK_PAD
+			: 0u;
 	const int j = L.cj[q];
 	const int o = L.co[q];
 	const int rule = L.segrule[o];
@@ -648,11 +651,12 @@ __device__ __forceinline__ void sepk_build_round(
 				(unsigned)(take >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)take, (unsigned)e0));
 			if (__builtin_amdgcn_inverse_ballot_w64(take) && e < cap) {
 				if (npad16 != 0u) {
-					// the atom's first slot as a scalar base, a 32-bit offset
+					// the atom's first slot as a scalar base, a 32-bit byte
+					// offset (sepk_entry)
 					const unsigned off = __umul24((unsigned)e / SEPK_PAD, npad16) +
 							     ((unsigned)e % SEPK_SUB) * SEPK_BATCH +
 							     ((unsigned)e / SEPK_SUB) % SEPK_BATCH;
-					(nbr + (size_t)t * SEPK_PAD)[off] = j;
+					*(int *)((char *)(nbr + (size_t)t * SEPK_PAD) + (size_t)(off * 4u)) = j;
 				} else {
 					nbr[sepk_entry(e, t, npad)] = j;
 				}
@@ -750,9 +754,9 @@ __device__ __forceinline__ void sepk_build_cell(
 #pragma unroll
 			for (int r = 0; r < PER; r++) {
 				if (oq[r] >= 0) {
-					pq[r] = s.pos[jq[r]];
+					pq[r] = sepk_at(s.pos, (unsigned)jq[r]);
 					if (EXCL == 3)
-						mq[r] = s.mol[jq[r]];
+						mq[r] = sepk_at(s.mol, (unsigned)jq[r]);
 				}
 			}
 #pragma unroll
@@ -890,7 +894,9 @@ extern "C" int sepdev_reserve_list(sepdev *d, int cap)
 	if (d->nbr)
 		(void)hipFree(d->nbr);
 	d->nbr = nullptr;
-	SEPK_CHECK(hipMalloc(&d->nbr, (size_t)lcap * SEPK_SUB * d->npad * sizeof(int)));
+	// a spare batch past the last: the pair loops load batch b + 1 while
+	// they compute batch b
+	SEPK_CHECK(hipMalloc(&d->nbr, ((size_t)lcap * SEPK_SUB + SEPK_PAD) * d->npad * sizeof(int)));
 	SEPK_CHECK(hipMemsetAsync(d->nbr_count, 0, d->npad * sizeof(int), d->st));
 	d->lcap = lcap;
 	d->cap = lcap * SEPK_SUB;

@@ -69,7 +69,9 @@ typedef struct {
 	int off[SEPDEV_MAX_OFF][4]; /* dx, dy, dz, rule                     */
 } sepdev_grid;
 
-/* ---- lifetime */
+/* ---- lifetime.  At most SEPDEV_MAXN atoms a device (2^27 - 64, so that
+ * every per-atom record is reached by a 32-bit byte offset). */
+#define SEPDEV_MAXN ((1L << 27) - 64)
 sepdev *sepdev_create(long npart, int *err);
 void sepdev_destroy(sepdev *d);
 const char *sepdev_error_message(void);
