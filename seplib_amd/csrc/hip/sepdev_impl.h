@@ -293,14 +293,20 @@ __device__ __forceinline__ size_t sepk_entry(long e, long t, long npad)
 	       (e / SEPK_SUB) % SEPK_BATCH;
 }
 
-// Element j of a per-atom array addressed by a 32-bit byte offset from the
-// wave-uniform base: one VGPR per address where an index would take two.
-// No per-atom array holds more than SEPDEV_MAXN + 64 records of at most 32
-// bytes (sepdev_create), so the offset fits.
+// Element j of a per-atom array (or of one component block of npad
+// entries) addressed by a 32-bit byte offset from the wave-uniform base:
+// one VGPR per address where an index would take two.  No block holds more
+// than SEPDEV_MAXN + 64 records of at most 32 bytes (sepdev_create), so the
+// offset fits.
 template <class T>
 __device__ __forceinline__ const T &sepk_at(const T *base, unsigned j)
 {
 	return *(const T *)((const char *)base + j * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ T &sepk_at(T *base, unsigned j)
+{
+	return *(T *)((char *)base + j * (unsigned)sizeof(T));
 }
 
 // Batch 0 of lane `sub` of atom t; batch b is at [b * npad * SEPK_SUB].

@@ -292,10 +292,10 @@ __device__ __forceinline__ void sepk_copy_partners(const sepk_state &src, const 
 		int p[8];
 #pragma unroll
 		for (int k = 0; k < 8; k++)
-			p[k] = src.partners[(k0 + k) * npad + o];
+			p[k] = sepk_at(src.partners + (size_t)(k0 + k) * npad, (unsigned)o);
 #pragma unroll
 		for (int k = 0; k < 8; k++)
-			dst.partners[(k0 + k) * npad + t] = p[k];
+			sepk_at(dst.partners + (size_t)(k0 + k) * npad, (unsigned)t) = p[k];
 	}
 }
 
@@ -307,49 +307,55 @@ __device__ __forceinline__ void sepk_move_atom(const sepk_state &src, long o,
 					       const sepk_state &dst, long t, long npad,
 					       bool pending)
 {
-	const sepk_pos p = src.pos[o];
+	const unsigned ou = (unsigned)o, tu = (unsigned)t;
+	const sepk_pos p = sepk_at(src.pos, ou);
 	{
-		const int4 q = src.qpos[o];
-		const double v0 = src.v[o], v1 = src.v[npad + o], v2 = src.v[2 * npad + o];
-		const double f0 = src.f[o], f1 = src.f[npad + o], f2 = src.f[2 * npad + o];
-		const double a0 = src.a[o], a1 = src.a[npad + o], a2 = src.a[2 * npad + o];
-		dst.pos[t] = p;
-		dst.qpos[t] = q;
-		dst.v[t] = v0;
-		dst.v[npad + t] = v1;
-		dst.v[2 * npad + t] = v2;
-		dst.f[t] = f0;
-		dst.f[npad + t] = f1;
-		dst.f[2 * npad + t] = f2;
-		dst.a[t] = a0;
-		dst.a[npad + t] = a1;
-		dst.a[2 * npad + t] = a2;
+		const int4 q = sepk_at(src.qpos, ou);
+		const double v0 = sepk_at(src.v, ou), v1 = sepk_at(src.v + npad, ou);
+		const double v2 = sepk_at(src.v + 2 * npad, ou);
+		const double f0 = sepk_at(src.f, ou), f1 = sepk_at(src.f + npad, ou);
+		const double f2 = sepk_at(src.f + 2 * npad, ou);
+		const double a0 = sepk_at(src.a, ou), a1 = sepk_at(src.a + npad, ou);
+		const double a2 = sepk_at(src.a + 2 * npad, ou);
+		sepk_at(dst.pos, tu) = p;
+		sepk_at(dst.qpos, tu) = q;
+		sepk_at(dst.v, tu) = v0;
+		sepk_at(dst.v + npad, tu) = v1;
+		sepk_at(dst.v + 2 * npad, tu) = v2;
+		sepk_at(dst.f, tu) = f0;
+		sepk_at(dst.f + npad, tu) = f1;
+		sepk_at(dst.f + 2 * npad, tu) = f2;
+		sepk_at(dst.a, tu) = a0;
+		sepk_at(dst.a + npad, tu) = a1;
+		sepk_at(dst.a + 2 * npad, tu) = a2;
 	}
-	double x0 = src.xn[o], x1 = src.xn[npad + o], x2 = src.xn[2 * npad + o];
-	int n0 = src.cn[o], n1 = src.cn[npad + o], n2 = src.cn[2 * npad + o];
-	const int r0 = src.cr[o], r1 = src.cr[npad + o], r2 = src.cr[2 * npad + o];
-	const double m = src.m[o];
-	const int mol = src.mol[o], orig = src.orig[o];
-	const signed char ty = src.type[o];
+	double x0 = sepk_at(src.xn, ou), x1 = sepk_at(src.xn + npad, ou);
+	double x2 = sepk_at(src.xn + 2 * npad, ou);
+	int n0 = sepk_at(src.cn, ou), n1 = sepk_at(src.cn + npad, ou), n2 = sepk_at(src.cn + 2 * npad, ou);
+	const int r0 = sepk_at(src.cr, ou), r1 = sepk_at(src.cr + npad, ou);
+	const int r2 = sepk_at(src.cr + 2 * npad, ou);
+	const double m = sepk_at(src.m, ou);
+	const int mol = sepk_at(src.mol, ou), orig = sepk_at(src.orig, ou);
+	const signed char ty = sepk_at(src.type, ou);
 	if (pending) {
 		x0 = p.x;
 		x1 = p.y;
 		x2 = p.z;
 		n0 = n1 = n2 = 0;
 	}
-	dst.xn[t] = x0;
-	dst.xn[npad + t] = x1;
-	dst.xn[2 * npad + t] = x2;
-	dst.cn[t] = n0;
-	dst.cn[npad + t] = n1;
-	dst.cn[2 * npad + t] = n2;
-	dst.cr[t] = r0;
-	dst.cr[npad + t] = r1;
-	dst.cr[2 * npad + t] = r2;
-	dst.m[t] = m;
-	dst.mol[t] = mol;
-	dst.orig[t] = orig;
-	dst.type[t] = ty;
+	sepk_at(dst.xn, tu) = x0;
+	sepk_at(dst.xn + npad, tu) = x1;
+	sepk_at(dst.xn + 2 * npad, tu) = x2;
+	sepk_at(dst.cn, tu) = n0;
+	sepk_at(dst.cn + npad, tu) = n1;
+	sepk_at(dst.cn + 2 * npad, tu) = n2;
+	sepk_at(dst.cr, tu) = r0;
+	sepk_at(dst.cr + npad, tu) = r1;
+	sepk_at(dst.cr + 2 * npad, tu) = r2;
+	sepk_at(dst.m, tu) = m;
+	sepk_at(dst.mol, tu) = mol;
+	sepk_at(dst.orig, tu) = orig;
+	sepk_at(dst.type, tu) = ty;
 }
 
 // HW lanes per cell (grid-stride over cells; HW = 32: the two halves of a
@@ -380,13 +386,13 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 		const int s = start[c], e = start[c + 1];
 		for (int i0 = s; i0 < e; i0 += HW) {
 			const int i = i0 + hl;
-			const long o = i < e ? order[i] : 0;
-			const int key = i < e ? src.orig[o] : 0;
+			const long o = i < e ? sepk_at(order, (unsigned)i) : 0;
+			const int key = i < e ? sepk_at(src.orig, (unsigned)o) : 0;
 			int rank = 0;
 			for (int b = s; b < e; b += HW) {
 				// keys of members b .. b + HW - 1, one per lane of the group
-				const int kj = b == i0 ? key
-						       : (b + hl < e ? src.orig[order[b + hl]] : 0);
+				const unsigned ob = b + hl < e ? (unsigned)sepk_at(order, (unsigned)(b + hl)) : 0u;
+				const int kj = b == i0 ? key : (b + hl < e ? sepk_at(src.orig, ob) : 0);
 				const int m = min(HW, e - b);
 				for (int jj = 0; jj < m; jj++)
 					rank += __shfl(kj, jj, HW) < key;
@@ -395,7 +401,7 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 				continue;
 			const long t = s + rank;
 			sepk_move_atom(src, o, dst, t, npad, pending);
-			iperm[key] = (int)t;
+			sepk_at(iperm, (unsigned)key) = (int)t;
 			if (has_partners)
 				sepk_copy_partners(src, dst, o, t, npad);
 		}
@@ -776,15 +782,15 @@ __device__ __forceinline__ void sepk_build_cell(
 			}
 			if (k0 == 0)
 				for (int i = threadIdx.x; i < hn; i += SEPK_BT) {
-					const long t = hs + hb + i;
-					const sepk_pos p = s.pos[t];
+					const unsigned t = (unsigned)(hs + hb + i);
+					const sepk_pos p = sepk_at(s.pos, t);
 					L.hx[i] = p.x;
 					L.hy[i] = p.y;
 					L.hz[i] = p.z;
 					if (qnt)
 						L.hq[i] = sepk_quantize(p, qt.scale);
-					L.horig[i] = s.orig[t];
-					L.hmol[i] = EXCL == 3 ? s.mol[t] : -1;
+					L.horig[i] = sepk_at(s.orig, t);
+					L.hmol[i] = EXCL == 3 ? sepk_at(s.mol, t) : -1;
 				}
 			__syncthreads();
 			const long tb = hs + hb;
