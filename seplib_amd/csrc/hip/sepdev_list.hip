@@ -551,8 +551,8 @@ __device__ __forceinline__ void sepk_build_round(
 	const int lane = threadIdx.x & 63;
 	const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); // wave-uniform
 	const int q = min(q0 + lane, kn - 1);
-	const int left = kn - q0;
-	const unsigned long long vmask = left >= 64 ? ~0ull : (1ull << left) - 1ull;
+	// lanes past the staged candidates take bounds that no distance meets
+	const bool valid = lane < kn - q0;
 	// lists of fewer than 2^20 padded atoms and under 4 GiB: 32-bit byte
 	// offsets with a full-rate 24-bit multiply (0: sepk_entry)
 	const unsigned npad16 =
@@ -577,6 +577,13 @@ __device__ __forceinline__ void sepk_build_round(
 		py = L.cy[q];
 		pz = L.cz[q];
 	}
+	int lo2 = valid ? qt.lo2 : INT_MIN, hi2 = valid ? qt.hi2 : -1;
+	double rl2 = valid ? g.rlist2 : -1.0;
+	// the home atom i's own candidate is the one with j - tbase == i
+	int jrel = j - (int)tbase;
+	// held in registers as they are: the compiler would otherwise fold the
+	// selects back into every test of the loop below
+	asm volatile("" : "+v"(lo2), "+v"(hi2), "+v"(rl2), "+v"(jrel));
 	double sx = 0.0, sy = 0.0, sz = 0.0;
 	if (SH && !QNT) {
 		sx = L.segsh[o][0];
@@ -590,8 +597,8 @@ __device__ __forceinline__ void sepk_build_round(
 		unsigned long long take;
 		if (QNT) {
 			const int r2q = sepk_r2q(L.hq[i], cq);
-			take = __ballot(r2q < qt.lo2) & vmask;
-			const unsigned long long near = __ballot(r2q <= qt.hi2) & vmask & ~take;
+			take = __ballot(r2q < lo2);
+			const unsigned long long near = __ballot(r2q <= hi2) & ~take;
 			if (near) {
 				// the reference's FP64 test, with the target cell's shift
 				const double dx = L.hx[i] - L.cx[q] + L.segsh[o][0];
@@ -617,12 +624,12 @@ __device__ __forceinline__ void sepk_build_round(
 				dz = sepk_wrap(L.hz[i] - pz, g.L[2]);
 			}
 			const double r2 = dx * dx + dy * dy + dz * dz;
-			take = __ballot(r2 < g.rlist2) & vmask;
+			take = __ballot(r2 < rl2);
 		}
 		if (SHIFT) {
 			// the 27-cell stencil reaches the home cell only through
 			// offset 0: drop the atom itself
-			take &= ~(__ballot(j == t32) & home);
+			take &= ~__ballot(jrel == i);
 		} else if (take & home) {
 			const int oi = L.horig[i];
 			const bool drop = rule == SEPDEV_RULE_NE ? j == t32
