@@ -488,6 +488,15 @@ struct sepk_build_lds {
 	int hs, h, anyshift;
 };
 
+// Lists of fewer than 2^20 padded atoms and under 4 GiB: a list slot is a
+// 32-bit byte offset from its atom's first slot, formed with a 24-bit
+// multiply (otherwise sepk_entry's 64-bit arithmetic).
+__device__ __forceinline__ bool sepk_np32(long npad, int cap)
+{
+	return npad < (1l << 20) &&
+	       (long)((cap + SEPK_PAD - 1) / SEPK_PAD) * npad * SEPK_PAD * 4 < (1l << 32);
+}
+
 // Bounds of the quantized test; on = the test applies (SHIFT, cubic box).
 struct sepk_qtest {
 	double scale; // 65536 / L
@@ -543,7 +552,7 @@ __device__ __forceinline__ void sepk_writelane(int &v, int val, int lane)
 // QNT: the quantized test (SHIFT holds).  SH: some stencil target of the
 // cell lies across the box (shift adds; the quantized test needs none).
 // EXCL: 0 = SEP_ALL, 2 = SEP_EXCL_BONDED, 3 = SEP_EXCL_SAME_MOL.
-template <bool SHIFT, bool SH, int EXCL, bool QNT>
+template <bool SHIFT, bool SH, int EXCL, bool QNT, bool NP32>
 __device__ __forceinline__ void sepk_build_round(
 	sepk_build_lds &L, const sepk_state &s, const sepdev_grid &g, const sepk_qtest &qt,
 	int q0, int kn, long tbase, int hn, long npad, int cap, int *nbr, int &cnt)
@@ -553,12 +562,8 @@ __device__ __forceinline__ void sepk_build_round(
 	const int q = min(q0 + lane, kn - 1);
 	// lanes past the staged candidates take bounds that no distance meets
 	const bool valid = lane < kn - q0;
-	// lists of fewer than 2^20 padded atoms and under 4 GiB: 32-bit byte
-	// offsets with a full-rate 24-bit multiply (0: sepk_entry)
-	const unsigned npad16 =
-		npad < (1l << 20) && (long)((cap + SEPK_PAD - 1) / SEPK_PAD) * npad * SEPK_PAD * 4 < (1l << 32)
-			? (unsigned)npad * SEPK_PAD
-			: 0u;
+	// NP32 (sepk_np32): 32-bit byte offsets with a full-rate 24-bit multiply
+	const unsigned npad16 = (unsigned)npad * SEPK_PAD;
 	const int j = L.cj[q];
 	const int o = L.co[q];
 	const int rule = L.segrule[o];
@@ -663,7 +668,7 @@ __device__ __forceinline__ void sepk_build_round(
 			const int e = (int)__builtin_amdgcn_mbcnt_hi(
 				(unsigned)(take >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)take, (unsigned)e0));
 			if (__builtin_amdgcn_inverse_ballot_w64(take) && e < cap) {
-				if (npad16 != 0u) {
+				if (NP32) {
 					// the atom's first slot as a scalar base, a 32-bit byte
 					// offset (sepk_entry)
 					const unsigned off = __umul24((unsigned)e / SEPK_PAD, npad16) +
@@ -738,6 +743,7 @@ __device__ __forceinline__ void sepk_build_cell(
 	const int hbeg = nslice == 1 ? 0 : (int)((unsigned)(h * slice) / (unsigned)nslice);
 	const int hend = nslice == 1 ? h : (int)((unsigned)(h * (slice + 1)) / (unsigned)nslice);
 	const bool qnt = SHIFT && qt.on;
+	const bool np32 = sepk_np32(npad, cap);
 	for (int hb = hbeg; hb < hend; hb += SEPK_BHOME) {
 		const int hn = min(hend - hb, SEPK_BHOME);
 		int len = 0; // lane kk: list length of the wave's kk-th home atom
@@ -802,15 +808,18 @@ __device__ __forceinline__ void sepk_build_cell(
 			__syncthreads();
 			const long tb = hs + hb;
 			for (int q0 = 0; q0 < kn; q0 += 64) {
-				if (qnt)
-					sepk_build_round<true, false, EXCL, true>(L, s, g, qt, q0, kn, tb, hn,
-										  npad, cap, nbr, len);
+				if (qnt && np32)
+					sepk_build_round<true, false, EXCL, true, true>(L, s, g, qt, q0, kn, tb,
+											hn, npad, cap, nbr, len);
+				else if (qnt)
+					sepk_build_round<true, false, EXCL, true, false>(L, s, g, qt, q0, kn, tb,
+											 hn, npad, cap, nbr, len);
 				else if (SHIFT && anyshift)
-					sepk_build_round<SHIFT, true, EXCL, false>(L, s, g, qt, q0, kn, tb, hn,
-										   npad, cap, nbr, len);
+					sepk_build_round<SHIFT, true, EXCL, false, false>(L, s, g, qt, q0, kn, tb,
+											  hn, npad, cap, nbr, len);
 				else
-					sepk_build_round<SHIFT, false, EXCL, false>(L, s, g, qt, q0, kn, tb, hn,
-										    npad, cap, nbr, len);
+					sepk_build_round<SHIFT, false, EXCL, false, false>(L, s, g, qt, q0, kn, tb,
+											   hn, npad, cap, nbr, len);
 			}
 		}
 		// lane kk of each wave: its home atom's count and length bound
