@@ -309,9 +309,8 @@ __device__ __forceinline__ void sepk_lj_pairs(
 			const size_t stride = (size_t)npad * SEPK_SUB;
 			const int4 *lp = sepk_wave_list(nbr, t, sub);
 			const unsigned ln = threadIdx.x & 63;
-			// batch 0 goes out with the count, and batch b + 1 while batch b
-			// is computed (the list allocation holds a spare batch past
-			// the last, so both stay inside it)
+			// batch 0 goes out with the count (every list allocation
+			// holds it), and batch b + 1 while batch b is computed
 			int4 jb = sepk_at(lp, ln);
 			for (int b = 0; b < nbat; b++) {
 				const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
@@ -323,7 +322,9 @@ __device__ __forceinline__ void sepk_lj_pairs(
 					if (TYPED)
 						tj[u] = s.type[(unsigned)j[u]];
 				}
-				lp += stride;
+				// the wave's last batch reloads itself (a cache hit)
+				if (__ballot(b + 1 < nbat))
+					lp += stride;
 				jb = sepk_at(lp, ln);
 #pragma unroll
 				for (int u = 0; u < SEPK_BATCH; u++) {
@@ -365,7 +366,9 @@ __device__ __forceinline__ void sepk_lj_pairs(
 				if (TYPED)
 					tj[u] = s.type[(unsigned)j[u]];
 			}
-			lp += stride;
+			// the wave's last batch reloads itself (a cache hit)
+			if (__ballot(b + 1 < nbat))
+				lp += stride;
 			jb = sepk_at(lp, ln);
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {
@@ -794,8 +797,7 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 		const int4 *lp = sepk_wave_list(nbr, t, sub);
 		const unsigned ln = threadIdx.x & 63;
 		const char *pb = (const char *)s.pos;
-		// batch b + 1 in flight while batch b is computed (the list
-		// allocation holds a spare batch past the last)
+		// batch b + 1 in flight while batch b is computed, as above
 		int4 jb = sepk_at(lp, ln);
 		for (int b = 0; b < nbat; b++) {
 			const int j[SEPK_BATCH] = {jb.x, jb.y, jb.z, jb.w};
@@ -807,7 +809,9 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 				if (TYPED)
 					tj[u] = s.type[(unsigned)j[u]];
 			}
-			lp += stride;
+			// the wave's last batch reloads itself (a cache hit)
+			if (__ballot(b + 1 < nbat))
+				lp += stride;
 			jb = sepk_at(lp, ln);
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {
@@ -857,7 +861,9 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_lj_coul(
 				if (TYPED)
 					tj[u] = s.type[(unsigned)j[u]];
 			}
-			lp += stride;
+			// the wave's last batch reloads itself (a cache hit)
+			if (__ballot(b + 1 < nbat))
+				lp += stride;
 			jb = sepk_at(lp, ln);
 #pragma unroll
 			for (int u = 0; u < SEPK_BATCH; u++) {

@@ -916,9 +916,7 @@ extern "C" int sepdev_reserve_list(sepdev *d, int cap)
 	if (d->nbr)
 		(void)hipFree(d->nbr);
 	d->nbr = nullptr;
-	// a spare batch past the last: the pair loops load batch b + 1 while
-	// they compute batch b
-	SEPK_CHECK(hipMalloc(&d->nbr, ((size_t)lcap * SEPK_SUB + SEPK_PAD) * d->npad * sizeof(int)));
+	SEPK_CHECK(hipMalloc(&d->nbr, (size_t)lcap * SEPK_SUB * d->npad * sizeof(int)));
 	SEPK_CHECK(hipMemsetAsync(d->nbr_count, 0, d->npad * sizeof(int), d->st));
 	d->lcap = lcap;
 	d->cap = lcap * SEPK_SUB;
