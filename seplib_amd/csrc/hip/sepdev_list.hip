@@ -408,6 +408,58 @@ __global__ void k_sort_permute(const int *order, const int *start, sepk_state sr
 	}
 }
 
+// The same for cells of more than 64 atoms on average (dense molecular
+// boxes): a workgroup per cell (grid-stride over cells).  The cell's keys
+// go to LDS once, SEPK_SORT_KEYS at a time, and each thread ranks its
+// members against them with broadcast reads, so a cell costs one round
+// trip for its keys and one for its moves where a wave per cell pays one
+// per 64 keys, and four waves work on it instead of one.  Same ranks, so
+// the same permutation as k_sort_permute.
+#define SEPK_SORT_KEYS 1024
+__global__ __launch_bounds__(SEPK_BLOCK) void k_sort_permute_wg(
+	const int *order, const int *start, sepk_state src, sepk_state dst, int *iperm,
+	long ncell, long npad, sepdev_sys *sys, int conditional, int has_partners, int *count)
+{
+	if (conditional && sys->neighb_flag == 0)
+		return;
+	for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < ncell;
+	     c += (long)gridDim.x * blockDim.x)
+		count[c] = 0;
+	if (sys->frozen)
+		return;
+	const int pending = sys->xn_pending;
+	__shared__ int keys[SEPK_SORT_KEYS];
+	for (long c = blockIdx.x; c < ncell; c += gridDim.x) {
+		const int s = start[c], e = start[c + 1];
+		for (int i0 = s; i0 < e; i0 += SEPK_BLOCK) {
+			const int i = i0 + (int)threadIdx.x;
+			const long o = i < e ? sepk_at(order, (unsigned)i) : 0;
+			const int key = i < e ? sepk_at(src.orig, (unsigned)o) : 0;
+			int rank = 0;
+			for (int b = s; b < e; b += SEPK_SORT_KEYS) {
+				const int m = min(SEPK_SORT_KEYS, e - b);
+				__syncthreads(); // the previous keys are no longer read
+				for (int q = threadIdx.x; q < m; q += SEPK_BLOCK) {
+					const int x = b + q; // a thread's own member: its key
+					keys[q] = x == i ? key
+							 : sepk_at(src.orig, (unsigned)sepk_at(order, (unsigned)x));
+				}
+				__syncthreads();
+#pragma unroll 4
+				for (int jj = 0; jj < m; jj++)
+					rank += keys[jj] < key;
+			}
+			if (i >= e)
+				continue;
+			const long t = s + rank;
+			sepk_move_atom(src, o, dst, t, npad, pending);
+			sepk_at(iperm, (unsigned)key) = (int)t;
+			if (has_partners)
+				sepk_copy_partners(src, dst, o, t, npad);
+		}
+	}
+}
+
 // Partner scan of source/sepprfrc.c:703-731: stops at the first -1 in
 // either atom's array.  Partner values are original indices.
 __device__ __forceinline__ int sepk_share(const int *P, long npad, long ti, long tj,
@@ -1015,9 +1067,10 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 	unsigned nb = sepk_blocks(d->n);
 	if (nb > SEPK_GRID_CAP)
 		nb = SEPK_GRID_CAP;
-	// half a wave per cell when the cells hold 32 atoms or fewer on average
-	const bool half = d->n <= 32 * ncell;
-	long sb = (ncell * (half ? 32 : 64) + SEPK_BLOCK - 1) / SEPK_BLOCK;
+	// half a wave per cell when the cells hold 32 atoms or fewer on
+	// average, a wave up to 64, a workgroup beyond
+	const bool half = d->n <= 32 * ncell, wg = d->n > 64 * ncell;
+	long sb = wg ? ncell : (ncell * (half ? 32 : 64) + SEPK_BLOCK - 1) / SEPK_BLOCK;
 	if (sb > SEPK_GRID_CAP)
 		sb = SEPK_GRID_CAP;
 	k_bin<<<nb, SEPK_BLOCK, 0, d->st>>>(cur.pos, d->n, g, d->cellid_old, d->cell_slot,
@@ -1038,7 +1091,7 @@ extern "C" int sepdev_build_list(sepdev *d, const sepdev_grid *gp, int condition
 							d->cell_start, d->order_tmp, d->n, d->sys,
 							conditional);
 	}
-	(half ? k_sort_permute<32> : k_sort_permute<64>)<<<(unsigned)sb, SEPK_BLOCK, 0, d->st>>>(
+	(wg ? k_sort_permute_wg : half ? k_sort_permute<32> : k_sort_permute<64>)<<<(unsigned)sb, SEPK_BLOCK, 0, d->st>>>(
 		d->order_tmp, d->cell_start, cur, spare, d->iperm, ncell, d->npad,
 		d->sys, conditional, hp, d->cell_count);
 	SEPK_LAUNCH_CHECK();
