@@ -1173,75 +1173,12 @@ struct sepk_lfout {
 };
 
 // sep_nosehoover's force term and sep_leapfrog for atom t
-// (source/sepintgr.c:149-168, :42-88) from its force F before the thermostat
-// term and its position p.  acc: m vh^2 summed, the six m vh (x) vh
-// components, d^2 and m v^2.
-__device__ __forceinline__ void sepk_lf_atom(const sepk_state &s, const sepk_lfout &o, long t,
-					     long npad, const double (&F)[3], sepk_pos p, double dt,
-					     double L0, double L1, double L2, int nh, double al,
-					     int pending, double qs0, double qs1, double qs2,
-					     double (&acc)[9])
-{
-	const double L[3] = {L0, L1, L2};
-	double x[3] = {p.x, p.y, p.z};
-	const double m = s.m[t];
-	double vh[3];
-	double d2 = 0.0;
-	for (int k = 0; k < 3; k++) {
-		const long i = k * npad + t;
-		double xn = s.xn[i];
-		int cn = s.cn[i];
-		if (pending) {
-			xn = x[k];
-			cn = 0;
-			o.xn[i] = xn;
-		}
-		double v = s.v[i];
-		double f = F[k];
-		if (nh)
-			f -= al * m * v;
-		const double a = f / m;
-		v += a * dt;
-		x[k] += v * dt;
-		vh[k] = v - 0.5 * a * dt;
-		acc[0] += vh[k] * vh[k] * m;
-		acc[8] += v * v * m;
-		int cr = s.cr[i];
-		if (x[k] > L[k]) {
-			x[k] -= L[k];
-			cn++;
-			cr++;
-		} else if (x[k] < 0.0) {
-			x[k] += L[k];
-			cn--;
-			cr--;
-		}
-		const double dr = (x[k] + cn * L[k]) - xn;
-		d2 += dr * dr;
-		o.f[i] = f;
-		o.a[i] = a;
-		o.v[i] = v;
-		o.cn[i] = cn;
-		o.cr[i] = cr;
-	}
-	p.x = x[0];
-	p.y = x[1];
-	p.z = x[2];
-	o.pos[t] = p;
-	o.qpos[t] = sepk_quant(p, qs0, qs1, qs2);
-	acc[1] = vh[0] * vh[0] * m;
-	acc[2] = vh[0] * vh[1] * m;
-	acc[3] = vh[0] * vh[2] * m;
-	acc[4] = vh[1] * vh[1] * m;
-	acc[5] = vh[1] * vh[2] * m;
-	acc[6] = vh[2] * vh[2] * m;
-	acc[7] = d2;
-}
-
-// sepk_lf_atom spread over lanes 0..2 of the atom's SEPK_SUB lanes, one
-// axis each; lane 0 gathers the other axes' values by shuffles and forms
-// the sums in sepk_lf_atom's order (the same bits).  Every lane of the wave
-// calls it.
+// (source/sepintgr.c:149-168, :42-88) from its force before the thermostat
+// term, spread over lanes 0..2 of the atom's SEPK_SUB lanes, one axis each
+// (lane k is given component k of the force as Fk).  Lane 0 gathers the
+// other axes' values by shuffles and forms the sums in axis order.  acc:
+// m vh^2 summed, the six m vh (x) vh components, d^2 and m v^2.  Every lane
+// of the wave calls it.
 __device__ __forceinline__ void sepk_lf_atom_lanes(const sepk_state &s, const sepk_lfout &o,
 						   long t, long npad, int sub, bool act, double F0,
 						   double F1, double F2, double dt, double L0,
@@ -1331,12 +1268,13 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 	double al = 0.0;
 	if (nh)
 		al = sepk_nh_alpha(*alpha, sys->sumv2, (double)n, Td, dt_tau2);
-	if (t < n && sub == 0) {
-		const double F[3] = {s.f[t], s.f[npad + t], s.f[2 * npad + t]};
-		const sepk_lfout o = {s.v, s.f, s.a, s.xn, s.cn, s.cr, s.pos, s.qpos};
-		sepk_lf_atom(s, o, t, npad, F, s.pos[t], dt, L0, L1, L2, nh, al, pending, qs0, qs1,
-			     qs2, acc);
-	}
+	// one axis a lane, as in the fused step kernel (sepk_lf_atom_lanes), so
+	// the two give the same bits; lane k needs only force component k
+	const bool act = t < n;
+	const double Fk = act && sub < 3 ? s.f[sub * npad + t] : 0.0;
+	const sepk_lfout o = {s.v, s.f, s.a, s.xn, s.cn, s.cr, s.pos, s.qpos};
+	sepk_lf_atom_lanes(s, o, t, npad, sub, act, Fk, Fk, Fk, dt, L0, L1, L2, nh, al, pending,
+			   qs0, qs1, qs2, acc);
 	sepk_block_partial<9>(acc, 1u << 7, partial, lb, rows);
 }
 
