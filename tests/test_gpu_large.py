@@ -160,3 +160,35 @@ def test_lj_1m_wide_list_slots(product, oracle):
     finally:
         ga.close()
         orc.close()
+
+
+def test_dense_cells_workgroup_sort(product, oracle):
+    """30000 atoms at random in a box of 3^3 cells, about 1100 atoms a cell:
+    the cell sort takes a workgroup per cell and ranks the members against
+    the cell's keys in LDS, more than one block of keys (1024) a cell.  A
+    short cutoff (rc 0.7, skin 0.3, cells of edge 3) keeps the lists near
+    170 entries.  The pair set is bit-exact against the oracle's before and
+    after a step that moves the atoms and rebuilds the list."""
+    n = 30000
+    L = 9.0
+    x = W.uniform(21, 3 * n).reshape(n, 3) * L
+    w = dict(x=x, v=np.zeros((n, 3)), m=np.ones(n), q=np.zeros(n),
+             type=np.full(n, b"A"), L=np.array([L, L, L]), mol=np.full(n, -1))
+    kw = dict(cf=0.7, dt=1e-6, skin=0.3, nsubbox=(3, 3, 3), nneighb=320)
+    ga = SepSystem.from_workload(product, w, **kw)
+    orc = SepSystem.from_workload(oracle, w, **kw)
+    try:
+        _pair(ga, orc, types="AA", cf=0.7)
+        npairs = _check_pairs(ga, orc, oracle)
+        assert 80 * n < npairs < 95 * n
+        # move every atom by a fixed random step and rebuild on both sides
+        d = (W.uniform(22, 3 * n).reshape(n, 3) - 0.5) * 0.4
+        w2 = dict(w, x=np.mod(x + d, L))
+        for s in (ga, orc):
+            s.load(w2)
+            s.sys.neighb_flag = 1
+        _pair(ga, orc, types="AA", cf=0.7)
+        _check_pairs(ga, orc, oracle)
+    finally:
+        ga.close()
+        orc.close()
