@@ -18,6 +18,9 @@
  *   - diagnostics used by tests and benchmarks (neighbour pairs, timings).
  *
  * Every function returns SEP_GPU_OK (0) or an error code and never exits.
+ * A system on the device holds at most 2^27 - 64 atoms (every per-atom
+ * record is reached by a 32-bit byte offset); a larger one is refused with
+ * SEP_GPU_EARG when its device state is created.
  */
 #ifndef SEP_MI355X_SEPGPU_H
 #define SEP_MI355X_SEPGPU_H
