@@ -240,3 +240,18 @@ def test_load_round_trip(product, tmp_path):
         assert any(C.cast(s.atoms[i].neighb, C.c_void_p).value != old[i] for i in range(s.n))
     finally:
         s.close()
+
+
+def test_device_atom_bound_is_refused_before_any_device_call():
+    """sepdev_create refuses more than 2^27 - 64 atoms (per-atom records are
+    reached by 32-bit byte offsets), and none, with SEP_GPU_EARG before it
+    touches the device."""
+    lib = C.CDLL(S.PRODUCT_LIB)
+    lib.sepdev_create.restype = C.c_void_p
+    lib.sepdev_create.argtypes = [C.c_long, C.POINTER(C.c_int)]
+    err = C.c_int(0)
+    assert lib.sepdev_create((1 << 27) - 63, C.byref(err)) is None
+    assert err.value == S.SEP_GPU_EARG
+    err.value = 0
+    assert lib.sepdev_create(0, C.byref(err)) is None
+    assert err.value == S.SEP_GPU_EARG
