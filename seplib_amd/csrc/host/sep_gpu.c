@@ -208,6 +208,8 @@ static int run_check(seph_ctx *c)
 	return seph_check_device_error(c);
 }
 
+#define SEPH_GRAPH_STEPS 16 /* steps a replayed graph holds in long runs (even) */
+
 static int run_steps(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op *ops,
 		     int nops, long nsteps, int flags);
 
@@ -255,24 +257,27 @@ static int run_steps(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op 
 				return rc;
 		return 0;
 	}
-	/* Two steps per graph: a fused step writes the new positions into the
-	 * other of two buffers, so consecutive steps alternate between them
-	 * and a replayed pair of steps returns to the captured assignment. */
+	/* An even number of steps per graph: a fused step writes the new
+	 * positions into the other of two buffers, so consecutive steps
+	 * alternate between them and a replayed graph returns to the captured
+	 * assignment.  Each replay costs a few microseconds on the device
+	 * before its first kernel, so long runs take SEPH_GRAPH_STEPS steps a
+	 * replay and run the few left over eagerly; short ones two a replay. */
 	const long rest = nsteps - 1;
-	if (rest >= 2) {
+	const long per = rest >= SEPH_GRAPH_STEPS ? SEPH_GRAPH_STEPS : 2;
+	if (rest >= per) {
 		if ((rc = seph_dev_fail(sepdev_graph_begin(c->dev))))
 			return rc;
-		rc = run_one_step(atoms, sys, ret, ops, nops, flags);
-		if (!rc)
+		for (long s = 0; !rc && s < per; s++)
 			rc = run_one_step(atoms, sys, ret, ops, nops, flags);
 		int rc2 = seph_dev_fail(sepdev_graph_end(c->dev));
 		if (rc || rc2)
 			return rc ? rc : rc2;
 		/* the captured steps were not executed */
-		if ((rc = seph_dev_fail(sepdev_graph_launch(c->dev, rest / 2))))
+		if ((rc = seph_dev_fail(sepdev_graph_launch(c->dev, rest / per))))
 			return rc;
 	}
-	if (rest % 2)
+	for (long s = rest % per; !rc && s > 0; s--)
 		rc = run_one_step(atoms, sys, ret, ops, nops, flags);
 	return rc;
 }
