@@ -125,7 +125,7 @@ struct sepdev_batch {
 	sepb_lj lj;
 	hipGraph_t graph;
 	hipGraphExec_t exec;
-	int graph_mixed;
+	int graph_mixed, graph_steps;
 };
 
 __device__ __forceinline__ int sepb_mod(int a, int n)
@@ -1450,6 +1450,8 @@ static int launch_step(sepdev_batch *b, int mixed, int integrate, const int4 *ti
 	return 0;
 }
 
+#define SEPB_GRAPH_STEPS 8
+
 extern "C" int sepdev_batch_run(sepdev_batch *b, long nsteps, int mixed, int graph)
 {
 	if (nsteps <= 0)
@@ -1461,12 +1463,17 @@ extern "C" int sepdev_batch_run(sepdev_batch *b, long nsteps, int mixed, int gra
 				return rc;
 		return 0;
 	}
-	if (!b->exec || b->graph_mixed != mixed) {
+	// SEPB_GRAPH_STEPS steps a replayed graph in long runs (each replay
+	// costs a few microseconds on the device before its first kernel), the
+	// few left over eagerly; one a replay in short runs
+	const int per = nsteps >= SEPB_GRAPH_STEPS ? SEPB_GRAPH_STEPS : 1;
+	if (!b->exec || b->graph_mixed != mixed || b->graph_steps != per) {
 		drop_graph(b);
 		SEPK_CHECK(hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed));
-		rc = launch_rebuild(b);
-		if (!rc)
-			rc = launch_step(b, mixed, 1, b->tiles, b->ntiles, 1);
+		rc = 0;
+		for (int k = 0; !rc && k < per; k++)
+			if (!(rc = launch_rebuild(b)))
+				rc = launch_step(b, mixed, 1, b->tiles, b->ntiles, 1);
 		hipGraph_t g = nullptr;
 		const hipError_t e = hipStreamEndCapture(b->st, &g);
 		if (rc) {
@@ -1479,9 +1486,13 @@ extern "C" int sepdev_batch_run(sepdev_batch *b, long nsteps, int mixed, int gra
 		b->graph = g;
 		SEPK_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
 		b->graph_mixed = mixed;
+		b->graph_steps = per;
 	}
-	for (long i = 0; i < nsteps; i++)
+	for (long i = 0; i < nsteps / per; i++)
 		SEPK_CHECK(hipGraphLaunch(b->exec, b->st));
+	for (long i = nsteps % per; i > 0; i--)
+		if ((rc = launch_rebuild(b)) || (rc = launch_step(b, mixed, 1, b->tiles, b->ntiles, 1)))
+			return rc;
 	return 0;
 }
 
