@@ -745,9 +745,14 @@ extern "C" int sepdev_timing_get(sepdev *d, const char *name, double *ms, long *
 	return rc;
 }
 
-extern "C" int sepdev_graph_begin(sepdev *d)
+extern "C" int sepdev_graph_begin(sepdev *d, int slot)
 {
-	sepdev_graph_drop(d);
+	if (slot < 0 || slot > 1)
+		return sepk_fail(SEP_GPU_EARG, "graph slot %d", slot);
+	if (slot == 0)
+		sepdev_graph_drop(d);
+	else if (d->exec[1] || d->graph[1])
+		return sepk_fail(SEP_GPU_ESTATE, "graph slot 1 in use");
 	SEPK_CHECK(hipStreamBeginCapture(d->st, hipStreamCaptureModeRelaxed));
 	d->capturing = 1;
 	return 0;
@@ -755,7 +760,7 @@ extern "C" int sepdev_graph_begin(sepdev *d)
 
 extern "C" int sepdev_capturing(sepdev *d) { return d->capturing; }
 
-extern "C" int sepdev_graph_end(sepdev *d)
+extern "C" int sepdev_graph_end(sepdev *d, int slot)
 {
 	// the captured step must leave nothing queued for the host side
 	int rc = sepk_flush_tasks(d);
@@ -766,32 +771,33 @@ extern "C" int sepdev_graph_end(sepdev *d)
 			(void)hipGraphDestroy(g);
 		return rc;
 	}
-	SEPK_CHECK(hipStreamEndCapture(d->st, &d->graph));
-	SEPK_CHECK(hipGraphInstantiate(&d->exec, d->graph, nullptr, nullptr, 0));
+	SEPK_CHECK(hipStreamEndCapture(d->st, &d->graph[slot & 1]));
+	SEPK_CHECK(hipGraphInstantiate(&d->exec[slot & 1], d->graph[slot & 1], nullptr, nullptr, 0));
 	return 0;
 }
 
-extern "C" int sepdev_graph_launch(sepdev *d, long times)
+extern "C" int sepdev_graph_launch(sepdev *d, int slot, long times)
 {
-	if (!d->exec)
-		return sepk_fail(SEP_GPU_ESTATE, "no graph");
+	if (slot < 0 || slot > 1 || !d->exec[slot])
+		return sepk_fail(SEP_GPU_ESTATE, "no graph in slot %d", slot);
 	for (long i = 0; i < times; i++)
-		SEPK_CHECK(hipGraphLaunch(d->exec, d->st));
+		SEPK_CHECK(hipGraphLaunch(d->exec[slot], d->st));
 	return 0;
 }
 
 extern "C" int sepdev_graph_drop(sepdev *d)
 {
-	if (d->exec) {
-		// the replays of the last run may still be queued on the stream: the
-		// executable graph goes only after they are done
+	// the replays of the last run may still be queued on the stream: the
+	// executable graphs go only after they are done
+	if (d->exec[0] || d->exec[1])
 		(void)hipStreamSynchronize(d->st);
-		(void)hipGraphExecDestroy(d->exec);
-		d->exec = nullptr;
-	}
-	if (d->graph) {
-		(void)hipGraphDestroy(d->graph);
-		d->graph = nullptr;
+	for (int k = 0; k < 2; k++) {
+		if (d->exec[k])
+			(void)hipGraphExecDestroy(d->exec[k]);
+		if (d->graph[k])
+			(void)hipGraphDestroy(d->graph[k]);
+		d->exec[k] = nullptr;
+		d->graph[k] = nullptr;
 	}
 	return 0;
 }

@@ -238,8 +238,8 @@ struct sepdev {
 	unsigned fork_used; // bonded families launched on side since fork_ev
 
 	// graph
-	hipGraph_t graph;
-	hipGraphExec_t exec;
+	hipGraph_t graph[2];     // captured step sequences (sepdev_graph_begin slots)
+	hipGraphExec_t exec[2];
 	int capturing;
 
 	int device; // the HIP device current at sepdev_create

@@ -261,23 +261,30 @@ static int run_steps(seppart *atoms, sepsys *sys, sepret *ret, const sep_gpu_op 
 	 * positions into the other of two buffers, so consecutive steps
 	 * alternate between them and a replayed graph returns to the captured
 	 * assignment.  Each replay costs a few microseconds on the device
-	 * before its first kernel, so long runs take SEPH_GRAPH_STEPS steps a
-	 * replay and run the few left over eagerly; short ones two a replay. */
+	 * before its first kernel, so runs of SEPH_GRAPH_STEPS steps or more
+	 * replay that many a launch (slot 0) and the pairs left over two a
+	 * launch (slot 1); both are captured before either is replayed. */
 	const long rest = nsteps - 1;
-	const long per = rest >= SEPH_GRAPH_STEPS ? SEPH_GRAPH_STEPS : 2;
-	if (rest >= per) {
-		if ((rc = seph_dev_fail(sepdev_graph_begin(c->dev))))
+	const long big = rest >= SEPH_GRAPH_STEPS ? rest / SEPH_GRAPH_STEPS : 0;
+	const long pairs = (rest - big * SEPH_GRAPH_STEPS) / 2;
+	const long sizes[2] = {big ? SEPH_GRAPH_STEPS : 2, 2};
+	const long times[2] = {big ? big : pairs, big ? pairs : 0};
+	for (int g = 0; g < 2; g++) {
+		if (!times[g])
+			continue;
+		if ((rc = seph_dev_fail(sepdev_graph_begin(c->dev, g))))
 			return rc;
-		for (long s = 0; !rc && s < per; s++)
+		for (long s = 0; !rc && s < sizes[g]; s++)
 			rc = run_one_step(atoms, sys, ret, ops, nops, flags);
-		int rc2 = seph_dev_fail(sepdev_graph_end(c->dev));
+		int rc2 = seph_dev_fail(sepdev_graph_end(c->dev, g));
 		if (rc || rc2)
 			return rc ? rc : rc2;
-		/* the captured steps were not executed */
-		if ((rc = seph_dev_fail(sepdev_graph_launch(c->dev, rest / per))))
-			return rc;
 	}
-	for (long s = rest % per; !rc && s > 0; s--)
+	/* the captured steps were not executed */
+	for (int g = 0; g < 2; g++)
+		if (times[g] && (rc = seph_dev_fail(sepdev_graph_launch(c->dev, g, times[g]))))
+			return rc;
+	if (rest % 2)
 		rc = run_one_step(atoms, sys, ret, ops, nops, flags);
 	return rc;
 }

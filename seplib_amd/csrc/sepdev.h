@@ -223,11 +223,13 @@ int sepdev_timing_get(sepdev *d, const char *name, double *ms, long *count);
 /* mean duration of the last pair-force configuration over reps back-to-back
  * launches (replaces f with the pair forces; queues no reduction) */
 int sepdev_time_pair(sepdev *d, int reps, double *ms_per_launch);
-int sepdev_graph_begin(sepdev *d);
+/* Two graph slots: capturing into slot 0 drops both graphs first (after
+ * their queued replays), capturing into slot 1 keeps slot 0. */
+int sepdev_graph_begin(sepdev *d, int slot);
 /* 1 between sepdev_graph_begin and sepdev_graph_end */
 int sepdev_capturing(sepdev *d);
-int sepdev_graph_end(sepdev *d);
-int sepdev_graph_launch(sepdev *d, long times);
+int sepdev_graph_end(sepdev *d, int slot);
+int sepdev_graph_launch(sepdev *d, int slot, long times);
 int sepdev_graph_drop(sepdev *d);
 
 /* ---- batches: many independent LJ systems (sep_force_pairs "AA" with
