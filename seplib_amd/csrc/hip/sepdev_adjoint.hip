@@ -289,7 +289,7 @@ extern "C" int sepdev_lj_table(sepdev *d, const sepdev_ljtab_args *a, int ret_sl
 	int rc = fill_table(a, c);
 	int region;
 	if (rc || (rc = sepk_flush_nh(d)) || (rc = sepk_term_region(d, &region)) ||
-	    (rc = sepk_fork_point(d)))
+	    (rc = sepk_fork_point(d)) || (rc = sepk_flush_gathers(d)))
 		return rc;
 	const bool over = d->f_zero_pending != 0;
 	const unsigned nb = sepk_blocks_xcd(d->n * SEPK_SUB);

@@ -104,9 +104,10 @@ extern "C" int sepdev_put_topology(sepdev *d, const unsigned *blist, int nb,
 
 // ---- gather ------------------------------------------------------------------
 
-template <bool OVERWRITE>
-__global__ void k_bonded_gather(sepk_state s, long n, long npad, const double *contrib,
-				const int *off, const int *idx)
+// The owed gathers of sepdev::gpend in order: f (or 0 when the first
+// starts from f = 0) plus each family's contributions of the atom, in CSR
+// order, so the sums are those of one gather launch per family.
+__global__ void k_bonded_gather(sepk_state s, long n, long npad, sepk_gpend p)
 {
 	const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n || *s.err)
@@ -114,22 +115,49 @@ __global__ void k_bonded_gather(sepk_state s, long n, long npad, const double *c
 	const int o = s.orig[t];
 	double f[3];
 	for (int k = 0; k < 3; k++)
-		f[k] = OVERWRITE ? 0.0 : s.f[k * npad + t];
-	for (int e = off[o]; e < off[o + 1]; e++) {
-		const long sl = idx[e];
-		for (int k = 0; k < 3; k++)
-			f[k] += contrib[3 * sl + k];
-	}
+		f[k] = p.over ? 0.0 : s.f[k * npad + t];
+	for (int q = 0; q < p.n; q++)
+		for (int e = p.g[q].off[o]; e < p.g[q].off[o + 1]; e++) {
+			const long sl = p.g[q].idx[e];
+			for (int k = 0; k < 3; k++)
+				f[k] += p.g[q].contrib[3 * sl + k];
+		}
 	for (int k = 0; k < 3; k++)
 		s.f[k * npad + t] = f[k];
 }
 
+int sepk_flush_gathers(sepdev *d)
+{
+	if (d->ngpend == 0)
+		return 0;
+	sepk_gpend p;
+	p.n = d->ngpend;
+	p.over = d->gpend_over;
+	for (int q = 0; q < p.n; q++)
+		p.g[q] = d->gpend[q];
+	d->ngpend = 0;
+	d->gpend_over = 0;
+	k_bonded_gather<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad, p);
+	SEPK_LAUNCH_CHECK();
+	return 0;
+}
+
+// A family's gather is owed, not launched: the next reader of f applies
+// it (the leapfrog inside its own kernel), so a step's bonded families
+// cost no launch of their own.
 static int gather(sepdev *d, const double *contrib, const int *off, const int *idx)
 {
-	const bool over = d->f_zero_pending != 0;
-	auto k = over ? k_bonded_gather<true> : k_bonded_gather<false>;
-	k<<<sepk_blocks(d->n), SEPK_BLOCK, 0, d->st>>>(d->s[0], d->n, d->npad, contrib, off, idx);
-	SEPK_LAUNCH_CHECK();
+	if (d->ngpend == SEPK_MAXGATHER) {
+		int rc = sepk_flush_gathers(d);
+		if (rc)
+			return rc;
+	}
+	if (d->ngpend == 0)
+		d->gpend_over = d->f_zero_pending != 0;
+	d->gpend[d->ngpend].contrib = contrib;
+	d->gpend[d->ngpend].off = off;
+	d->gpend[d->ngpend].idx = idx;
+	d->ngpend++;
 	d->f_zero_pending = 0;
 	return 0;
 }

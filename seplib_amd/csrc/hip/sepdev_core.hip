@@ -354,8 +354,9 @@ extern "C" int sepdev_put_vec(sepdev *d, int which, const double *src3n)
 {
 	if (d->capturing)
 		return sepk_fail(SEP_GPU_ESTATE, "transfer during graph capture");
+	// owed bonded gathers belong to the forces before this upload
 	int rc = sepk_flush_nh(d);
-	if (rc)
+	if (rc || (rc = sepk_flush_gathers(d)))
 		return rc;
 	if (which == SEPDEV_V)
 		d->sumv2_valid = 0;
@@ -749,6 +750,10 @@ extern "C" int sepdev_graph_begin(sepdev *d, int slot)
 {
 	if (slot < 0 || slot > 1)
 		return sepk_fail(SEP_GPU_EARG, "graph slot %d", slot);
+	// gathers owed before the capture are applied now, not in every replay
+	int rc = sepk_flush_gathers(d);
+	if (rc)
+		return rc;
 	if (slot == 0)
 		sepdev_graph_drop(d);
 	else if (d->exec[1] || d->graph[1])
@@ -763,7 +768,9 @@ extern "C" int sepdev_capturing(sepdev *d) { return d->capturing; }
 extern "C" int sepdev_graph_end(sepdev *d, int slot)
 {
 	// the captured step must leave nothing queued for the host side
-	int rc = sepk_flush_tasks(d);
+	int rc = sepk_flush_gathers(d);
+	if (!rc)
+		rc = sepk_flush_tasks(d);
 	d->capturing = 0;
 	if (rc) {
 		hipGraph_t g = nullptr;
@@ -954,6 +961,8 @@ extern "C" int sepdev_checkpoint_read(sepdev *d, void *file)
 	for (int k = 0; k < 3; k++)
 		d->qL[k] = h.qL[k];
 	d->f_zero_pending = 0;
+	d->ngpend = 0; // owed gathers were for the forces just replaced
+	d->gpend_over = 0;
 	d->max_reset_pending = 0;
 	d->nh_pending = 0;
 	d->fork_ready = 0;

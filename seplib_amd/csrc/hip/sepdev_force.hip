@@ -28,7 +28,7 @@ __global__ void k_zero_forces(double *f, long count, const int *err)
 int sepk_materialize_forces(sepdev *d)
 {
 	int rc = sepk_flush_pair(d);
-	if (rc)
+	if (rc || (rc = sepk_flush_gathers(d)))
 		return rc;
 	if (!d->f_zero_pending)
 		return 0;
@@ -42,8 +42,11 @@ int sepk_materialize_forces(sepdev *d)
 extern "C" int sepdev_zero_forces(sepdev *d)
 {
 	// The next force kernel overwrites instead of accumulating; anything
-	// else that reads f first materialises the zeros.
+	// else that reads f first materialises the zeros.  Owed gathers would
+	// be zeroed with the rest: they are dropped.
 	d->f_zero_pending = 1;
+	d->ngpend = 0;
+	d->gpend_over = 0;
 	return 0;
 }
 
@@ -483,6 +486,9 @@ static int launch_lj(sepdev *d, const sepk_lj &c, int typed, int mixed, bool fas
 	// consumes the pending f = 0
 	int rc, region;
 	if ((rc = sepk_term_region(d, &region)) || (rc = sepk_fork_point(d)))
+		return rc;
+	// owed bonded gathers go to f first (the sums keep their order)
+	if ((rc = sepk_flush_gathers(d)))
 		return rc;
 	const bool over = d->f_zero_pending != 0;
 	sepk_lj_kernel k;
@@ -994,7 +1000,7 @@ static int launch_lj_coul(sepdev *d, const sepk_coul &cc, int ret_slot)
 		r1 = SEPK_R_PAIR;
 		r2 = SEPK_R_PAIR + 1;
 	}
-	if ((rc = sepk_fork_point(d)))
+	if ((rc = sepk_fork_point(d)) || (rc = sepk_flush_gathers(d)))
 		return rc;
 	const bool over = d->f_zero_pending != 0;
 	typedef void (*kfn)(sepk_state, const int *, const int *, long, long, sepk_lj, sepk_coul,
@@ -1054,7 +1060,8 @@ extern "C" int sepdev_coulomb(sepdev *d, const sepdev_coul_args *a, int ret_slot
 	    (a->mixed || (fast_image_ok(d, a->L) ? 1 : 0) == (d->pc_fast ? 1 : 0)))
 		return launch_lj_coul(d, c, ret_slot);
 	int region;
-	if ((rc = sepk_term_region(d, &region)) || (rc = sepk_fork_point(d)))
+	if ((rc = sepk_term_region(d, &region)) || (rc = sepk_fork_point(d)) ||
+	    (rc = sepk_flush_gathers(d)))
 		return rc;
 	const bool over = d->f_zero_pending != 0;
 	const bool fast = fast_image_ok(d, a->L);
@@ -1256,7 +1263,7 @@ __device__ __forceinline__ void sepk_lf_atom_lanes(const sepk_state &s, const se
 __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 	sepk_state s, long n, long npad, double dt, double L0, double L1, double L2,
 	const sepdev_sys *sys, int nh, const double *alpha, double Td, double dt_tau2,
-	double *partial, long rows, double qs0, double qs1, double qs2)
+	double *partial, long rows, double qs0, double qs1, double qs2, sepk_gpend gp)
 {
 	if (sys->error)
 		return; // positions are updated in place: nothing to copy
@@ -1269,9 +1276,19 @@ __global__ __launch_bounds__(SEPK_BLOCK) void k_leapfrog(
 	if (nh)
 		al = sepk_nh_alpha(*alpha, sys->sumv2, (double)n, Td, dt_tau2);
 	// one axis a lane, as in the fused step kernel (sepk_lf_atom_lanes), so
-	// the two give the same bits; lane k needs only force component k
+	// the two give the same bits; lane k needs only force component k, to
+	// which it adds the owed bonded gathers as k_bonded_gather would
 	const bool act = t < n;
-	const double Fk = act && sub < 3 ? s.f[sub * npad + t] : 0.0;
+	double Fk = 0.0;
+	if (act && sub < 3) {
+		Fk = gp.over ? 0.0 : s.f[sub * npad + t];
+		if (gp.n) {
+			const int o = s.orig[t];
+			for (int q = 0; q < gp.n; q++)
+				for (int e = gp.g[q].off[o]; e < gp.g[q].off[o + 1]; e++)
+					Fk += gp.g[q].contrib[3 * (long)gp.g[q].idx[e] + sub];
+		}
+	}
 	const sepk_lfout o = {s.v, s.f, s.a, s.xn, s.cn, s.cr, s.pos, s.qpos};
 	sepk_lf_atom_lanes(s, o, t, npad, sub, act, Fk, Fk, Fk, dt, L0, L1, L2, nh, al, pending,
 			   qs0, qs1, qs2, acc);
@@ -1437,8 +1454,19 @@ extern "C" int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *
 	d->fork_ready = 0; // the atoms move
 	if (d->fused_pending)
 		return sepdev_fused_step(d, dt, skin, L, ret_slot);
-	int rc = sepk_materialize_forces(d);
+	// a recorded pair call goes first (it applies any owed gathers before
+	// itself); the gathers owed after it are folded into the kernel
+	int rc = sepk_flush_pair(d);
 	if (rc)
+		return rc;
+	sepk_gpend gp;
+	gp.n = d->ngpend;
+	gp.over = d->gpend_over;
+	for (int q = 0; q < gp.n; q++)
+		gp.g[q] = d->gpend[q];
+	d->ngpend = 0;
+	d->gpend_over = 0;
+	if ((rc = sepk_materialize_forces(d)))
 		return rc;
 	if ((rc = sepk_claim_region(d, SEPK_R_LF)))
 		return rc;
@@ -1450,7 +1478,7 @@ extern "C" int sepdev_leapfrog(sepdev *d, double dt, double skin, const double *
 						  d->sys, nh, alpha, d->nh_Td, d->nh_dt_tau2,
 						  sepk_region(d, SEPK_R_LF), d->region_rows,
 						  SEPK_QSCALE / L[0], SEPK_QSCALE / L[1],
-						  SEPK_QSCALE / L[2]);
+						  SEPK_QSCALE / L[2], gp);
 	SEPK_LAUNCH_CHECK();
 	for (int k = 0; k < 3; k++)
 		d->qL[k] = L[k];
@@ -1533,8 +1561,9 @@ extern "C" int sepdev_time_pair(sepdev *d, int reps, double *ms_per_launch)
 	if (reps < 1 || d->capturing)
 		return sepk_fail(SEP_GPU_EARG, "time_pair: reps %d%s", reps,
 				 d->capturing ? " during capture" : "");
+	// owed bonded gathers go to f before the timed launches replace it
 	int rc = sepk_flush_nh(d);
-	if (rc || (rc = sepk_flush_tasks(d)))
+	if (rc || (rc = sepk_flush_gathers(d)) || (rc = sepk_flush_tasks(d)))
 		return rc;
 	sepk_lj c;
 	memcpy(&c, d->last_fused ? (const void *)&d->fused_lj : d->last_lj, sizeof c);

@@ -101,6 +101,18 @@ struct sepk_task {
 	unsigned long long *ring_n;
 };
 
+// One bonded term family's per-atom gather: atom o (original numbering)
+// adds contrib[3 idx[e] + k] for e in [off[o], off[o + 1]) to its force.
+#define SEPK_MAXGATHER 3
+struct sepk_gsrc {
+	const double *contrib;
+	const int *off, *idx;
+};
+struct sepk_gpend {
+	sepk_gsrc g[SEPK_MAXGATHER];
+	int n, over;
+};
+
 struct sepk_fin {
 	sepk_task t[SEPK_MAXTASK];
 	int ntask;
@@ -167,6 +179,10 @@ struct sepdev {
 	// Deferred work, decided on the host so that a captured step replays
 	// the same kernels every time:
 	int f_zero_pending;          // f = 0 owed (the next force kernel overwrites)
+	// bonded per-atom gathers owed to f, applied in order by the next reader
+	// of f (the leapfrog folds them in; sepk_flush_gathers otherwise)
+	sepk_gsrc gpend[SEPK_MAXGATHER];
+	int ngpend, gpend_over;      // gpend_over: the first starts from f = 0
 	int max_reset_pending;       // sys->max_dist2 = 0 owed (the next leapfrog starts at 0)
 	unsigned char *ret_zero;     // per return slot: zero owed before the next write
 	int nh_pending;              // a Nose-Hoover call waits to be fused into leapfrog
@@ -457,6 +473,8 @@ __device__ __forceinline__ void sepk_add_sym9(double *dst9, const double *w6,
 
 // deferred-work helpers (sepdev_force.hip, sepdev_reduce.hip)
 int sepk_materialize_forces(sepdev *d);
+// Launches the owed bonded gathers (one kernel for all of them).
+int sepk_flush_gathers(sepdev *d);
 // launches a pair call recorded for a fused step, if any (sepdev_force.hip)
 int sepk_flush_pair(sepdev *d);
 int sepk_flush_nh(sepdev *d);
