@@ -131,3 +131,42 @@ def test_water_96k_mixed_against_fp64(product, water):
     assert abs(r32[-1][0] - r64[-1][0]) <= 1e-5 * abs(r64[-1][0])
     assert abs(r32[-1][2] - r64[-1][2]) <= 1e-5 * abs(r64[-1][2])
     assert tensor_error(r32[-1][3:12], r64[-1][3:12]) <= 1e-5
+
+
+def test_owed_bonded_gathers_reach_every_force_reader(product):
+    """The bonded families' per-atom gathers are owed to the next reader of
+    the forces.  On a resident system: bond and angle calls and then (a) a
+    host read of the forces, (b) a pair call adding to them, (c) a
+    leapfrog (which folds the gathers into its kernel) give the same bits
+    as the synchronised system, which reads the forces after every call."""
+    # 900 molecules: a box of 3 cells an axis, lists within NNEIGHB
+    w = W.water(900, 3.16, TEMP, seed=7, l0=CFG["lbond"], theta0=CFG["angle"])
+    n = len(w["m"])
+    w["x"] = np.mod(w["x"] + (W.uniform(43, 3 * n).reshape(n, 3) - 0.5) * 0.01, w["L"])
+    p = CFG
+    bonded = [lambda s: s.stretch_harmonic(0, p["lbond"], p["kspring"]),
+              lambda s: s.angle_harmonic(0, p["angle"], p["kangle"])]
+    pair = lambda s: s.force_pairs("OO", 2.5, "sep_lj_shift", S.SEP_EXCL_SAME_MOL)
+    res, syn = _system(product, w), _system(product, w)
+    try:
+        res.gpu_resident(True)
+        for tail in ((), (pair,), ("leapfrog",)):
+            out = []
+            for s in (res, syn):
+                s.reset_retval()
+                s.reset_force()
+                for call in bonded:
+                    call(s)
+                for call in tail:
+                    if call == "leapfrog":
+                        s.leapfrog()
+                    else:
+                        call(s)
+                if s is res:
+                    s.gpu_sync()
+                out.append((np.array(s.view["f"]), np.array(s.view["x"])))
+            assert np.array_equal(out[0][0], out[1][0]), tail
+            assert np.array_equal(out[0][1], out[1][1]), tail
+    finally:
+        res.close()
+        syn.close()
