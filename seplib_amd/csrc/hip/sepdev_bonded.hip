@@ -162,6 +162,16 @@ static int gather(sepdev *d, const double *contrib, const int *off, const int *i
 	return 0;
 }
 
+// A second call of a family in one step rewrites the family's per-term
+// slots: a gather still owed from the first goes to f before that.
+static int settle(sepdev *d, const double *contrib)
+{
+	for (int q = 0; q < d->ngpend; q++)
+		if (d->gpend[q].contrib == contrib)
+			return sepk_flush_gathers(d);
+	return 0;
+}
+
 static int bonded_finish(sepdev *d, unsigned nb, int ret_slot, int with_stress, int region)
 {
 	sepk_task f;
@@ -228,7 +238,8 @@ extern "C" int sepdev_bond(sepdev *d, int type, double lbond, double ks,
 	int region;
 	hipStream_t st;
 	int rc = sepk_flush_nh(d);
-	if (rc || (rc = sepk_term_region(d, &region)) || (rc = sepk_term_stream(d, 1, &st)))
+	if (rc || (rc = settle(d, d->bcontrib)) || (rc = sepk_term_region(d, &region)) ||
+	    (rc = sepk_term_stream(d, 1, &st)))
 		return rc;
 	const unsigned nb = sepk_blocks(d->nb);
 	k_bond<<<nb, SEPK_BLOCK, 0, st>>>(d->s[0], d->iperm, d->blist, d->nb, type, lbond, ks,
@@ -308,7 +319,8 @@ extern "C" int sepdev_angle(sepdev *d, int type, double angle0, double k, int co
 	int region;
 	hipStream_t st;
 	int rc = sepk_flush_nh(d);
-	if (rc || (rc = sepk_term_region(d, &region)) || (rc = sepk_term_stream(d, 2, &st)))
+	if (rc || (rc = settle(d, d->acontrib)) || (rc = sepk_term_region(d, &region)) ||
+	    (rc = sepk_term_stream(d, 2, &st)))
 		return rc;
 	const unsigned nb = sepk_blocks(d->na);
 	k_angle<<<nb, SEPK_BLOCK, 0, st>>>(d->s[0], d->iperm, d->alist, d->na, type, angle0, k,
@@ -390,7 +402,8 @@ extern "C" int sepdev_torsion(sepdev *d, int type, const double *g, const double
 	int region;
 	hipStream_t st;
 	int rc = sepk_flush_nh(d);
-	if (rc || (rc = sepk_term_region(d, &region)) || (rc = sepk_term_stream(d, 4, &st)))
+	if (rc || (rc = settle(d, d->dcontrib)) || (rc = sepk_term_region(d, &region)) ||
+	    (rc = sepk_term_stream(d, 4, &st)))
 		return rc;
 	const unsigned nb = sepk_blocks(d->nd);
 	k_torsion<<<nb, SEPK_BLOCK, 0, st>>>(d->s[0], d->iperm, d->dlist, d->nd, type, gg, L[0],

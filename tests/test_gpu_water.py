@@ -138,7 +138,10 @@ def test_owed_bonded_gathers_reach_every_force_reader(product):
     the forces.  On a resident system: bond and angle calls and then (a) a
     host read of the forces, (b) a pair call adding to them, (c) a
     leapfrog (which folds the gathers into its kernel) give the same bits
-    as the synchronised system, which reads the forces after every call."""
+    as the synchronised system, which reads the forces after every call.
+    The same with each family called twice: the second call rewrites the
+    family's per-term slots, so the first call's owed gather must be
+    applied before it."""
     # 900 molecules: a box of 3 cells an axis, lists within NNEIGHB
     w = W.water(900, 3.16, TEMP, seed=7, l0=CFG["lbond"], theta0=CFG["angle"])
     n = len(w["m"])
@@ -150,12 +153,13 @@ def test_owed_bonded_gathers_reach_every_force_reader(product):
     res, syn = _system(product, w), _system(product, w)
     try:
         res.gpu_resident(True)
-        for tail in ((), (pair,), ("leapfrog",)):
+        for calls, tail in [(c, t) for c in (bonded, bonded + bonded)
+                            for t in ((), (pair,), ("leapfrog",))]:
             out = []
             for s in (res, syn):
                 s.reset_retval()
                 s.reset_force()
-                for call in bonded:
+                for call in calls:
                     call(s)
                 for call in tail:
                     if call == "leapfrog":
@@ -165,8 +169,8 @@ def test_owed_bonded_gathers_reach_every_force_reader(product):
                 if s is res:
                     s.gpu_sync()
                 out.append((np.array(s.view["f"]), np.array(s.view["x"])))
-            assert np.array_equal(out[0][0], out[1][0]), tail
-            assert np.array_equal(out[0][1], out[1][1]), tail
+            assert np.array_equal(out[0][0], out[1][0]), (len(calls), tail)
+            assert np.array_equal(out[0][1], out[1][1]), (len(calls), tail)
     finally:
         res.close()
         syn.close()
